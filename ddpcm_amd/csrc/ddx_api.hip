@@ -1,0 +1,502 @@
+// C-ABI of libddx: the reference module's procedures on device-resident state.
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <vector>
+
+#include "ddx_device.hpp"
+#include "ddx_kernels.hpp"
+#include "tables.hpp"
+
+namespace ddx {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+namespace {
+
+template <typename T>
+int upload(DevArray<T> &d, const T *src, size_t n, hipStream_t s) {
+  DDX_TRY(d.reserve(n));
+  if (n) DDX_HIP(hipMemcpyAsync(d.p, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+  return DDX_OK;
+}
+
+int download(double *dst, const double *src, size_t n, hipStream_t s) {
+  if (n) DDX_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  return DDX_OK;
+}
+
+int sync(ddx_handle *h) {
+  DDX_HIP(hipStreamSynchronize(h->stream));
+  return DDX_OK;
+}
+
+double tol_of(int iconv) {
+  double p = 1.0;
+  for (int k = 0; k < iconv; ++k) p *= 10.0;
+  return 1.0 / p;   // 10^-iconv (cosmo.f90:80)
+}
+
+int set_geometry(ddx_handle *h, const double *x, const double *y, const double *z,
+                 const double *r) {
+  std::vector<double4> sph(h->n);
+  for (int i = 0; i < h->n; ++i) {
+    if (!(r[i] > 0.0)) {
+      set_error("radius of sphere %d is not positive", i + 1);
+      return DDX_ERR_ARG;
+    }
+    sph[i] = make_double4(x[i], y[i], z[i], r[i]);
+  }
+  h->h_x.assign(x, x + h->n);
+  h->h_y.assign(y, y + h->n);
+  h->h_z.assign(z, z + h->n);
+  h->h_r.assign(r, r + h->n);
+  DDX_TRY(upload(h->sph, sph.data(), h->n, h->stream));
+  return sync(h);
+}
+
+// Direct solve L X = g from the rhs in h->rhs; X in h->sigma; E in energy[0].
+int run_direct(ddx_handle *h, const double *d_psi) {
+  int nit = 0, ok = 0;
+  DDX_TRY(device_ldm1x(h, h->rhs.p, h->sigma.p));   // guess D^-1 rhs (cosmo.f90:136-138)
+  DDX_TRY(solve_jacobi_diis(h, 0, kDiisMax, tol_of(h->iconv), h->rhs.p, h->sigma.p, kMaxIter,
+                            &nit, &ok, &h->hist_direct));
+  h->stats.niter_direct = nit;
+  DDX_TRY(energy_dot(h, h->sigma.p, d_psi, 0.5 * ((h->eps - 1.0) / h->eps), h->energy.p));
+  h->have_sigma = true;
+  if (!ok) {
+    set_error("ddCOSMO did not converge");
+    return DDX_ERR_NOCONV;
+  }
+  return DDX_OK;
+}
+
+int run_adjoint(ddx_handle *h, const double *d_psi) {
+  int nit = 0, ok = 0;
+  DDX_TRY(device_ldm1x(h, d_psi, h->s.p));   // guess D^-1 psi (cosmo.f90:174-176)
+  DDX_TRY(solve_jacobi_diis(h, 1, kDiisMax, tol_of(h->iconv), d_psi, h->s.p, kMaxIter, &nit, &ok,
+                            &h->hist_adjoint));
+  h->stats.niter_adjoint = nit;
+  h->have_s = true;
+  if (!ok) {
+    set_error("adjoint ddCOSMO did not converge");
+    return DDX_ERR_NOCONV;
+  }
+  return DDX_OK;
+}
+
+int reserve_vectors(ddx_handle *h) {
+  size_t nn = (size_t)h->nylm * h->n;
+  DDX_TRY(h->rhs.reserve(nn));
+  DDX_TRY(h->sigma.reserve(nn));
+  DDX_TRY(h->s.reserve(nn));
+  DDX_TRY(h->psi.reserve(nn));
+  DDX_TRY(h->x.reserve(nn));
+  DDX_TRY(h->fx.reserve(3 * (size_t)h->n + 3));
+  DDX_TRY(h->energy.reserve(4));
+  return DDX_OK;
+}
+
+// The whole E (+F) pipeline on device state: setup must be current and the
+// charges uploaded.  main.f90:178-265.
+int pipeline(ddx_handle *h) {
+  hipStream_t s = h->stream;
+  DDX_TRY(reserve_vectors(h));
+  DDX_TRY(h->phi.reserve((size_t)h->ncav + 1));
+  DDX_TRY(h->zeta.reserve((size_t)h->ncav + 1));
+  DDX_HIP(hipEventRecord(h->ev[1], s));
+  DDX_TRY(coulomb_potential(h, h->n, h->charge.p, h->sph.p, h->ncav, h->ccav.p, h->phi.p));
+  DDX_TRY(make_psi(h, h->charge.p, h->psi.p));
+  DDX_TRY(build_rhs(h, h->phi.p, h->rhs.p));
+  DDX_HIP(hipEventRecord(h->ev[2], s));
+  DDX_TRY(run_direct(h, h->psi.p));
+  DDX_HIP(hipEventRecord(h->ev[3], s));
+  if (h->igrad) {
+    DDX_TRY(run_adjoint(h, h->psi.p));
+    DDX_HIP(hipEventRecord(h->ev[4], s));
+    DDX_TRY(forces_dd(h, h->phi.p, h->sigma.p, h->s.p, h->fx.p));
+    DDX_TRY(make_zeta(h, h->s.p, h->zeta.p));
+    DDX_TRY(field_forces(h, h->charge.p, h->zeta.p, h->fx.p));
+    DDX_HIP(hipEventRecord(h->ev[5], s));
+  }
+  DDX_TRY(sync(h));
+  float ms = 0.f;
+  DDX_HIP(hipEventElapsedTime(&ms, h->ev[1], h->ev[2]));
+  h->stats.ms_rhs = ms;
+  DDX_HIP(hipEventElapsedTime(&ms, h->ev[2], h->ev[3]));
+  h->stats.ms_direct = ms;
+  if (h->igrad) {
+    DDX_HIP(hipEventElapsedTime(&ms, h->ev[3], h->ev[4]));
+    h->stats.ms_adjoint = ms;
+    DDX_HIP(hipEventElapsedTime(&ms, h->ev[4], h->ev[5]));
+    h->stats.ms_forces = ms;
+  }
+  return DDX_OK;
+}
+
+int timed_setup(ddx_handle *h) {
+  DDX_HIP(hipEventRecord(h->ev[0], h->stream));
+  DDX_TRY(run_setup(h));
+  DDX_HIP(hipEventRecord(h->ev[6], h->stream));
+  DDX_TRY(sync(h));
+  float ms = 0.f;
+  DDX_HIP(hipEventElapsedTime(&ms, h->ev[0], h->ev[6]));
+  h->stats.ms_setup = ms;
+  return DDX_OK;
+}
+
+}  // namespace
+}  // namespace ddx
+
+using namespace ddx;
+
+extern "C" {
+
+const char *ddx_last_error(void) { return g_err; }
+
+int ddx_init(ddx_handle **out, int n, const double *x, const double *y, const double *z,
+             const double *rvdw, int iprint, int lmax, int ngrid, int iconv, int igrad,
+             double eps, double eta) {
+  if (!out || n < 1 || !x || !y || !z || !rvdw) {
+    set_error("ddx_init: invalid arguments");
+    return DDX_ERR_ARG;
+  }
+  if (lmax < 0 || lmax > kMaxLmax) {
+    set_error("lmax %d not supported (0..%d)", lmax, kMaxLmax);
+    return DDX_ERR_ARG;
+  }
+  if (!(eta > 0.0 && eta <= 1.0) || !(eps > 0.0) || iconv < 0 || iconv > 15) {
+    set_error("ddx_init: eps, eta or iconv out of range");
+    return DDX_ERR_ARG;
+  }
+  ddx_handle *h = new ddx_handle();
+  h->n = n;
+  h->lmax = lmax;
+  h->nylm = (lmax + 1) * (lmax + 1);
+  h->ngrid = snap_ngrid(ngrid);
+  h->iconv = iconv;
+  h->igrad = igrad != 0;
+  h->iprint = iprint;
+  h->eps = eps;
+  h->eta = eta;
+  int rc = DDX_OK;
+  auto fail = [&](int code) {
+    ddx_free(h);
+    return code;
+  };
+  if (hipStreamCreate(&h->stream) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    return fail(DDX_ERR_HIP);
+  }
+  for (auto &e : h->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      set_error("hipEventCreate failed");
+      return fail(DDX_ERR_HIP);
+    }
+  if (hipHostMalloc((void **)&h->h_diis, sizeof(DiisState), hipHostMallocDefault) != hipSuccess) {
+    set_error("hipHostMalloc failed");
+    return fail(DDX_ERR_ALLOC);
+  }
+  // one-time tables (ddcosmo.f90:225-267)
+  if (lebedev_grid(h->ngrid, h->h_grid, h->h_w) != 0) {
+    set_error("no Lebedev grid of %d points", h->ngrid);
+    return fail(DDX_ERR_ARG);
+  }
+  harmonic_factors(lmax, h->h_facl, h->h_facs);
+  int nylm = h->nylm, ng = h->ngrid;
+  h->h_basis.assign((size_t)nylm * ng, 0.0);
+  for (int i = 0; i < ng; ++i)
+    real_harmonics(lmax, h->h_facs.data(), &h->h_grid[3 * i], &h->h_basis[(size_t)nylm * i]);
+  std::vector<double> yw((size_t)nylm * ng);
+  for (int i = 0; i < ng; ++i)
+    for (int k = 0; k < nylm; ++k) yw[(size_t)nylm * i + k] = h->h_basis[(size_t)nylm * i + k] * h->h_w[i];
+  std::vector<double4> g4(ng);
+  for (int i = 0; i < ng; ++i)
+    g4[i] = make_double4(h->h_grid[3 * i], h->h_grid[3 * i + 1], h->h_grid[3 * i + 2], h->h_w[i]);
+  // cf = facs / facl, then 1/(l+1) (hsnorm) and 1/facl = 4pi/(2l+1) (intmlp)
+  std::vector<double> cf(3 * (size_t)nylm);
+  for (int l = 0; l <= lmax; ++l)
+    for (int m = -l; m <= l; ++m) {
+      int k = l * l + l + m;
+      cf[k] = h->h_facs[k] / h->h_facl[k];
+      cf[nylm + k] = 1.0 / (1.0 + (double)l);
+      cf[2 * nylm + k] = 1.0 / h->h_facl[k];
+    }
+  hipStream_t s = h->stream;
+  if ((rc = upload(h->grid, g4.data(), ng, s)) ||
+      (rc = upload(h->Y, h->h_basis.data(), h->h_basis.size(), s)) ||
+      (rc = upload(h->Yw, yw.data(), yw.size(), s)) ||
+      (rc = upload(h->facl, h->h_facl.data(), nylm, s)) ||
+      (rc = upload(h->facs, h->h_facs.data(), nylm, s)) ||
+      (rc = upload(h->cf, cf.data(), cf.size(), s)) || (rc = sync(h)))
+    return fail(rc);
+  if ((rc = set_geometry(h, x, y, z, rvdw)) || (rc = timed_setup(h))) return fail(rc);
+  *out = h;
+  return DDX_OK;
+}
+
+int ddx_free(ddx_handle *h) {
+  if (!h) return DDX_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->h_diis) (void)hipHostFree(h->h_diis);
+  for (auto &e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  hipStream_t s = h->stream;
+  delete h;   // device arrays release themselves
+  if (s) (void)hipStreamDestroy(s);
+  return DDX_OK;
+}
+
+int ddx_get_sizes(const ddx_handle *h, int *nsph, int *ncav, int *nylm, int *ngrid, int *lmax) {
+  if (!h) return DDX_ERR_ARG;
+  if (nsph) *nsph = h->n;
+  if (ncav) *ncav = h->ncav;
+  if (nylm) *nylm = h->nylm;
+  if (ngrid) *ngrid = h->ngrid;
+  if (lmax) *lmax = h->lmax;
+  return DDX_OK;
+}
+
+int ddx_get_array(const ddx_handle *hc, const char *name, double *dst) {
+  ddx_handle *h = const_cast<ddx_handle *>(hc);
+  if (!h || !name || !dst) return DDX_ERR_ARG;
+  size_t np = h->np;
+  auto cp = [&](const std::vector<double> &v) {
+    std::memcpy(dst, v.data(), v.size() * sizeof(double));
+    return DDX_OK;
+  };
+  if (!strcmp(name, "grid")) return cp(h->h_grid);
+  if (!strcmp(name, "w")) return cp(h->h_w);
+  if (!strcmp(name, "basis")) return cp(h->h_basis);
+  if (!strcmp(name, "facl")) return cp(h->h_facl);
+  if (!strcmp(name, "facs")) return cp(h->h_facs);
+  if (!strcmp(name, "rsph")) return cp(h->h_r);
+  if (!strcmp(name, "csph")) {
+    for (int i = 0; i < h->n; ++i) {
+      dst[3 * i] = h->h_x[i];
+      dst[3 * i + 1] = h->h_y[i];
+      dst[3 * i + 2] = h->h_z[i];
+    }
+    return DDX_OK;
+  }
+  const double *src = nullptr;
+  size_t cnt = 0;
+  if (!strcmp(name, "fi")) { src = h->fi.p; cnt = np; }
+  else if (!strcmp(name, "ui")) { src = h->ui.p; cnt = np; }
+  else if (!strcmp(name, "zi")) {
+    if (!h->igrad) { set_error("zi exists only with igrad = 1"); return DDX_ERR_STATE; }
+    src = h->zi.p; cnt = 3 * np;
+  } else if (!strcmp(name, "ccav")) { src = h->ccav.p; cnt = 3 * (size_t)h->ncav; }
+  else {
+    set_error("unknown array '%s'", name);
+    return DDX_ERR_ARG;
+  }
+  DDX_TRY(download(dst, src, cnt, h->stream));
+  return sync(h);
+}
+
+int ddx_get_neighbours(const ddx_handle *hc, int *inl, int *nl) {
+  ddx_handle *h = const_cast<ddx_handle *>(hc);
+  if (!h || !inl || !nl) return DDX_ERR_ARG;
+  DDX_HIP(hipMemcpyAsync(inl, h->inl.p, (h->n + 1) * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  if (h->nnl)
+    DDX_HIP(hipMemcpyAsync(nl, h->nl.p, h->nnl * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  DDX_TRY(sync(h));
+  for (int i = 0; i <= h->n; ++i) inl[i] += 1;
+  for (int k = 0; k < h->nnl; ++k) nl[k] += 1;
+  return DDX_OK;
+}
+
+int ddx_mkrhs(ddx_handle *h, const double *charge, double *phi, double *psi) {
+  if (!h || !charge || !phi || !psi) return DDX_ERR_ARG;
+  DDX_TRY(reserve_vectors(h));
+  DDX_TRY(upload(h->charge, charge, h->n, h->stream));
+  DDX_TRY(h->phi.reserve((size_t)h->ncav + 1));
+  DDX_TRY(coulomb_potential(h, h->n, h->charge.p, h->sph.p, h->ncav, h->ccav.p, h->phi.p));
+  DDX_TRY(make_psi(h, h->charge.p, h->psi.p));
+  DDX_TRY(download(phi, h->phi.p, h->ncav, h->stream));
+  DDX_TRY(download(psi, h->psi.p, (size_t)h->nylm * h->n, h->stream));
+  h->have_charge = true;
+  return sync(h);
+}
+
+int ddx_cosmo(ddx_handle *h, int star, int cart, const double *phi, const double *glm,
+              const double *psi, double *sigma, double *esolv) {
+  if (!h || !psi || !sigma) return DDX_ERR_ARG;
+  hipStream_t s = h->stream;
+  size_t nn = (size_t)h->nylm * h->n;
+  DDX_TRY(reserve_vectors(h));
+  DDX_TRY(upload(h->psi, psi, nn, s));
+  int rc;
+  if (!star) {
+    if (cart) {
+      if (!phi) return DDX_ERR_ARG;
+      DDX_TRY(upload(h->phi, phi, (size_t)h->ncav, s));
+      DDX_TRY(build_rhs(h, h->phi.p, h->rhs.p));
+    } else {
+      if (!glm) return DDX_ERR_ARG;
+      DDX_TRY(upload(h->rhs, glm, nn, s));
+    }
+    rc = run_direct(h, h->psi.p);
+    if (rc != DDX_OK && rc != DDX_ERR_NOCONV) return rc;
+    DDX_TRY(download(sigma, h->sigma.p, nn, s));
+    if (esolv) DDX_TRY(download(esolv, h->energy.p, 1, s));
+  } else {
+    rc = run_adjoint(h, h->psi.p);
+    if (rc != DDX_OK && rc != DDX_ERR_NOCONV) return rc;
+    DDX_TRY(download(sigma, h->s.p, nn, s));
+  }
+  DDX_TRY(sync(h));
+  return rc;
+}
+
+int ddx_forces_dd(ddx_handle *h, const double *phi, const double *sigma, const double *s_adj,
+                  double *fx) {
+  if (!h || !phi || !sigma || !s_adj || !fx) return DDX_ERR_ARG;
+  hipStream_t s = h->stream;
+  size_t nn = (size_t)h->nylm * h->n;
+  DDX_TRY(reserve_vectors(h));
+  DDX_TRY(upload(h->phi, phi, (size_t)h->ncav, s));
+  DDX_TRY(upload(h->sigma, sigma, nn, s));
+  DDX_TRY(upload(h->s, s_adj, nn, s));
+  DDX_TRY(forces_dd(h, h->phi.p, h->sigma.p, h->s.p, h->fx.p));
+  DDX_TRY(download(fx, h->fx.p, 3 * (size_t)h->n, s));
+  return sync(h);
+}
+
+int ddx_mkzeta(ddx_handle *h, const double *s_adj, double *zeta) {
+  if (!h || !s_adj || !zeta) return DDX_ERR_ARG;
+  hipStream_t s = h->stream;
+  DDX_TRY(reserve_vectors(h));
+  DDX_TRY(upload(h->s, s_adj, (size_t)h->nylm * h->n, s));
+  DDX_TRY(h->zeta.reserve((size_t)h->ncav + 1));
+  DDX_TRY(make_zeta(h, h->s.p, h->zeta.p));
+  DDX_TRY(download(zeta, h->zeta.p, h->ncav, s));
+  return sync(h);
+}
+
+int ddx_efld(ddx_handle *h, int nsrc, const double *src, const double *csrc, int ntrg,
+             const double *ctrg, double *ef) {
+  if (!h || nsrc < 0 || ntrg < 0 || (nsrc && (!src || !csrc)) || (ntrg && (!ctrg || !ef)))
+    return DDX_ERR_ARG;
+  hipStream_t s = h->stream;
+  DevArray<double> dq, dsrc, dtrg, def;
+  DDX_TRY(upload(dq, src, nsrc, s));
+  DDX_TRY(upload(dsrc, csrc, 3 * (size_t)nsrc, s));
+  DDX_TRY(upload(dtrg, ctrg, 3 * (size_t)ntrg, s));
+  DDX_TRY(def.reserve(3 * (size_t)ntrg + 3));
+  DDX_TRY(coulomb_field(h, nsrc, dq.p, dsrc.p, ntrg, dtrg.p, def.p));
+  DDX_TRY(download(ef, def.p, 3 * (size_t)ntrg, s));
+  return sync(h);
+}
+
+static int op_apply(ddx_handle *h, int which, const double *x, double *y) {
+  if (!h || !x || !y) return DDX_ERR_ARG;
+  hipStream_t s = h->stream;
+  size_t nn = (size_t)h->nylm * h->n;
+  DDX_TRY(reserve_vectors(h));
+  DDX_TRY(h->y.reserve(nn));
+  DDX_TRY(upload(h->x, x, nn, s));
+  if (which == 0) DDX_TRY(apply_lx(h, h->x.p, h->y.p, nullptr));
+  else if (which == 1) DDX_TRY(apply_lstarx(h, h->x.p, h->y.p, nullptr));
+  else DDX_TRY(device_ldm1x(h, h->x.p, h->y.p));
+  DDX_TRY(download(y, h->y.p, nn, s));
+  return sync(h);
+}
+
+int ddx_lx(ddx_handle *h, const double *x, double *y) { return op_apply(h, 0, x, y); }
+int ddx_lstarx(ddx_handle *h, const double *x, double *y) { return op_apply(h, 1, x, y); }
+int ddx_ldm1x(ddx_handle *h, const double *x, double *y) { return op_apply(h, 2, x, y); }
+
+int ddx_hnorm(ddx_handle *h, const double *x, double *norm) {
+  if (!h || !x || !norm) return DDX_ERR_ARG;
+  size_t nn = (size_t)h->nylm * h->n;
+  DDX_TRY(reserve_vectors(h));
+  DDX_TRY(upload(h->x, x, nn, h->stream));
+  return device_hnorm(h, h->x.p, norm);
+}
+
+int ddx_jacobi_diis(ddx_handle *h, int star, int diis_max, double tol, const double *rhs,
+                    double *x, int *n_iter, int *ok) {
+  if (!h || !rhs || !x || !n_iter || !ok) return DDX_ERR_ARG;
+  hipStream_t s = h->stream;
+  size_t nn = (size_t)h->nylm * h->n;
+  DDX_TRY(reserve_vectors(h));
+  DDX_TRY(upload(h->rhs, rhs, nn, s));
+  DDX_TRY(upload(h->x, x, nn, s));
+  DDX_TRY(solve_jacobi_diis(h, star ? 1 : 0, diis_max, tol, h->rhs.p, h->x.p, *n_iter, n_iter, ok,
+                            star ? &h->hist_adjoint : &h->hist_direct));
+  DDX_TRY(download(x, h->x.p, nn, s));
+  return sync(h);
+}
+
+int ddx_get_history(const ddx_handle *h, int star, int cap, double *norms, int *count) {
+  if (!h || !count) return DDX_ERR_ARG;
+  const std::vector<double> &v = star ? h->hist_adjoint : h->hist_direct;
+  int c = (int)v.size();
+  *count = c;
+  if (norms)
+    for (int k = 0; k < c && k < cap; ++k) norms[k] = v[k];
+  return DDX_OK;
+}
+
+int ddx_upload(ddx_handle *h, const double *x, const double *y, const double *z,
+               const double *rvdw, const double *charge) {
+  if (!h || !x || !y || !z || !rvdw || !charge) return DDX_ERR_ARG;
+  DDX_TRY(set_geometry(h, x, y, z, rvdw));
+  DDX_TRY(upload(h->charge, charge, h->n, h->stream));
+  h->have_charge = true;
+  h->have_setup = false;
+  return sync(h);
+}
+
+int ddx_run(ddx_handle *h) {
+  if (!h) return DDX_ERR_ARG;
+  if (!h->have_charge) {
+    set_error("ddx_run needs ddx_upload first");
+    return DDX_ERR_STATE;
+  }
+  DDX_TRY(timed_setup(h));
+  return pipeline(h);
+}
+
+int ddx_result(ddx_handle *h, double *esolv, double *fx) {
+  if (!h) return DDX_ERR_ARG;
+  if (esolv) DDX_TRY(download(esolv, h->energy.p, 1, h->stream));
+  if (fx && h->igrad) DDX_TRY(download(fx, h->fx.p, 3 * (size_t)h->n, h->stream));
+  return sync(h);
+}
+
+int ddx_energy_forces(ddx_handle *h, const double *charge, double *esolv, double *fx) {
+  if (!h || !charge || !esolv) return DDX_ERR_ARG;
+  DDX_TRY(upload(h->charge, charge, h->n, h->stream));
+  h->have_charge = true;
+  if (!h->have_setup) DDX_TRY(timed_setup(h));
+  DDX_TRY(pipeline(h));
+  return ddx_result(h, esolv, fx);
+}
+
+int ddx_set_precision(ddx_handle *h, int prec) {
+  if (!h || (prec != DDX_PREC_FP64 && prec != DDX_PREC_MIXED)) return DDX_ERR_ARG;
+  if (prec == DDX_PREC_MIXED) {
+    set_error("mixed precision is not built yet");
+    return DDX_ERR_ARG;
+  }
+  h->prec = prec;
+  return DDX_OK;
+}
+
+int ddx_get_stats(const ddx_handle *h, ddx_stats *st) {
+  if (!h || !st) return DDX_ERR_ARG;
+  *st = h->stats;
+  return DDX_OK;
+}
+
+}  // extern "C"

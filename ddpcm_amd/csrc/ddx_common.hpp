@@ -1,0 +1,140 @@
+// Shared definitions of libddx: status handling, device buffers, the handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/ddx.h"
+
+namespace ddx {
+
+constexpr int kMaxLmax = 16;       // template instantiations of the sweeps
+constexpr int kMaxDegree = 64;     // neighbours per sphere (64-bit point masks)
+constexpr int kDiisMax = 25;       // ndiis of the reference (ddcosmo.f90:87)
+constexpr int kMaxIter = 200;      // n_iter of cosmo (cosmo.f90:81)
+
+void set_error(const char *fmt, ...);
+
+#define DDX_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      ::ddx::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,            \
+                       hipGetErrorString(e_));                               \
+      return DDX_ERR_HIP;                                                    \
+    }                                                                        \
+  } while (0)
+
+#define DDX_TRY(call)                                                        \
+  do {                                                                       \
+    int s_ = (call);                                                         \
+    if (s_ != DDX_OK) return s_;                                             \
+  } while (0)
+
+// Device buffer that only grows; contents are not preserved across growth.
+template <typename T>
+struct DevArray {
+  T *p = nullptr;
+  size_t cap = 0;
+  DevArray() = default;
+  DevArray(const DevArray &) = delete;
+  DevArray &operator=(const DevArray &) = delete;
+  ~DevArray() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  int reserve(size_t n) {
+    if (n <= cap && p) return DDX_OK;
+    release();
+    size_t bytes = (n ? n : 1) * sizeof(T);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+      return DDX_ERR_ALLOC;
+    }
+    cap = n;
+    return DDX_OK;
+  }
+  operator T *() const { return p; }
+};
+
+// Device-side Jacobi/DIIS bookkeeping (one per solve in flight).
+struct DiisState {
+  int nmat;                 // reference's nmat (1-based count, jacobi_diis.f90:89)
+  int it;                   // iterations done
+  int done;                 // converged or failed: later kernels return at once
+  int ok;                   // converged
+  int singular;             // DIIS resets from a singular B
+  int slot[kDiisMax];       // logical history index -> physical slot
+  int ncoef;                // coefficients valid in coef[] (0: keep Jacobi step)
+  double coef[kDiisMax];    // extrapolation weights, by logical index
+  double b[(kDiisMax + 1) * (kDiisMax + 1)];  // bordered Gram matrix, 1-based, column-major
+  double norm;              // increment norm of the last iteration
+  double hist[kMaxIter + 1];
+};
+
+}  // namespace ddx
+
+struct ddx_handle {
+  // parameters (ddcosmo.f90:106-115)
+  int n = 0, lmax = 0, nylm = 0, ngrid = 0, iconv = 8, igrad = 0, iprint = 0;
+  int prec = DDX_PREC_FP64;
+  double eps = 78.39, eta = 0.1;
+  size_t np = 0;            // n * ngrid points
+
+  // host copies of tables and geometry
+  std::vector<double> h_grid, h_w, h_basis, h_facl, h_facs;
+  std::vector<double> h_x, h_y, h_z, h_r;
+
+  // sizes produced by setup
+  int ncav = 0, nnl = 0, ntri = 0, maxdeg = 0;
+  bool have_setup = false, have_sigma = false, have_s = false, have_charge = false;
+
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8] = {};
+
+  // tables
+  ddx::DevArray<double4> grid;    // (x, y, z, w) per Lebedev point
+  ddx::DevArray<double> Y, Yw;    // basis (nylm, ngrid) and basis * w
+  ddx::DevArray<double> facl, cf; // (2l+1)/4pi and facs * 4pi/(2l+1)
+  ddx::DevArray<double> facs;
+
+  // geometry
+  ddx::DevArray<double4> sph;     // (x, y, z, r)
+  ddx::DevArray<double> charge;
+
+  // neighbours
+  ddx::DevArray<int> inl, nl, rev, prow, degcnt;
+
+  // per point
+  ddx::DevArray<double> fi, ui, zi;
+  ddx::DevArray<unsigned long long> tmask;
+  ddx::DevArray<int> ptcnt, ptoff, cavflag, cavidx;
+  ddx::DevArray<double> ccav;
+
+  // triples grouped by the sphere that contains the point
+  ddx::DevArray<int> paircnt, pairoff;
+  ddx::DevArray<double4> tgeo;    // (cos theta, sin theta, cos phi, sin phi)
+  ddx::DevArray<double2> ttw;     // (t, W)
+  ddx::DevArray<int> tp;          // owner point index i*ngrid + n
+  ddx::DevArray<int> pttri;       // per point: its triples in neighbour order
+
+  // solver workspace
+  ddx::DevArray<double> x, y, rhs, xnew, P, xiw, r, xd, ed, part;
+  ddx::DevArray<ddx::DiisState> diis;
+  ddx::DiisState *h_diis = nullptr;  // pinned host mirror
+
+  // problem data / results
+  ddx::DevArray<double> phi, psi, sigma, s, xi, fx, zeta, ef, energy, fpart;
+  ddx::DevArray<int> scan_tmp, misc;
+
+  std::vector<double> hist_direct, hist_adjoint;
+  ddx_stats stats{};
+};

@@ -1,0 +1,309 @@
+// Device setup of one ddCOSMO system (the per-geometry part of ddinit,
+// ddcosmo.f90:308-468), plus the triple lists the sweeps run on.
+//
+//   neighbours  CSR, increasing j, |c_i - c_j| <= r_i + r_j (ddcosmo.f90:308-324)
+//   per point   fi = sum_j chi(t_n^ij), ui = max(0, 1 - fi) when fi <= 1,
+//               zi in the switch region (ddcosmo.f90:359-411), plus a 64-bit
+//               mask of the neighbours j with t_n^ij < 1
+//   cavity      exposed points ui > 0, sphere-major (ddcosmo.f90:422-468)
+//   triples     every (owner sphere o, point n, container c) with t < 1,
+//               grouped by container c (owners ascending, then points, the
+//               loop order of adjrhs, ddcosmo.f90:1316-1381), with the
+//               geometry the sweeps need: cos/sin theta, cos/sin phi of
+//               s = v/|v| (ylmbas, ddcosmo.f90:846-866), t and the weight
+//               W = chi(t)/max(fi, 1) (ddcosmo.f90:1722-1730, 1340-1348).
+//               pttri lists the triples of each point in neighbour order
+//               (the loop order of calcv, ddcosmo.f90:1701-1739).
+#include "ddx_device.hpp"
+#include "ddx_kernels.hpp"
+
+namespace ddx {
+namespace {
+
+constexpr int kSetupThreads = 256;
+
+__global__ __launch_bounds__(kSetupThreads) void k_nbr_count(const double4 *__restrict__ sph,
+                                                             int n, int *__restrict__ deg) {
+  __shared__ int wcnt[kSetupThreads / kWave];
+  int i = blockIdx.x;
+  double4 si = sph[i];
+  int cnt = 0;
+  for (int j = threadIdx.x; j < n; j += kSetupThreads) {
+    cnt += (j != i && spheres_overlap(si, sph[j])) ? 1 : 0;
+  }
+  cnt = wave_sum_int(cnt);
+  if (lane_id() == 0) wcnt[threadIdx.x / kWave] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) deg[i] = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+}
+
+__global__ __launch_bounds__(kSetupThreads) void k_nbr_fill(const double4 *__restrict__ sph,
+                                                            int n, const int *__restrict__ inl,
+                                                            int *__restrict__ nl,
+                                                            int *__restrict__ prow) {
+  __shared__ int wcnt[kSetupThreads / kWave];
+  int i = blockIdx.x;
+  double4 si = sph[i];
+  int run = inl[i];
+  int wave = threadIdx.x / kWave;
+  for (int j0 = 0; j0 < n; j0 += kSetupThreads) {
+    int j = j0 + threadIdx.x;
+    bool hit = false;
+    if (j < n && j != i) hit = spheres_overlap(si, sph[j]);
+    unsigned long long b = __ballot(hit);
+    if (lane_id() == 0) wcnt[wave] = __popcll(b);
+    __syncthreads();
+    int before = 0;
+    for (int w = 0; w < wave; ++w) before += wcnt[w];
+    int pos = run + before + ballot_prefix(b);
+    if (hit && pos < inl[i + 1]) {
+      nl[pos] = j;
+      prow[pos] = i;
+    }
+    run += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+    __syncthreads();
+  }
+}
+
+// rev[q] = position of sphere prow[q] in the neighbour list of nl[q]
+__global__ void k_nbr_rev(int nnl, const int *__restrict__ inl, const int *__restrict__ nl,
+                          const int *__restrict__ prow, int *__restrict__ rev) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nnl) return;
+  int c = prow[q], o = nl[q];
+  int r = -1;
+  for (int k = inl[o]; k < inl[o + 1]; ++k)
+    if (nl[k] == c) { r = k - inl[o]; break; }
+  rev[q] = r;
+}
+
+// One thread per point (n, i): fi, ui, zi, the t < 1 mask, the triple count
+// and the cavity flag.  blockIdx.y is the sphere.
+__global__ __launch_bounds__(kSetupThreads) void k_points(
+    const double4 *__restrict__ sph, const double4 *__restrict__ grid, int ngrid,
+    const int *__restrict__ inl, const int *__restrict__ nl, double eta, int igrad,
+    double *__restrict__ fi, double *__restrict__ ui, double *__restrict__ zi,
+    unsigned long long *__restrict__ tmask, int *__restrict__ ptcnt,
+    int *__restrict__ cavflag) {
+  int i = blockIdx.y;
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= ngrid) return;
+  size_t p = (size_t)i * ngrid + n;
+  double4 si = sph[i], g = grid[n];
+  double f = 0.0, z0 = 0.0, z1 = 0.0, z2 = 0.0;
+  unsigned long long mask = 0;
+  int beg = inl[i], end = inl[i + 1];
+  for (int k = beg; k < end; ++k) {
+    double4 sj = sph[nl[k]];
+    double v[3];
+    point_vec(si, g, sj, v);
+    double vv = norm3(v);
+    double t = vv / sj.w;
+    if (t < 1.0) {
+      f += fsw(t, eta);
+      mask |= 1ull << (k - beg);
+      if (igrad && t > 1.0 - eta) {
+        double fac = dfsw(t, eta) / sj.w;
+        z0 += fac * v[0] / vv;
+        z1 += fac * v[1] / vv;
+        z2 += fac * v[2] / vv;
+      }
+    }
+  }
+  double u = (f <= 1.0) ? 1.0 - f : 0.0;
+  fi[p] = f;
+  ui[p] = u;
+  if (igrad) {
+    zi[3 * p + 0] = z0;
+    zi[3 * p + 1] = z1;
+    zi[3 * p + 2] = z2;
+  }
+  tmask[p] = mask;
+  ptcnt[p] = __popcll(mask);
+  cavflag[p] = u > 0.0 ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kSetupThreads) void k_ccav(const double4 *__restrict__ sph,
+                                                        const double4 *__restrict__ grid,
+                                                        int ngrid, const int *__restrict__ cavflag,
+                                                        const int *__restrict__ cavidx,
+                                                        double *__restrict__ ccav) {
+  int i = blockIdx.y;
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= ngrid) return;
+  size_t p = (size_t)i * ngrid + n;
+  if (!cavflag[p]) return;
+  int ic = cavidx[p];
+  double4 si = sph[i], g = grid[n];
+  ccav[3 * (size_t)ic + 0] = si.x + si.w * g.x;
+  ccav[3 * (size_t)ic + 1] = si.y + si.w * g.y;
+  ccav[3 * (size_t)ic + 2] = si.z + si.w * g.z;
+}
+
+// One wave per neighbour pair q = (container c = prow[q], owner o = nl[q]):
+// the number of points of o inside c.
+__global__ __launch_bounds__(kSetupThreads) void k_pair_count(
+    int nnl, int ngrid, const int *__restrict__ nl, const int *__restrict__ rev,
+    const unsigned long long *__restrict__ tmask, int *__restrict__ paircnt) {
+  int q = blockIdx.x * (kSetupThreads / kWave) + threadIdx.x / kWave;
+  if (q >= nnl) return;
+  int o = nl[q], kr = rev[q];
+  if (kr < 0) {   // cannot happen: the overlap test is symmetric
+    if (lane_id() == 0) paircnt[q] = 0;
+    return;
+  }
+  int cnt = 0;
+  for (int n = lane_id(); n < ngrid; n += kWave)
+    cnt += (int)((tmask[(size_t)o * ngrid + n] >> kr) & 1ull);
+  cnt = wave_sum_int(cnt);
+  if (lane_id() == 0) paircnt[q] = cnt;
+}
+
+__global__ __launch_bounds__(kSetupThreads) void k_pair_fill(
+    int nnl, int ngrid, double eta, const double4 *__restrict__ sph,
+    const double4 *__restrict__ grid, const int *__restrict__ nl,
+    const int *__restrict__ prow, const int *__restrict__ rev,
+    const unsigned long long *__restrict__ tmask, const double *__restrict__ fi,
+    const int *__restrict__ pairoff, const int *__restrict__ ptoff,
+    double4 *__restrict__ tgeo, double2 *__restrict__ ttw, int *__restrict__ tp,
+    int *__restrict__ pttri) {
+  int q = blockIdx.x * (kSetupThreads / kWave) + threadIdx.x / kWave;
+  if (q >= nnl) return;
+  int c = prow[q], o = nl[q], kr = rev[q];
+  if (kr < 0) return;
+  double4 sc = sph[c], so = sph[o];
+  int run = pairoff[q];
+  unsigned long long below = (1ull << kr) - 1ull;
+  for (int n0 = 0; n0 < ngrid; n0 += kWave) {
+    int n = n0 + lane_id();
+    size_t p = (size_t)o * ngrid + n;
+    unsigned long long m = (n < ngrid) ? tmask[p] : 0ull;
+    bool hit = (m >> kr) & 1ull;
+    unsigned long long b = __ballot(hit);
+    if (hit) {
+      int idx = run + ballot_prefix(b);
+      double4 g = grid[n];
+      double v[3];
+      point_vec(so, g, sc, v);
+      double vv = norm3(v);
+      double t = vv / sc.w;
+      double s0 = v[0] / vv, s1 = v[1] / vv, s2 = v[2] / vv;
+      double cthe = s2;
+      double sthe = sqrt(1.0 - cthe * cthe);
+      double cphi = 1.0, sphi = 0.0;
+      if (sthe != 0.0) {
+        cphi = s0 / sthe;
+        sphi = s1 / sthe;
+      }
+      double chi = fsw(t, eta);
+      double f = fi[p];
+      double w = (f > 1.0) ? chi / f : chi;
+      tgeo[idx] = make_double4(cthe, sthe, cphi, sphi);
+      ttw[idx] = make_double2(t, w);
+      tp[idx] = (int)p;
+      pttri[ptoff[p] + __popcll(m & below)] = idx;
+    }
+    run += __popcll(b);
+  }
+}
+
+}  // namespace
+
+int run_setup(ddx_handle *h) {
+  hipStream_t s = h->stream;
+  int n = h->n, ng = h->ngrid;
+  size_t np = (size_t)n * ng;
+  h->np = np;
+
+  // neighbour list
+  DDX_TRY(h->degcnt.reserve(n + 1));
+  DDX_TRY(h->inl.reserve(n + 1));
+  hipLaunchKernelGGL(k_nbr_count, dim3(n), dim3(kSetupThreads), 0, s, h->sph.p, n, h->degcnt.p);
+  DDX_HIP(hipGetLastError());
+  DDX_TRY(scan_exclusive(h->degcnt.p, h->inl.p, n, h->scan_tmp, s));
+  std::vector<int> deg(n);
+  int nnl = 0;
+  DDX_HIP(hipMemcpyAsync(deg.data(), h->degcnt.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+  DDX_HIP(hipMemcpyAsync(&nnl, h->inl.p + n, sizeof(int), hipMemcpyDeviceToHost, s));
+  DDX_HIP(hipStreamSynchronize(s));
+  int maxdeg = 0;
+  for (int d : deg) maxdeg = d > maxdeg ? d : maxdeg;
+  h->nnl = nnl;
+  h->maxdeg = maxdeg;
+  if (maxdeg > kMaxDegree) {
+    set_error("a sphere overlaps %d others; at most %d are supported", maxdeg, kMaxDegree);
+    return DDX_ERR_CAPACITY;
+  }
+  DDX_TRY(h->nl.reserve(nnl + 1));
+  DDX_TRY(h->rev.reserve(nnl + 1));
+  DDX_TRY(h->paircnt.reserve(nnl + 1));
+  DDX_TRY(h->pairoff.reserve(nnl + 1));
+  DDX_TRY(h->prow.reserve(nnl + 1));
+  if (nnl > 0) {
+    hipLaunchKernelGGL(k_nbr_fill, dim3(n), dim3(kSetupThreads), 0, s, h->sph.p, n, h->inl.p,
+                       h->nl.p, h->prow.p);
+    DDX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_nbr_rev, dim3((nnl + 255) / 256), dim3(256), 0, s, nnl, h->inl.p,
+                       h->nl.p, h->prow.p, h->rev.p);
+    DDX_HIP(hipGetLastError());
+  }
+
+  // per-point quantities
+  DDX_TRY(h->fi.reserve(np));
+  DDX_TRY(h->ui.reserve(np));
+  if (h->igrad) DDX_TRY(h->zi.reserve(3 * np));
+  DDX_TRY(h->tmask.reserve(np));
+  DDX_TRY(h->ptcnt.reserve(np + 1));
+  DDX_TRY(h->ptoff.reserve(np + 1));
+  DDX_TRY(h->cavflag.reserve(np + 1));
+  DDX_TRY(h->cavidx.reserve(np + 1));
+  dim3 pg((ng + kSetupThreads - 1) / kSetupThreads, n);
+  hipLaunchKernelGGL(k_points, pg, dim3(kSetupThreads), 0, s, h->sph.p, h->grid.p, ng,
+                     h->inl.p, h->nl.p, h->eta, h->igrad, h->fi.p, h->ui.p,
+                     h->igrad ? h->zi.p : nullptr, h->tmask.p, h->ptcnt.p, h->cavflag.p);
+  DDX_HIP(hipGetLastError());
+  DDX_TRY(scan_exclusive(h->ptcnt.p, h->ptoff.p, np, h->scan_tmp, s));
+  DDX_TRY(scan_exclusive(h->cavflag.p, h->cavidx.p, np, h->scan_tmp, s));
+  int ntri = 0, ncav = 0;
+  DDX_HIP(hipMemcpyAsync(&ntri, h->ptoff.p + np, sizeof(int), hipMemcpyDeviceToHost, s));
+  DDX_HIP(hipMemcpyAsync(&ncav, h->cavidx.p + np, sizeof(int), hipMemcpyDeviceToHost, s));
+
+  // triples per neighbour pair, grouped by container
+  if (nnl > 0) {
+    int wpb = kSetupThreads / kWave;
+    hipLaunchKernelGGL(k_pair_count, dim3((nnl + wpb - 1) / wpb), dim3(kSetupThreads), 0, s,
+                       nnl, ng, h->nl.p, h->rev.p, h->tmask.p, h->paircnt.p);
+    DDX_HIP(hipGetLastError());
+    DDX_TRY(scan_exclusive(h->paircnt.p, h->pairoff.p, nnl, h->scan_tmp, s));
+  } else {
+    DDX_HIP(hipMemsetAsync(h->pairoff.p, 0, sizeof(int), s));
+  }
+  DDX_HIP(hipStreamSynchronize(s));
+  h->ntri = ntri;
+  h->ncav = ncav;
+
+  DDX_TRY(h->tgeo.reserve(ntri + 1));
+  DDX_TRY(h->ttw.reserve(ntri + 1));
+  DDX_TRY(h->tp.reserve(ntri + 1));
+  DDX_TRY(h->pttri.reserve(ntri + 1));
+  if (nnl > 0) {
+    int wpb = kSetupThreads / kWave;
+    hipLaunchKernelGGL(k_pair_fill, dim3((nnl + wpb - 1) / wpb), dim3(kSetupThreads), 0, s,
+                       nnl, ng, h->eta, h->sph.p, h->grid.p, h->nl.p, h->prow.p, h->rev.p,
+                       h->tmask.p, h->fi.p, h->pairoff.p, h->ptoff.p, h->tgeo.p, h->ttw.p,
+                       h->tp.p, h->pttri.p);
+    DDX_HIP(hipGetLastError());
+  }
+  DDX_TRY(h->ccav.reserve(3 * (size_t)ncav + 3));
+  hipLaunchKernelGGL(k_ccav, pg, dim3(kSetupThreads), 0, s, h->sph.p, h->grid.p, ng,
+                     h->cavflag.p, h->cavidx.p, h->ccav.p);
+  DDX_HIP(hipGetLastError());
+  h->have_setup = true;
+  h->stats.ntriples = ntri;
+  h->stats.nnl = nnl;
+  h->stats.ncav = ncav;
+  h->stats.max_degree = maxdeg;
+  return DDX_OK;
+}
+
+}  // namespace ddx

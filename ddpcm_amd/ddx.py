@@ -1,0 +1,281 @@
+"""Python host mirror of the reference ddCOSMO module over libddx's C-ABI.
+
+Procedure names and argument meaning follow the reference module
+(ddcosmo.f90, cosmo.f90, forces_dd.f90, matvec.f90, jacobi_diis.f90,
+mkrhs.f90, efld.f90); arrays use the reference's Fortran layout, exposed as
+C-ordered numpy arrays with the slow index first, e.g. sigma(nylm, nsph)
+is an array of shape (nsph, nylm).  All compute runs in libddx.so on the GPU;
+this module has no CPU fallback and raises if the library cannot be loaded.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libddx.so")
+
+DDX_OK, DDX_ERR_ARG, DDX_ERR_ALLOC, DDX_ERR_NOCONV = 0, 1, 2, 3
+DDX_ERR_CAPACITY, DDX_ERR_HIP, DDX_ERR_STATE = 4, 5, 6
+DDX_PREC_FP64, DDX_PREC_MIXED = 0, 1
+
+EXPORTS = (
+    "ddx_last_error", "ddx_init", "ddx_free", "ddx_get_sizes", "ddx_get_array",
+    "ddx_get_neighbours", "ddx_mkrhs", "ddx_cosmo", "ddx_forces_dd", "ddx_mkzeta",
+    "ddx_efld", "ddx_lx", "ddx_lstarx", "ddx_ldm1x", "ddx_hnorm", "ddx_jacobi_diis",
+    "ddx_get_history", "ddx_energy_forces", "ddx_upload", "ddx_run", "ddx_result",
+    "ddx_set_precision", "ddx_get_stats",
+)
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class DDXError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("libddx error %d: %s" % (code, msg))
+        self.code = code
+
+
+class NotConverged(DDXError):
+    pass
+
+
+class Stats(C.Structure):
+    _fields_ = [("niter_direct", C.c_int), ("niter_adjoint", C.c_int),
+                ("ntriples", C.c_int), ("nnl", C.c_int), ("ncav", C.c_int),
+                ("max_degree", C.c_int), ("ms_setup", C.c_double), ("ms_rhs", C.c_double),
+                ("ms_direct", C.c_double), ("ms_adjoint", C.c_double),
+                ("ms_forces", C.c_double)]
+
+
+_LIB = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libddx.so and declares its signatures (no GPU call is made)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise ImportError("libddx.so is not built (%s); run __graft_entry__.build()" % path)
+    L = C.CDLL(path)
+    L.ddx_last_error.restype = C.c_char_p
+    L.ddx_init.argtypes = [C.POINTER(C.c_void_p), C.c_int, _dp, _dp, _dp, _dp, C.c_int,
+                           C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
+    L.ddx_free.argtypes = [C.c_void_p]
+    L.ddx_get_sizes.argtypes = [C.c_void_p, _ip, _ip, _ip, _ip, _ip]
+    L.ddx_get_array.argtypes = [C.c_void_p, C.c_char_p, _dp]
+    L.ddx_get_neighbours.argtypes = [C.c_void_p, _ip, _ip]
+    L.ddx_mkrhs.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    L.ddx_cosmo.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.ddx_forces_dd.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+    L.ddx_mkzeta.argtypes = [C.c_void_p, _dp, _dp]
+    L.ddx_efld.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int, _dp, _dp]
+    for f in ("ddx_lx", "ddx_lstarx", "ddx_ldm1x"):
+        getattr(L, f).argtypes = [C.c_void_p, _dp, _dp]
+    L.ddx_hnorm.argtypes = [C.c_void_p, _dp, _dp]
+    L.ddx_jacobi_diis.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp, _ip, _ip]
+    L.ddx_get_history.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _ip]
+    L.ddx_energy_forces.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    L.ddx_upload.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
+    L.ddx_run.argtypes = [C.c_void_p]
+    L.ddx_result.argtypes = [C.c_void_p, _dp, _dp]
+    L.ddx_set_precision.argtypes = [C.c_void_p, C.c_int]
+    L.ddx_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
+    _LIB = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+class DDCosmo:
+    """One ddCOSMO system on the GPU (ddinit, ddcosmo.f90:159).
+
+    x, y, z, rvdw in bohr.  ngrid is snapped to the nearest Lebedev size.
+    """
+
+    def __init__(self, x, y, z, rvdw, lmax, ngrid, iconv=8, igrad=1, eps=78.39,
+                 eta=0.1, iprint=0):
+        self.L = load_library()
+        x, y, z, r = (_f64(a).ravel() for a in (x, y, z, rvdw))
+        self.h = C.c_void_p()
+        self._check(self.L.ddx_init(C.byref(self.h), x.size, _p(x), _p(y), _p(z), _p(r),
+                                    iprint, lmax, ngrid, iconv, igrad, eps, eta))
+        s = [C.c_int() for _ in range(5)]
+        self._check(self.L.ddx_get_sizes(self.h, *[C.byref(v) for v in s]))
+        self.nsph, self.ncav, self.nylm, self.ngrid, self.lmax = (v.value for v in s)
+        self.iconv, self.igrad, self.eps, self.eta, self.iprint = iconv, igrad, eps, eta, iprint
+
+    @classmethod
+    def from_input(cls, d, **over):
+        x, y, z, r = d.bohr()
+        kw = dict(lmax=d.lmax, ngrid=d.ngrid, iconv=d.iconv, igrad=d.igrad, eps=d.eps,
+                  eta=d.eta, iprint=d.iprint)
+        kw.update(over)
+        return cls(x, y, z, r, **kw)
+
+    def _check(self, rc, allow_noconv=False):
+        if rc == DDX_OK:
+            return
+        msg = self.L.ddx_last_error().decode(errors="replace")
+        if rc == DDX_ERR_NOCONV:
+            if allow_noconv:
+                return
+            raise NotConverged(rc, msg)
+        raise DDXError(rc, msg)
+
+    def memfree(self):
+        """memfree (ddcosmo.f90:491)."""
+        if getattr(self, "h", None):
+            self.L.ddx_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.memfree()
+        except Exception:
+            pass
+
+    # ---- module data -------------------------------------------------------
+    def get(self, name):
+        shapes = {"grid": (self.ngrid, 3), "w": (self.ngrid,), "basis": (self.ngrid, self.nylm),
+                  "facl": (self.nylm,), "facs": (self.nylm,), "fi": (self.nsph, self.ngrid),
+                  "ui": (self.nsph, self.ngrid), "zi": (self.nsph, self.ngrid, 3),
+                  "ccav": (self.ncav, 3), "csph": (self.nsph, 3), "rsph": (self.nsph,)}
+        a = np.zeros(shapes[name])
+        self._check(self.L.ddx_get_array(self.h, name.encode(), _p(a)))
+        return a
+
+    @property
+    def ccav(self):
+        return self.get("ccav")
+
+    def neighbours(self):
+        """(inl, nl), 1-based as the reference stores them."""
+        inl = np.zeros(self.nsph + 1, dtype=np.int32)
+        st = self.stats()
+        nl = np.zeros(max(st.nnl, 1), dtype=np.int32)
+        self._check(self.L.ddx_get_neighbours(self.h, inl.ctypes.data_as(_ip),
+                                              nl.ctypes.data_as(_ip)))
+        return inl, nl[:st.nnl]
+
+    def stats(self):
+        s = Stats()
+        self._check(self.L.ddx_get_stats(self.h, C.byref(s)))
+        return s
+
+    def history(self, star):
+        cnt = C.c_int()
+        buf = np.zeros(256)
+        self._check(self.L.ddx_get_history(self.h, int(star), 256, _p(buf), C.byref(cnt)))
+        return buf[:cnt.value].copy()
+
+    # ---- reference procedures ----------------------------------------------
+    def mkrhs(self, charge):
+        """mkrhs (mkrhs.f90:1): phi(ncav), psi(nylm, nsph)."""
+        q = _f64(charge).ravel()
+        phi = np.zeros(self.ncav)
+        psi = np.zeros((self.nsph, self.nylm))
+        self._check(self.L.ddx_mkrhs(self.h, _p(q), _p(phi), _p(psi)))
+        return phi, psi
+
+    def cosmo(self, star, cart=True, phi=None, glm=None, psi=None):
+        """cosmo (cosmo.f90:1): returns (sigma, esolv)."""
+        psi = _f64(psi, (self.nsph, self.nylm))
+        sigma = np.zeros((self.nsph, self.nylm))
+        es = C.c_double(0.0)
+        z = np.zeros(1)
+        phi_a = _f64(phi) if phi is not None else z
+        glm_a = _f64(glm, (self.nsph, self.nylm)) if glm is not None else z
+        rc = self.L.ddx_cosmo(self.h, int(bool(star)), int(bool(cart)), _p(phi_a), _p(glm_a),
+                              _p(psi), _p(sigma), C.cast(C.byref(es), _dp))
+        self._check(rc)
+        return sigma, es.value
+
+    def forces_dd(self, phi, sigma, s):
+        """forces_dd (forces_dd.f90:1): fx(3, nsph)."""
+        fx = np.zeros((self.nsph, 3))
+        self._check(self.L.ddx_forces_dd(self.h, _p(_f64(phi)), _p(_f64(sigma)), _p(_f64(s)),
+                                         _p(fx)))
+        return fx
+
+    def ddmkzeta(self, s):
+        zeta = np.zeros(self.ncav)
+        self._check(self.L.ddx_mkzeta(self.h, _p(_f64(s)), _p(zeta)))
+        return zeta
+
+    def efld(self, src, csrc, ctrg):
+        """efld (efld.f90:1): field of point sources at targets, (ntrg, 3)."""
+        src, csrc, ctrg = _f64(src).ravel(), _f64(csrc), _f64(ctrg)
+        ef = np.zeros((ctrg.shape[0], 3))
+        self._check(self.L.ddx_efld(self.h, src.size, _p(src), _p(csrc), ctrg.shape[0],
+                                    _p(ctrg), _p(ef)))
+        return ef
+
+    def _op(self, f, x):
+        x = _f64(x, (self.nsph, self.nylm))
+        y = np.zeros_like(x)
+        self._check(f(self.h, _p(x), _p(y)))
+        return y
+
+    def lx(self, x):
+        return self._op(self.L.ddx_lx, x)
+
+    def lstarx(self, x):
+        return self._op(self.L.ddx_lstarx, x)
+
+    def ldm1x(self, x):
+        return self._op(self.L.ddx_ldm1x, x)
+
+    def hnorm(self, x):
+        x = _f64(x, (self.nsph, self.nylm))
+        v = C.c_double()
+        self._check(self.L.ddx_hnorm(self.h, _p(x), C.byref(v)))
+        return v.value
+
+    def jacobi_diis(self, star, rhs, x, diis_max=25, tol=None, n_iter=200):
+        """jacobi_diis (jacobi_diis.f90:43) with norm 4; returns (x, n_iter, ok)."""
+        tol = 10.0 ** (-self.iconv) if tol is None else tol
+        rhs = _f64(rhs, (self.nsph, self.nylm))
+        x = _f64(x, (self.nsph, self.nylm)).copy()
+        nit, ok = C.c_int(n_iter), C.c_int(0)
+        self._check(self.L.ddx_jacobi_diis(self.h, int(star), diis_max, tol, _p(rhs), _p(x),
+                                           C.byref(nit), C.byref(ok)))
+        return x, nit.value, bool(ok.value)
+
+    def energy_forces(self, charge):
+        """main.f90:164-265 for point charges: (esolv in Eh, fx (nsph, 3) in Eh/bohr)."""
+        q = _f64(charge).ravel()
+        es = C.c_double()
+        fx = np.zeros((self.nsph, 3))
+        self._check(self.L.ddx_energy_forces(self.h, _p(q), C.cast(C.byref(es), _dp), _p(fx)))
+        return es.value, fx
+
+    # ---- device-resident path ------------------------------------------------
+    def upload(self, x, y, z, rvdw, charge):
+        x, y, z, r, q = (_f64(a).ravel() for a in (x, y, z, rvdw, charge))
+        self._check(self.L.ddx_upload(self.h, _p(x), _p(y), _p(z), _p(r), _p(q)))
+
+    def run(self):
+        self._check(self.L.ddx_run(self.h))
+
+    def result(self):
+        es = C.c_double()
+        fx = np.zeros((self.nsph, 3))
+        self._check(self.L.ddx_result(self.h, C.cast(C.byref(es), _dp), _p(fx)))
+        return es.value, fx
+
+    def set_precision(self, prec):
+        self._check(self.L.ddx_set_precision(self.h, prec))

@@ -1,0 +1,133 @@
+/*
+ * ddx — ddCOSMO on one AMD Instinct MI355X (gfx950).
+ *
+ * C-ABI of libddx.so: plain pointers, sizes and int status codes.  Every
+ * entry point replaces one procedure of the reference ddCOSMO module
+ * (filippolipparini/ddPCM) and keeps its argument meaning; the citation on
+ * each line names the reference interface (file:line).
+ *
+ * Conventions (the reference's): FP64, column-major, harmonic index fastest
+ * in (nylm, nsph) arrays with lm = l*l + l + m (0-based here), grid point
+ * fastest in (ngrid, nsph) arrays, coordinates and radii in bohr.  Pointers
+ * are host memory; all device state lives inside the handle.  One handle
+ * belongs to one host thread at a time.  Every call returns DDX_OK (0) or a
+ * non-zero status; ddx_last_error() gives the message.  Where the reference
+ * prints and stops, ddx returns the status instead.
+ */
+#ifndef DDX_H
+#define DDX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DDX_OK 0
+#define DDX_ERR_ARG 1        /* invalid argument (sizes, lmax, grid, name)      */
+#define DDX_ERR_ALLOC 2      /* device or host allocation failed                */
+#define DDX_ERR_NOCONV 3     /* Jacobi/DIIS did not converge in n_iter steps    */
+#define DDX_ERR_CAPACITY 4   /* a sphere has more than 64 overlapping spheres   */
+#define DDX_ERR_HIP 5        /* HIP runtime error                               */
+#define DDX_ERR_STATE 6      /* call out of order (e.g. forces before solves)   */
+
+/* precision of the solve (ddx_set_precision) */
+#define DDX_PREC_FP64 0      /* FP64 throughout: the reference's arithmetic      */
+#define DDX_PREC_MIXED 1     /* FP32 sweeps inside FP64 iterative refinement     */
+
+typedef struct ddx_handle ddx_handle;
+
+typedef struct ddx_stats {
+  int niter_direct;         /* Jacobi/DIIS iterations of the last direct solve  */
+  int niter_adjoint;        /* ... of the last adjoint solve                     */
+  int ntriples;             /* (sphere, point, neighbour) triples with t < 1    */
+  int nnl;                  /* neighbour pairs                                   */
+  int ncav;                 /* exposed cavity points                             */
+  int max_degree;           /* largest neighbour count                           */
+  double ms_setup, ms_rhs, ms_direct, ms_adjoint, ms_forces; /* last run, HIP events */
+} ddx_stats;
+
+const char *ddx_last_error(void);
+
+/* ddinit (ddcosmo.f90:159): tables, neighbour list, fi/ui/zi, cavity.
+ * ngrid is snapped to the nearest Lebedev size (ddcosmo.f90:188-199). */
+int ddx_init(ddx_handle **h, int n, const double *x, const double *y,
+             const double *z, const double *rvdw, int iprint, int lmax,
+             int ngrid, int iconv, int igrad, double eps, double eta);
+
+/* memfree (ddcosmo.f90:491) */
+int ddx_free(ddx_handle *h);
+
+/* module scalars nsph, ncav, nylm, ngrid, lmax (ddcosmo.f90:106-115) */
+int ddx_get_sizes(const ddx_handle *h, int *nsph, int *ncav, int *nylm,
+                  int *ngrid, int *lmax);
+
+/* module arrays (ddcosmo.f90:119-123): "ccav" (3,ncav), "csph" (3,nsph),
+ * "rsph", "w", "grid" (3,ngrid), "basis" (nylm,ngrid), "facl", "facs",
+ * "fi", "ui" (ngrid,nsph), "zi" (3,ngrid,nsph). */
+int ddx_get_array(const ddx_handle *h, const char *name, double *dst);
+/* inl (nsph+1, 1-based as in the reference) and nl (inl[nsph]-1 entries) */
+int ddx_get_neighbours(const ddx_handle *h, int *inl, int *nl);
+
+/* mkrhs (mkrhs.f90:1): potential of point charges at the cavity points and
+ * the psi vector; phi(ncav), psi(nylm,nsph). */
+int ddx_mkrhs(ddx_handle *h, const double *charge, double *phi, double *psi);
+
+/* cosmo (cosmo.f90:1): star=0 solves L X = g (g from phi when cart=1, else
+ * glm) and returns esolv; star=1 solves L* S = psi.  The guess is always
+ * D^-1 rhs, as in the reference. */
+int ddx_cosmo(ddx_handle *h, int star, int cart, const double *phi,
+              const double *glm, const double *psi, double *sigma,
+              double *esolv);
+
+/* forces_dd (forces_dd.f90:1): ddCOSMO part of the forces, fx(3,nsph). */
+int ddx_forces_dd(ddx_handle *h, const double *phi, const double *sigma,
+                  const double *s, double *fx);
+
+/* ddmkzeta (ddcosmo.f90:1787): zeta(ncav). */
+int ddx_mkzeta(ddx_handle *h, const double *s, double *zeta);
+
+/* efld (efld.f90:1): field of nsrc point sources at ntrg targets; ef(3,ntrg). */
+int ddx_efld(ddx_handle *h, int nsrc, const double *src, const double *csrc,
+             int ntrg, const double *ctrg, double *ef);
+
+/* operator layer (matvec.f90): lx (:94), lstarx (:161), ldm1x (:246),
+ * hnorm (:281).  x, y are (nylm, nsph). */
+int ddx_lx(ddx_handle *h, const double *x, double *y);
+int ddx_lstarx(ddx_handle *h, const double *x, double *y);
+int ddx_ldm1x(ddx_handle *h, const double *x, double *y);
+int ddx_hnorm(ddx_handle *h, const double *x, double *norm);
+
+/* jacobi_diis (jacobi_diis.f90:43) with norm = 4 (hnorm); the matvec
+ * callback of the reference is the device operator selected by star
+ * (0: lx, 1: lstarx) and dm1vec is ldm1x.  x holds the guess on entry.
+ * n_iter: in = maximum iterations, out = iterations done; ok = converged. */
+int ddx_jacobi_diis(ddx_handle *h, int star, int diis_max, double tol,
+                    const double *rhs, double *x, int *n_iter, int *ok);
+
+/* residual history of the last solve (star 0 / 1): up to cap norms */
+int ddx_get_history(const ddx_handle *h, int star, int cap, double *norms,
+                    int *count);
+
+/* The reference driver's whole calling sequence for point charges
+ * (main.f90:164-265): mkrhs, direct solve + energy, and with igrad the
+ * adjoint solve, forces_dd, zeta and both field contractions.
+ * esolv in hartree; fx(3,nsph) in hartree/bohr (may be NULL if igrad = 0). */
+int ddx_energy_forces(ddx_handle *h, const double *charge, double *esolv,
+                      double *fx);
+
+/* Device-resident path (no counterpart in the reference: it splits
+ * ddx_energy_forces so that a caller can keep inputs in HBM).
+ * ddx_upload copies coordinates (bohr), radii and charges to the device;
+ * ddx_run repeats setup (neighbours, fi/ui/zi, cavity), RHS, both solves
+ * and the forces from them; ddx_result copies E and F back. */
+int ddx_upload(ddx_handle *h, const double *x, const double *y,
+               const double *z, const double *rvdw, const double *charge);
+int ddx_run(ddx_handle *h);
+int ddx_result(ddx_handle *h, double *esolv, double *fx);
+
+int ddx_set_precision(ddx_handle *h, int prec);
+int ddx_get_stats(const ddx_handle *h, ddx_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
