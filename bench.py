@@ -1,0 +1,131 @@
+"""Energy + forces ddCOSMO solves per second on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload c2] [--precision fp64]
+
+One step is one complete solve from coordinates already resident in HBM:
+device setup (neighbours, fi/ui/zi, cavity, triple lists), RHS (solute
+potential at the cavity points, projection), direct Jacobi/DIIS solve and
+energy, adjoint solve, analytic forces with zeta and both field terms.  The
+workload is the 2000-atom synthetic system (lmax 8, 590 Lebedev points,
+eps 78.39, eta 0.1, convergence 1e-8).
+
+Beside the GPU figure the reference's own OpenMP Fortran program (compiled
+from its sources into oracle/_ref/main.exe) is timed on the host cores on
+one solve of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "E+forces solves/s at conv 1e-8: mixed+learned vs plain FP64 GPU; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def cpu_baseline(d, cores, timeout=300):
+    """One solve of the workload by the reference program on `cores` threads."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "main.exe")
+    if not os.path.exists(exe):
+        return None
+    from ddpcm_amd import inputs
+    tmp = tempfile.mkdtemp(prefix="ddpcm_ref_")
+    try:
+        dd = inputs.DDInput(1, cores, d.lmax, d.ngrid, d.iconv, d.igrad, d.eps, d.eta,
+                            d.charge, d.xyz, d.rvdw)
+        inputs.write_input(os.path.join(tmp, "Input.txt"), dd)
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+        t0 = time.perf_counter()
+        r = subprocess.run([exe], cwd=tmp, capture_output=True, text=True, timeout=timeout,
+                           env=env)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            return None
+        m = re.search(r"solvation energy \(kcal/mol\):\s*([-0-9.Ee+]+)", r.stdout)
+        e_kcal = float(m.group(1)) if m else None
+        return {"value": 1.0 / wall, "unit": "solves/s", "cores": cores, "kind": "reference",
+                "sample": "1 E+F solve of the %d-atom workload by the reference program "
+                          "(oracle/_ref/main.exe, flang -O3 -fopenmp, nproc=%d), process wall "
+                          "time %.2f s" % (d.n, cores, wall),
+                "energy_kcal": e_kcal}
+    except subprocess.TimeoutExpired:
+        return None
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--cpu-cores", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X; this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from ddpcm_amd import inputs
+    from ddpcm_amd.ddx import DDCosmo
+
+    d = inputs.config_input(args.workload)
+    x, y, z, r = d.bohr()
+    h = DDCosmo.from_input(d)
+    h.upload(x, y, z, r, d.charge)
+    for _ in range(args.warmup):
+        h.run()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        h.run()
+    t1 = time.perf_counter()
+    es, fx = h.result()
+    st = h.stats()
+    sec = (t1 - t0) / args.steps
+
+    parity = None
+    gpath = os.path.join(ROOT, "tests", "golden", args.workload + ".npz")
+    if os.path.exists(gpath) and args.workload in ("c2",):
+        with np.load(gpath, allow_pickle=False) as g:
+            parity = {"energy_rel_err": abs(es - float(g["esolv"])) / abs(float(g["esolv"])),
+                      "force_max_abs_err": float(np.max(np.abs(fx - g["fx"]))),
+                      "iters_match": bool(st.niter_direct == int(g["niter_direct"]) and
+                                          st.niter_adjoint == int(g["niter_adjoint"]))}
+
+    out = {
+        "metric": METRIC, "value": 1.0 / sec, "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "%s: %d synthetic atoms, lmax %d, %d Lebedev points, eps %.2f, "
+                               "eta %.2f, conv 1e-%d, energy + forces" % (
+                                   args.workload, d.n, d.lmax, h.ngrid, d.eps, d.eta, d.iconv),
+                   "precision": args.precision, "nsph": d.n, "ncav": h.ncav,
+                   "ntriples": st.ntriples},
+        "stages_ms": {"setup": st.ms_setup, "rhs": st.ms_rhs, "direct": st.ms_direct,
+                      "adjoint": st.ms_adjoint, "forces": st.ms_forces},
+        "iterations": {"direct": st.niter_direct, "adjoint": st.niter_adjoint},
+        "energy_hartree": es, "parity_vs_reference": parity,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(d, args.cpu_cores)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -12,10 +12,13 @@ int scan_exclusive(const int *in, int *out, size_t n, DevArray<int> &tmp,
 // k_setup.hip — ddinit on the device (ddcosmo.f90:308-468)
 int run_setup(ddx_handle *h);
 
-// k_gemm.hip — C(m,j) = alpha * sum_k A(m,k) B(k,j) on FP64 MFMA
-int gemm_f64(int M, int N, int K, double alpha, const double *A, long sam,
-             long sak, const double *B, long sbk, long sbn, double *C,
-             long scm, long scn, const DiisState *gate, hipStream_t s);
+// k_gemm.hip — FP64 MFMA products with the harmonic tables
+// y(lm, i) = alpha sum_n Yw(lm, n) P(n, i); P == nullptr fuses the point
+// gather of the L sweep from the per-triple values r
+int project(ddx_handle *h, double alpha, const double *P, const double *r, double *y,
+            const DiisState *gate);
+// xi(n, i) = [w_n] sum_lm Y(lm, n) S(lm, i)
+int expand(ddx_handle *h, const double *S, double *xi, bool wscale, const DiisState *gate);
 
 // k_sweep.hip — off-diagonal operators (matvec.f90:94-235)
 int apply_lx(ddx_handle *h, const double *x, double *y, const DiisState *gate);

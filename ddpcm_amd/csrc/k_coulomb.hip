@@ -2,6 +2,9 @@
 //   potential  phi(j) = sum_i q_i / |r_j - r_i|            (mkrhs.f90:102-114)
 //   field      E(j)   = sum_i q_i (r_j - r_i) / |r_j - r_i|^3  (efld.f90:98-114)
 // Each thread owns one target; sources stream through LDS in tiles of 256.
+// 1/r comes from the hardware reciprocal square root refined by two Newton
+// steps (relative error a few ulp), which replaces an IEEE sqrt and divide
+// per pair.
 // When there are few targets (the field of the cavity charges at the nuclei)
 // the sources are split over several workgroups and the partial fields are
 // summed afterwards in a fixed order, so results are deterministic.
@@ -12,6 +15,15 @@ namespace ddx {
 namespace {
 
 constexpr int kCT = 256;
+
+// 1/sqrt(x) for x > 0: v_rsq_f64 and two Newton-Raphson steps
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = y * (1.5 - h * y * y);
+  y = y * (1.5 - h * y * y);
+  return y;
+}
 
 __global__ __launch_bounds__(kCT) void k_potential(int nsrc, const double *__restrict__ q,
                                                    const double4 *__restrict__ src, int ntrg,
@@ -38,7 +50,7 @@ __global__ __launch_bounds__(kCT) void k_potential(int nsrc, const double *__res
       double4 s = tile[k];
       double dx = tx - s.x, dy = ty - s.y, dz = tz - s.z;
       double d2 = dx * dx + dy * dy + dz * dz;
-      v = v + s.w / sqrt(d2);
+      v = v + s.w * rsqrt_nr(d2);
     }
     __syncthreads();
   }
@@ -72,8 +84,8 @@ __global__ __launch_bounds__(kCT) void k_field(int nsrc, const double *__restric
       double4 s = tile[k];
       double dx = tx - s.x, dy = ty - s.y, dz = tz - s.z;
       double r2 = dx * dx + dy * dy + dz * dz;
-      double r3 = r2 * sqrt(r2);
-      double f = s.w / r3;
+      double y = rsqrt_nr(r2);
+      double f = s.w * (y * y * y);
       e0 = e0 + f * dx;
       e1 = e1 + f * dy;
       e2 = e2 + f * dz;

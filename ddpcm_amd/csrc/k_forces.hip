@@ -378,8 +378,7 @@ int forces_dd(ddx_handle *h, const double *d_phi, const double *d_sigma, const d
   hipStream_t s = h->stream;
   int n = h->n;
   DDX_TRY(h->xi.reserve(h->np));
-  DDX_TRY(gemm_f64(h->ngrid, n, h->nylm, 1.0, h->Y.p, h->nylm, 1, d_s, 1, h->nylm, h->xi.p, 1,
-                   h->ngrid, nullptr, s));
+  DDX_TRY(expand(h, d_s, h->xi.p, false, nullptr));
   DDX_TRY(h->r.reserve(4 * (size_t)h->ntri + 4));
   double4 *fb = reinterpret_cast<double4 *>(h->r.p);
   DDX_TRY(h->fpart.reserve(6 * (size_t)n + 6));
@@ -405,8 +404,7 @@ int forces_dd(ddx_handle *h, const double *d_phi, const double *d_sigma, const d
 int make_zeta(ddx_handle *h, const double *d_s, double *d_zeta) {
   hipStream_t s = h->stream;
   DDX_TRY(h->xi.reserve(h->np));
-  DDX_TRY(gemm_f64(h->ngrid, h->n, h->nylm, 1.0, h->Y.p, h->nylm, 1, d_s, 1, h->nylm, h->xi.p, 1,
-                   h->ngrid, nullptr, s));
+  DDX_TRY(expand(h, d_s, h->xi.p, false, nullptr));
   double scale = 0.5 * ((h->eps - 1.0) / h->eps);
   hipLaunchKernelGGL(k_zeta, dim3((h->np + 255) / 256), dim3(256), 0, s, h->np, h->ngrid, scale,
                      h->grid.p, h->ui.p, h->cavidx.p, h->xi.p, d_zeta);

@@ -1,13 +1,20 @@
-// FP64 matrix products on the gfx950 matrix cores (v_mfma_f64_16x16x4_f64).
+// FP64 matrix products on the gfx950 matrix cores (v_mfma_f64_16x16x4_f64),
+// shaped for the two products of the ddCOSMO operators:
 //
-// C(m, j) = alpha * sum_k A(m, k) B(k, j) with arbitrary element strides, so
-// one kernel serves
-//   the projection onto harmonics  y(lm, i) = -sum_n w_n Y(lm, n) P(n, i)
-//     (intrhs, ddcosmo.f90:789-821, applied to every sphere at once), and
-//   the expansion on the grid     xi(n, i) = sum_lm Y(lm, n) S(lm, i)
-//     (matvec.f90:196-204, forces_dd.f90:108-113).
-// Tile 64 x 64 per 256-thread workgroup, K staged through LDS 16 at a time;
-// each wave owns a 32 x 32 quadrant = 2 x 2 MFMA 16x16 blocks.
+//   projection   y(lm, i) = alpha * sum_n Yw(lm, n) P(n, i)
+//                (intrhs, ddcosmo.f90:789-821, for every sphere at once;
+//                 Yw = basis * w).  P is either a dense (ngrid, nsph) array
+//                 or, fused, the point values of the L sweep gathered from
+//                 the triples: P(n, i) = sum_{t at (i, n)} W_t r_t where
+//                 ui(n, i) < 1, else 0 (calcv, ddcosmo.f90:1695-1741).
+//   expansion    xi(n, i) = [w_n] * sum_lm Y(lm, n) S(lm, i)
+//                (matvec.f90:196-204, forces_dd.f90:108-113).
+//
+// Projection: one workgroup per 16 spheres holds all nylm rows (up to 8
+// MFMA row tiles per pass); its four waves split the ngrid reduction and
+// are summed in LDS in a fixed order.  Expansion: one wave per 16 x 16
+// output tile.  Operands go straight from L2 to registers: Y (<= 1.2 MB at
+// lmax 10 / 1202 points) stays cache-resident across workgroups.
 //
 // Operand layout of v_mfma_f64_16x16x4_f64 (one double per lane):
 //   A(i, k) in lane i + 16 k, B(k, j) in lane j + 16 k,
@@ -19,79 +26,135 @@
 namespace ddx {
 namespace {
 
-constexpr int kTM = 64, kTN = 64, kTK = 16;
+constexpr int kMaxTiles = 8;   // row tiles of 16 per projection pass
 
-__global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, double alpha,
-                                                 const double *__restrict__ A, long sam, long sak,
-                                                 const double *__restrict__ B, long sbk, long sbn,
-                                                 double *__restrict__ C, long scm, long scn,
-                                                 const DiisState *__restrict__ gate) {
+template <bool FUSED>
+__global__ __launch_bounds__(256) void k_project(
+    int nylm, int ngrid, int nsph, double alpha, const double *__restrict__ Yw,
+    const double *__restrict__ P, const int *__restrict__ ptoff,
+    const int *__restrict__ pttri, const double2 *__restrict__ ttw,
+    const double *__restrict__ r, const double *__restrict__ ui, double *__restrict__ y,
+    const DiisState *__restrict__ gate) {
   if (gate && gate->done) return;
-  __shared__ double As[kTK][kTM + 1];
-  __shared__ double Bs[kTK][kTN + 1];
-  const int m0 = blockIdx.x * kTM, n0 = blockIdx.y * kTN;
-  const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
-  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-  const bool a_kfast = (sak == 1);
-  const bool b_kfast = (sbk == 1);
-  v4d acc[2][2];
+  __shared__ v4d red[3][kMaxTiles][kWave];
+  const int lane = lane_id(), wave = threadIdx.x / kWave;
+  const int i0 = blockIdx.x * 16;
+  const int col = lane & 15, krow = lane >> 4;
+  const int sph = i0 + col;
+  const int ntile = (nylm + 15) / 16;
+  for (int t0 = 0; t0 < ntile; t0 += kMaxTiles) {
+    const int nt = min(kMaxTiles, ntile - t0);
+    v4d acc[kMaxTiles];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+    for (int t = 0; t < kMaxTiles; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 4 * wave; k0 < ngrid; k0 += 16) {
+      const int n = k0 + krow;
+      double b = 0.0;
+      if (n < ngrid && sph < nsph) {
+        const size_t p = (size_t)sph * ngrid + n;
+        if (FUSED) {
+          if (ui[p] < 1.0) {
+            for (int k = ptoff[p]; k < ptoff[p + 1]; ++k) {
+              const int tr = pttri[k];
+              b += ttw[tr].y * r[tr];
+            }
+          }
+        } else {
+          b = P[p];
+        }
+      }
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
-
-  for (int k0 = 0; k0 < K; k0 += kTK) {
-#pragma unroll
-    for (int e = tid; e < kTK * kTM; e += 256) {
-      int kk, mm;
-      if (a_kfast) { kk = e % kTK; mm = e / kTK; }
-      else { kk = e / kTM; mm = e % kTM; }
-      int gm = m0 + mm, gk = k0 + kk;
-      As[kk][mm] = (gm < M && gk < K) ? A[gm * sam + gk * sak] : 0.0;
+      for (int t = 0; t < kMaxTiles; ++t) {
+        if (t < nt) {
+          const int m = 16 * (t0 + t) + (lane & 15);
+          const double a = (m < nylm && n < ngrid) ? Yw[m + (size_t)nylm * n] : 0.0;
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+        }
+      }
     }
+    if (wave > 0) {
 #pragma unroll
-    for (int e = tid; e < kTK * kTN; e += 256) {
-      int kk, nn;
-      if (b_kfast) { kk = e % kTK; nn = e / kTK; }
-      else { kk = e / kTN; nn = e % kTN; }
-      int gn = n0 + nn, gk = k0 + kk;
-      Bs[kk][nn] = (gn < N && gk < K) ? B[gk * sbk + gn * sbn] : 0.0;
+      for (int t = 0; t < kMaxTiles; ++t)
+        if (t < nt) red[wave - 1][t][lane] = acc[t];
     }
     __syncthreads();
+    if (wave == 0 && sph < nsph) {
 #pragma unroll
-    for (int kk = 0; kk < kTK; kk += 4) {
-      const int r = lane & 15, kq = kk + (lane >> 4);
-      double a0 = As[kq][wm + r], a1 = As[kq][wm + 16 + r];
-      double b0 = Bs[kq][wn + r], b1 = Bs[kq][wn + 16 + r];
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      for (int t = 0; t < kMaxTiles; ++t) {
+        if (t < nt) {
+          v4d s = acc[t] + red[0][t][lane];
+          s = s + red[1][t][lane];
+          s = s + red[2][t][lane];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int m = 16 * (t0 + t) + krow + 4 * q;
+            if (m < nylm) y[m + (size_t)nylm * sph] = alpha * s[q];
+          }
+        }
+      }
     }
     __syncthreads();
   }
+}
+
+// xi(n, i) = scale_n * sum_lm Y(lm, n) S(lm, i); one wave per 16 x 16 tile,
+// blockIdx.x over point tiles of 64, blockIdx.y over sphere tiles of 16.
+__global__ __launch_bounds__(256) void k_expand(int nylm, int ngrid, int nsph,
+                                                const double *__restrict__ Y,
+                                                const double *__restrict__ S,
+                                                const double4 *__restrict__ grid, int wscale,
+                                                double *__restrict__ xi,
+                                                const DiisState *__restrict__ gate) {
+  if (gate && gate->done) return;
+  const int lane = lane_id(), wave = threadIdx.x / kWave;
+  const int n0 = blockIdx.x * 64 + 16 * wave;
+  const int i0 = blockIdx.y * 16;
+  if (n0 >= ngrid) return;
+  const int row = lane & 15, kq = lane >> 4;
+  const int na = n0 + row;          // A row: grid point
+  const int sb = i0 + row;          // B column: sphere
+  v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < nylm; k0 += 4) {
+    const int k = k0 + kq;
+    const double a = (na < ngrid && k < nylm) ? Y[k + (size_t)nylm * na] : 0.0;
+    const double b = (sb < nsph && k < nylm) ? S[k + (size_t)nylm * sb] : 0.0;
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  }
+  const int sph = i0 + row;
+  if (sph >= nsph) return;
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int gj = n0 + wn + b * 16 + (lane & 15);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int gi = m0 + wm + a * 16 + (lane >> 4) + 4 * q;
-        if (gi < M && gj < N) C[gi * scm + gj * scn] = alpha * acc[a][b][q];
-      }
+  for (int q = 0; q < 4; ++q) {
+    const int n = n0 + kq + 4 * q;
+    if (n < ngrid) {
+      double v = acc[q];
+      if (wscale) v *= grid[n].w;
+      xi[n + (size_t)ngrid * sph] = v;
     }
+  }
 }
 
 }  // namespace
 
-int gemm_f64(int M, int N, int K, double alpha, const double *A, long sam, long sak,
-             const double *B, long sbk, long sbn, double *C, long scm, long scn,
-             const DiisState *gate, hipStream_t s) {
-  if (M <= 0 || N <= 0) return DDX_OK;
-  dim3 g((M + kTM - 1) / kTM, (N + kTN - 1) / kTN);
-  hipLaunchKernelGGL(k_gemm_f64, g, dim3(256), 0, s, M, N, K, alpha, A, sam, sak, B, sbk, sbn,
-                     C, scm, scn, gate);
+int project(ddx_handle *h, double alpha, const double *P, const double *r, double *y,
+            const DiisState *gate) {
+  if (h->n <= 0) return DDX_OK;
+  dim3 g((h->n + 15) / 16);
+  if (P)
+    hipLaunchKernelGGL(k_project<false>, g, dim3(256), 0, h->stream, h->nylm, h->ngrid, h->n,
+                       alpha, h->Yw.p, P, nullptr, nullptr, nullptr, nullptr, nullptr, y, gate);
+  else
+    hipLaunchKernelGGL(k_project<true>, g, dim3(256), 0, h->stream, h->nylm, h->ngrid, h->n,
+                       alpha, h->Yw.p, nullptr, h->ptoff.p, h->pttri.p, h->ttw.p, r, h->ui.p, y,
+                       gate);
+  DDX_HIP(hipGetLastError());
+  return DDX_OK;
+}
+
+int expand(ddx_handle *h, const double *S, double *xi, bool wscale, const DiisState *gate) {
+  if (h->n <= 0) return DDX_OK;
+  dim3 g((h->ngrid + 63) / 64, (h->n + 15) / 16);
+  hipLaunchKernelGGL(k_expand, g, dim3(256), 0, h->stream, h->nylm, h->ngrid, h->n, h->Y.p, S,
+                     h->grid.p, wscale ? 1 : 0, xi, gate);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }

@@ -139,75 +139,100 @@ __global__ __launch_bounds__(kWave) void k_diis_small(int nblk, int diis_max,
                                                       DiisState *__restrict__ st,
                                                       int *__restrict__ newslot_out) {
   if (st->done) return;
+  constexpr int LB = kDiisMax + 1;
   __shared__ double dots[kDiisMax + 1];
-  __shared__ double a[(kDiisMax + 1) * (kDiisMax + 1)];
-  __shared__ double bv[kDiisMax + 1];
-  __shared__ double dj[kDiisMax + 1];
-  __shared__ int piv[kDiisMax + 1];
-  __shared__ int slot_new[kDiisMax];
+  __shared__ double sb[LB * LB];          // B (1-based in the reference), 0-based here
+  __shared__ double a[LB * LB];
+  __shared__ double bv[LB];
+  __shared__ double dj[LB];
+  __shared__ int piv[LB];
+  __shared__ int slots[kDiisMax];
   const int lane = threadIdx.x;
   const int lenb = diis_max + 1;
   int nmat = st->nmat;
   const int off = (nmat >= diis_max) ? 10 : 0;
   const int ndot = nmat - off - 1;
-  // finish the Gram-row dot products (fixed block order)
+  // Gram-row dot products from the per-block partial sums (fixed order)
   if (lane <= kDiisMax && (lane < ndot || lane == kDiisMax)) {
     double s = 0.0;
     for (int b = 0; b < nblk; ++b) s += part[(size_t)b * kPartStride + lane];
     dots[lane] = s;
   }
+  for (int idx = lane; idx < lenb * lenb; idx += kWave) sb[idx] = st->b[idx];
+  for (int j = lane; j < diis_max; j += kWave) slots[j] = st->slot[j];
   __syncthreads();
-  if (lane == 0) *newslot_out = st->slot[nmat - 1];
-  double *B = st->b;
-#define BB(r, c) B[((r) - 1) + ((c) - 1) * lenb]
+  if (lane == 0) *newslot_out = slots[nmat - 1];
+#define SB(r, c) sb[((r) - 1) + ((c) - 1) * lenb]
   if (nmat >= diis_max) {
-    // drop the 10 oldest vectors (jacobi_diis.f90:216-227): shift B, relabel slots
-    if (lane == 0) {
-      for (int j = 2; j <= nmat - 10; ++j)
-        for (int k = 2; k <= nmat - 10; ++k) BB(j, k) = BB(j + 10, k + 10);
-      for (int j = 0; j < diis_max; ++j) slot_new[j] = st->slot[j];
-      for (int j = 1; j <= nmat - 10; ++j) st->slot[j - 1] = slot_new[j + 9];
-      for (int j = nmat - 10 + 1; j <= diis_max; ++j) st->slot[j - 1] = slot_new[j - (nmat - 10) - 1];
-      st->nmat = nmat - 10;
+    // drop the 10 oldest vectors (jacobi_diis.f90:216-227).  All sources of
+    // the B shift lie at indices >= 12 and are read before any of them is
+    // overwritten in the reference's loop order, so a gather-then-scatter
+    // copy is the same operation.
+    const int m = nmat - 10;
+    double v[8];
+    int cnt = 0;
+    for (int idx = lane; idx < (m - 1) * (m - 1); idx += kWave) {
+      int j = 2 + idx % (m - 1), k = 2 + idx / (m - 1);
+      v[cnt++] = SB(j + 10, k + 10);
     }
-    nmat -= 10;
+    __syncthreads();
+    cnt = 0;
+    for (int idx = lane; idx < (m - 1) * (m - 1); idx += kWave) {
+      int j = 2 + idx % (m - 1), k = 2 + idx / (m - 1);
+      SB(j, k) = v[cnt++];
+    }
+    int sl = 0;
+    if (lane < diis_max) {
+      int j = lane + 1;
+      sl = (j <= m) ? slots[j + 9] : slots[j - m - 1];
+    }
+    __syncthreads();
+    if (lane < diis_max) slots[lane] = sl;
+    nmat = m;
     __syncthreads();
   }
   // makeb (jacobi_diis.f90:258-297)
-  if (lane == 0) {
-    if (nmat == 1) {
-      BB(1, 1) = 0.0;
-      BB(1, 2) = 1.0;
-      BB(2, 1) = 1.0;
-      BB(2, 2) = dots[kDiisMax];
-    } else {
-      BB(nmat + 1, 1) = 1.0;
-      BB(1, nmat + 1) = 1.0;
-      for (int i = 1; i <= nmat - 1; ++i) {
-        BB(nmat + 1, i + 1) = dots[i - 1];
-        BB(i + 1, nmat + 1) = dots[i - 1];
-      }
-      BB(nmat + 1, nmat + 1) = dots[kDiisMax];
+  if (nmat == 1) {
+    if (lane == 0) {
+      SB(1, 1) = 0.0;
+      SB(1, 2) = 1.0;
+      SB(2, 1) = 1.0;
+      SB(2, 2) = dots[kDiisMax];
+    }
+  } else {
+    if (lane == 0) {
+      SB(nmat + 1, 1) = 1.0;
+      SB(1, nmat + 1) = 1.0;
+      SB(nmat + 1, nmat + 1) = dots[kDiisMax];
+    }
+    for (int i = 1 + lane; i <= nmat - 1; i += kWave) {
+      SB(nmat + 1, i + 1) = dots[i - 1];
+      SB(i + 1, nmat + 1) = dots[i - 1];
     }
   }
   __syncthreads();
   const int n1 = nmat + 1;
   for (int idx = lane; idx < n1 * n1; idx += kWave) {
     int r = idx % n1, c = idx / n1;
-    a[idx] = BB(r + 1, c + 1);
+    a[idx] = SB(r + 1, c + 1);
   }
   for (int j = lane; j < n1; j += kWave) bv[j] = (j == 0) ? 1.0 : 0.0;
-#undef BB
+#undef SB
   __syncthreads();
   int ok = gj_solve(n1, a, bv, piv, dj);
   __syncthreads();
+  // write back B and the slot map
+  for (int idx = lane; idx < lenb * lenb; idx += kWave) st->b[idx] = sb[idx];
+  for (int j = lane; j < diis_max; j += kWave) st->slot[j] = slots[j];
+  if (ok) {
+    for (int i = 1 + lane; i <= nmat; i += kWave) st->coef[i - 1] = bv[i];
+  }
   if (lane == 0) {
     if (!ok) {
       st->nmat = 1;
       st->ncoef = 0;
       st->singular += 1;
     } else {
-      for (int i = 1; i <= nmat; ++i) st->coef[i - 1] = bv[i];
       st->ncoef = nmat;
       st->nmat = nmat + 1;
     }
@@ -378,8 +403,7 @@ int build_rhs(ddx_handle *h, const double *d_phi, double *d_rhs) {
                      h->ui.p, h->cavidx.p, d_phi, h->P.p);
   DDX_HIP(hipGetLastError());
   // rhs(lm, i) = sum_n Y(lm, n) w_n g(n, i)   (intrhs per sphere, cosmo.f90:114-116)
-  return gemm_f64(h->nylm, h->n, h->ngrid, 1.0, h->Yw.p, 1, h->nylm, h->P.p, 1, h->ngrid, d_rhs,
-                  1, h->nylm, nullptr, h->stream);
+  return project(h, 1.0, h->P.p, nullptr, d_rhs, nullptr);
 }
 
 int energy_dot(ddx_handle *h, const double *a, const double *b, double scale, double *d_out) {
@@ -407,7 +431,7 @@ int solve_jacobi_diis(ddx_handle *h, int star, int diis_max, double tol, const d
   }
   hipStream_t s = h->stream;
   size_t nn = (size_t)h->nylm * h->n;
-  int nbj = grid_blocks(nn, kJThreads * 8, 1024);
+  int nbj = grid_blocks(nn, kJThreads * 2, 1024);
   int nbe = grid_blocks(h->n, kJThreads / kWave, 1024);
   DDX_TRY(h->y.reserve(nn));
   DDX_TRY(h->xnew.reserve(nn));

@@ -134,28 +134,6 @@ __global__ __launch_bounds__(kLThreads) void k_sweep_contract(
   }
 }
 
-// P(n,o) = sum over the triples of point (o,n), in neighbour order, of W_t r_t;
-// zero where ui = 1 (calcv skips those points, ddcosmo.f90:1698).
-__global__ __launch_bounds__(256) void k_gather_points(size_t np, const int *__restrict__ ptoff,
-                                                       const int *__restrict__ pttri,
-                                                       const double2 *__restrict__ ttw,
-                                                       const double *__restrict__ r,
-                                                       const double *__restrict__ ui,
-                                                       double *__restrict__ P,
-                                                       const DiisState *__restrict__ gate) {
-  if (gate && gate->done) return;
-  size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np) return;
-  double s = 0.0;
-  if (ui[p] < 1.0) {
-    for (int k = ptoff[p]; k < ptoff[p + 1]; ++k) {
-      int tr = pttri[k];
-      s += ttw[tr].y * r[tr];
-    }
-  }
-  P[p] = s;
-}
-
 constexpr int kRedChunk = 32;
 
 // (L*S)_c for container c = blockIdx.x: one wave, lanes stride over the
@@ -164,8 +142,8 @@ template <int LMAX>
 __global__ __launch_bounds__(kWave) void k_sweep_adjoint(
     const int *__restrict__ inl, const int *__restrict__ pairoff,
     const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
-    const int *__restrict__ tp, int ngrid, const double4 *__restrict__ grid,
-    const double *__restrict__ xi, const double *__restrict__ cf, double *__restrict__ y,
+    const int *__restrict__ tp, const double *__restrict__ xiw,
+    const double *__restrict__ cf, double *__restrict__ y,
     const DiisState *__restrict__ gate) {
   if (gate && gate->done) return;
   constexpr int NY = (LMAX + 1) * (LMAX + 1);
@@ -178,9 +156,7 @@ __global__ __launch_bounds__(kWave) void k_sweep_adjoint(
   for (int k = beg + lane; k < end; k += kWave) {
     double4 g = tgeo[k];
     double2 tw = ttw[k];
-    int p = tp[k];
-    int n = p % ngrid;
-    double coef = grid[n].w * xi[p] * tw.y;
+    double coef = xiw[tp[k]] * tw.y;
     accum_R<LMAX>(g.x, g.y, g.z, g.w, tw.x, coef, cf, acc);
   }
   double *yc = y + (size_t)c * NY;
@@ -210,8 +186,7 @@ int launch_contract(ddx_handle *h, const double *X, double *r, const DiisState *
 template <int LMAX>
 int launch_adjoint(ddx_handle *h, const double *xi, double *y, const DiisState *gate) {
   hipLaunchKernelGGL(k_sweep_adjoint<LMAX>, dim3(h->n), dim3(kWave), 0, h->stream, h->inl.p,
-                     h->pairoff.p, h->tgeo.p, h->ttw.p, h->tp.p, h->ngrid, h->grid.p, xi,
-                     h->cf.p, y, gate);
+                     h->pairoff.p, h->tgeo.p, h->ttw.p, h->tp.p, xi, h->cf.p, y, gate);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
@@ -239,21 +214,15 @@ int sweep_contract(ddx_handle *h, const double *X, double *r, const DiisState *g
 
 int apply_lx(ddx_handle *h, const double *x, double *y, const DiisState *gate) {
   DDX_TRY(h->r.reserve(h->ntri + 1));
-  DDX_TRY(h->P.reserve(h->np));
   DDX_TRY(sweep_contract(h, x, h->r.p, gate));
-  hipLaunchKernelGGL(k_gather_points, dim3((h->np + 255) / 256), dim3(256), 0, h->stream, h->np,
-                     h->ptoff.p, h->pttri.p, h->ttw.p, h->r.p, h->ui.p, h->P.p, gate);
-  DDX_HIP(hipGetLastError());
-  // y(lm, o) = - sum_n Yw(lm, n) P(n, o)
-  return gemm_f64(h->nylm, h->n, h->ngrid, -1.0, h->Yw.p, 1, h->nylm, h->P.p, 1, h->ngrid, y, 1,
-                  h->nylm, gate, h->stream);
+  // y(lm, o) = - sum_n Yw(lm, n) P(n, o), P gathered from r on the fly
+  return project(h, -1.0, nullptr, h->r.p, y, gate);
 }
 
 int apply_lstarx(ddx_handle *h, const double *x, double *y, const DiisState *gate) {
   DDX_TRY(h->xiw.reserve(h->np));
-  // xi(n, o) = sum_lm Y(lm, n) x(lm, o)
-  DDX_TRY(gemm_f64(h->ngrid, h->n, h->nylm, 1.0, h->Y.p, h->nylm, 1, x, 1, h->nylm, h->xiw.p, 1,
-                   h->ngrid, gate, h->stream));
+  // w_n xi(n, o) = w_n sum_lm Y(lm, n) x(lm, o)
+  DDX_TRY(expand(h, x, h->xiw.p, true, gate));
   if (h->n == 0) return DDX_OK;
   DDX_LMAX_CASES(launch_adjoint, h, h->xiw.p, y, gate);
 }

@@ -45,6 +45,8 @@ def _lib():
     lib.orc_efld.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp]
     lib.orc_wghpot.argtypes = [C.c_void_p, _dp, _dp]
     lib.orc_intrhs.argtypes = [C.c_void_p, _dp, _dp]
+    lib.orc_jacobi_diis.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp, C.c_int,
+                                    C.c_void_p, C.c_void_p]
     lib.orc_cosmo.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_void_p]
     lib.orc_forces_dd.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
     lib.orc_ddmkzeta.argtypes = [C.c_void_p, _dp, _dp]
