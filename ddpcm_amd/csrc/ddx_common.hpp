@@ -75,7 +75,7 @@ struct DiisState {
   int slot[kDiisMax];       // logical history index -> physical slot
   int ncoef;                // coefficients valid in coef[] (0: keep Jacobi step)
   double coef[kDiisMax];    // extrapolation weights, by logical index
-  double b[(kDiisMax + 1) * (kDiisMax + 1)];  // bordered Gram matrix, 1-based, column-major
+  double b[(kDiisMax + 1) * (kDiisMax + 1)];  // bordered Gram matrix, rows of 25, 0-based
   double norm;              // increment norm of the last iteration
   double hist[kMaxIter + 1];
 };
@@ -125,9 +125,11 @@ struct ddx_handle {
   ddx::DevArray<double2> ttw;     // (t, W)
   ddx::DevArray<int> tp;          // owner point index i*ngrid + n
   ddx::DevArray<int> pttri;       // per point: its triples in neighbour order
+  ddx::DevArray<int> tslot;       // per triple: its slot in pttri
 
   // solver workspace
   ddx::DevArray<double> x, y, rhs, xnew, P, xiw, r, xd, ed, part;
+  ddx::DevArray<int> counter;     // arrival counters of the last-workgroup reductions
   ddx::DevArray<ddx::DiisState> diis;
   ddx::DiisState *h_diis = nullptr;  // pinned host mirror
 

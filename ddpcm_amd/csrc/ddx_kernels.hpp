@@ -13,18 +13,16 @@ int scan_exclusive(const int *in, int *out, size_t n, DevArray<int> &tmp,
 int run_setup(ddx_handle *h);
 
 // k_gemm.hip — FP64 MFMA products with the harmonic tables
-// y(lm, i) = alpha sum_n Yw(lm, n) P(n, i); P == nullptr fuses the point
-// gather of the L sweep from the per-triple values r
-int project(ddx_handle *h, double alpha, const double *P, const double *r, double *y,
-            const DiisState *gate);
+// y(lm, i) = alpha sum_n Yw(lm, n) P(n, i)
+int project(ddx_handle *h, double alpha, const double *P, double *y, const DiisState *gate);
 // xi(n, i) = [w_n] sum_lm Y(lm, n) S(lm, i)
 int expand(ddx_handle *h, const double *S, double *xi, bool wscale, const DiisState *gate);
 
 // k_sweep.hip — off-diagonal operators (matvec.f90:94-235)
 int apply_lx(ddx_handle *h, const double *x, double *y, const DiisState *gate);
 int apply_lstarx(ddx_handle *h, const double *x, double *y, const DiisState *gate);
-// per-triple raw contraction r_t = sum_lm R_lm(t) X_{c,lm} (no W)
-int sweep_contract(ddx_handle *h, const double *X, double *r, const DiisState *gate);
+// per-triple W_t sum_lm R_lm(t) X_{c,lm}, stored in point-slot order
+int sweep_contract(ddx_handle *h, const double *X, double *rw, const DiisState *gate);
 
 // k_solver.hip — Jacobi/DIIS (jacobi_diis.f90:43-438), RHS, norms
 int solve_jacobi_diis(ddx_handle *h, int star, int diis_max, double tol,

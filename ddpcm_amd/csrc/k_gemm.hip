@@ -3,16 +3,14 @@
 //
 //   projection   y(lm, i) = alpha * sum_n Yw(lm, n) P(n, i)
 //                (intrhs, ddcosmo.f90:789-821, for every sphere at once;
-//                 Yw = basis * w).  P is either a dense (ngrid, nsph) array
-//                 or, fused, the point values of the L sweep gathered from
-//                 the triples: P(n, i) = sum_{t at (i, n)} W_t r_t where
-//                 ui(n, i) < 1, else 0 (calcv, ddcosmo.f90:1695-1741).
+//                 Yw = basis * w) with P dense (ngrid, nsph).
 //   expansion    xi(n, i) = [w_n] * sum_lm Y(lm, n) S(lm, i)
 //                (matvec.f90:196-204, forces_dd.f90:108-113).
 //
 // Projection: one workgroup per 16 spheres holds all nylm rows (up to 8
 // MFMA row tiles per pass); its four waves split the ngrid reduction and
-// are summed in LDS in a fixed order.  Expansion: one wave per 16 x 16
+// are summed in LDS in a fixed order; each wave loads the operands of four
+// k-steps before issuing their MFMAs, to keep loads in flight.  Expansion: one wave per 16 x 16
 // output tile.  Operands go straight from L2 to registers: Y (<= 1.2 MB at
 // lmax 10 / 1202 points) stays cache-resident across workgroups.
 //
@@ -28,13 +26,13 @@ namespace {
 
 constexpr int kMaxTiles = 8;   // row tiles of 16 per projection pass
 
-template <bool FUSED>
-__global__ __launch_bounds__(256) void k_project(
-    int nylm, int ngrid, int nsph, double alpha, const double *__restrict__ Yw,
-    const double *__restrict__ P, const int *__restrict__ ptoff,
-    const int *__restrict__ pttri, const double2 *__restrict__ ttw,
-    const double *__restrict__ r, const double *__restrict__ ui, double *__restrict__ y,
-    const DiisState *__restrict__ gate) {
+constexpr int kUnroll = 4;   // k-steps whose operands are loaded before their MFMAs
+
+__global__ __launch_bounds__(256) void k_project(int nylm, int ngrid, int nsph, double alpha,
+                                                 const double *__restrict__ Yw,
+                                                 const double *__restrict__ P,
+                                                 double *__restrict__ y,
+                                                 const DiisState *__restrict__ gate) {
   if (gate && gate->done) return;
   __shared__ v4d red[3][kMaxTiles][kWave];
   const int lane = lane_id(), wave = threadIdx.x / kWave;
@@ -42,35 +40,32 @@ __global__ __launch_bounds__(256) void k_project(
   const int col = lane & 15, krow = lane >> 4;
   const int sph = i0 + col;
   const int ntile = (nylm + 15) / 16;
+  const double *Pc = P + (size_t)min(sph, nsph - 1) * ngrid;
   for (int t0 = 0; t0 < ntile; t0 += kMaxTiles) {
     const int nt = min(kMaxTiles, ntile - t0);
     v4d acc[kMaxTiles];
 #pragma unroll
     for (int t = 0; t < kMaxTiles; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 4 * wave; k0 < ngrid; k0 += 16) {
-      const int n = k0 + krow;
-      double b = 0.0;
-      if (n < ngrid && sph < nsph) {
-        const size_t p = (size_t)sph * ngrid + n;
-        if (FUSED) {
-          if (ui[p] < 1.0) {
-            for (int k = ptoff[p]; k < ptoff[p + 1]; ++k) {
-              const int tr = pttri[k];
-              b += ttw[tr].y * r[tr];
-            }
-          }
-        } else {
-          b = P[p];
+    // wave w takes k-steps w, w+4, w+8, ... (4 points each)
+    for (int kb = 4 * wave; kb < ngrid; kb += 16 * kUnroll) {
+      double b[kUnroll];
+      double a[kUnroll][kMaxTiles];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int n = kb + 16 * u + krow;
+        const bool okn = n < ngrid;
+        b[u] = (okn && sph < nsph) ? Pc[n] : 0.0;
+#pragma unroll
+        for (int t = 0; t < kMaxTiles; ++t) {
+          const int m = 16 * (t0 + t) + col;
+          a[u][t] = (t < nt && okn && m < nylm) ? Yw[m + (size_t)nylm * n] : 0.0;
         }
       }
 #pragma unroll
-      for (int t = 0; t < kMaxTiles; ++t) {
-        if (t < nt) {
-          const int m = 16 * (t0 + t) + (lane & 15);
-          const double a = (m < nylm && n < ngrid) ? Yw[m + (size_t)nylm * n] : 0.0;
-          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
-        }
-      }
+      for (int u = 0; u < kUnroll; ++u)
+#pragma unroll
+        for (int t = 0; t < kMaxTiles; ++t)
+          if (t < nt) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][t], b[u], acc[t], 0, 0, 0);
     }
     if (wave > 0) {
 #pragma unroll
@@ -82,13 +77,13 @@ __global__ __launch_bounds__(256) void k_project(
 #pragma unroll
       for (int t = 0; t < kMaxTiles; ++t) {
         if (t < nt) {
-          v4d s = acc[t] + red[0][t][lane];
-          s = s + red[1][t][lane];
-          s = s + red[2][t][lane];
+          v4d sum = acc[t] + red[0][t][lane];
+          sum = sum + red[1][t][lane];
+          sum = sum + red[2][t][lane];
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int m = 16 * (t0 + t) + krow + 4 * q;
-            if (m < nylm) y[m + (size_t)nylm * sph] = alpha * s[q];
+            if (m < nylm) y[m + (size_t)nylm * sph] = alpha * sum[q];
           }
         }
       }
@@ -135,17 +130,10 @@ __global__ __launch_bounds__(256) void k_expand(int nylm, int ngrid, int nsph,
 
 }  // namespace
 
-int project(ddx_handle *h, double alpha, const double *P, const double *r, double *y,
-            const DiisState *gate) {
+int project(ddx_handle *h, double alpha, const double *P, double *y, const DiisState *gate) {
   if (h->n <= 0) return DDX_OK;
-  dim3 g((h->n + 15) / 16);
-  if (P)
-    hipLaunchKernelGGL(k_project<false>, g, dim3(256), 0, h->stream, h->nylm, h->ngrid, h->n,
-                       alpha, h->Yw.p, P, nullptr, nullptr, nullptr, nullptr, nullptr, y, gate);
-  else
-    hipLaunchKernelGGL(k_project<true>, g, dim3(256), 0, h->stream, h->nylm, h->ngrid, h->n,
-                       alpha, h->Yw.p, nullptr, h->ptoff.p, h->pttri.p, h->ttw.p, r, h->ui.p, y,
-                       gate);
+  hipLaunchKernelGGL(k_project, dim3((h->n + 15) / 16), dim3(256), 0, h->stream, h->nylm,
+                     h->ngrid, h->n, alpha, h->Yw.p, P, y, gate);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }

@@ -13,7 +13,8 @@
 //               s = v/|v| (ylmbas, ddcosmo.f90:846-866), t and the weight
 //               W = chi(t)/max(fi, 1) (ddcosmo.f90:1722-1730, 1340-1348).
 //               pttri lists the triples of each point in neighbour order
-//               (the loop order of calcv, ddcosmo.f90:1701-1739).
+//               (the loop order of calcv, ddcosmo.f90:1701-1739); tslot is
+//               its inverse, the slot of each triple in its point's list.
 #include "ddx_device.hpp"
 #include "ddx_kernels.hpp"
 
@@ -166,7 +167,7 @@ __global__ __launch_bounds__(kSetupThreads) void k_pair_fill(
     const unsigned long long *__restrict__ tmask, const double *__restrict__ fi,
     const int *__restrict__ pairoff, const int *__restrict__ ptoff,
     double4 *__restrict__ tgeo, double2 *__restrict__ ttw, int *__restrict__ tp,
-    int *__restrict__ pttri) {
+    int *__restrict__ pttri, int *__restrict__ tslot) {
   int q = blockIdx.x * (kSetupThreads / kWave) + threadIdx.x / kWave;
   if (q >= nnl) return;
   int c = prow[q], o = nl[q], kr = rev[q];
@@ -201,7 +202,9 @@ __global__ __launch_bounds__(kSetupThreads) void k_pair_fill(
       tgeo[idx] = make_double4(cthe, sthe, cphi, sphi);
       ttw[idx] = make_double2(t, w);
       tp[idx] = (int)p;
-      pttri[ptoff[p] + __popcll(m & below)] = idx;
+      const int slot = ptoff[p] + __popcll(m & below);
+      pttri[slot] = idx;
+      tslot[idx] = slot;
     }
     run += __popcll(b);
   }
@@ -286,12 +289,13 @@ int run_setup(ddx_handle *h) {
   DDX_TRY(h->ttw.reserve(ntri + 1));
   DDX_TRY(h->tp.reserve(ntri + 1));
   DDX_TRY(h->pttri.reserve(ntri + 1));
+  DDX_TRY(h->tslot.reserve(ntri + 1));
   if (nnl > 0) {
     int wpb = kSetupThreads / kWave;
     hipLaunchKernelGGL(k_pair_fill, dim3((nnl + wpb - 1) / wpb), dim3(kSetupThreads), 0, s,
                        nnl, ng, h->eta, h->sph.p, h->grid.p, h->nl.p, h->prow.p, h->rev.p,
                        h->tmask.p, h->fi.p, h->pairoff.p, h->ptoff.p, h->tgeo.p, h->ttw.p,
-                       h->tp.p, h->pttri.p);
+                       h->tp.p, h->pttri.p, h->tslot.p);
     DDX_HIP(hipGetLastError());
   }
   DDX_TRY(h->ccav.reserve(3 * (size_t)ncav + 3));

@@ -21,6 +21,7 @@ namespace {
 
 constexpr int kJThreads = 256;
 constexpr int kPartStride = kDiisMax + 1;   // 25 dots + self dot per block
+constexpr int kMaxBlk = 1024;               // workgroups of k_jacobi (partials per dot)
 
 __global__ __launch_bounds__(kJThreads) void k_jacobi(
     size_t nn, int nylm, int diis_max, const double *__restrict__ facl,
@@ -28,7 +29,7 @@ __global__ __launch_bounds__(kJThreads) void k_jacobi(
     double *__restrict__ xd, double *__restrict__ ed, double *__restrict__ xnew,
     double *__restrict__ part, const DiisState *__restrict__ st) {
   if (st->done) return;
-  __shared__ double lds[kJThreads / kWave];
+  __shared__ double wpart[kJThreads / kWave][kPartStride];
   const int nmat = st->nmat;
   const int off = (diis_max > 0 && nmat >= diis_max) ? 10 : 0;
   const int ndot = diis_max > 0 ? nmat - off - 1 : 0;
@@ -56,176 +57,208 @@ __global__ __launch_bounds__(kJThreads) void k_jacobi(
     }
   }
   if (diis_max == 0) return;
-  double *pb = part + (size_t)blockIdx.x * kPartStride;
+  // all 26 partial dots of the block: butterflies per wave, one barrier
+  const int wave = threadIdx.x / kWave;
 #pragma unroll
   for (int q = 0; q < kDiisMax; ++q) {
     if (q < ndot) {
-      double s = block_sum_256(acc[q], lds);
-      if (threadIdx.x == 0) pb[q] = s;
+      double v = wave_sum(acc[q]);
+      if (lane_id() == 0) wpart[wave][q] = v;
     }
   }
-  double s = block_sum_256(self, lds);
-  if (threadIdx.x == 0) pb[kDiisMax] = s;
+  self = wave_sum(self);
+  if (lane_id() == 0) wpart[wave][kDiisMax] = self;
+  __syncthreads();
+  const int q = threadIdx.x;
+  if (q < ndot || q == kDiisMax)
+    part[(size_t)q * kMaxBlk + blockIdx.x] =
+        (wpart[0][q] + wpart[1][q]) + (wpart[2][q] + wpart[3][q]);
 }
 
-// Gauss-Jordan with full pivoting on the (n1 x n1) bordered matrix in LDS,
-// right-hand side e_1 (gjinv, jacobi_diis.f90:299-400).  Pivot choice: the
-// largest |a(j,k)| over rows with piv != 1 and columns with piv == 0, ties to
-// the first in (j, k) order, as the reference's nested loops pick it.
-// Returns 1 on success; rhs holds the solution.
-__device__ int gj_solve(int n1, double *a /* n1*n1, column-major */, double *bv,
-                        int *piv, double *dj) {
-  const int lane = threadIdx.x;
-  for (int j = lane; j < n1; j += kWave) piv[j] = 0;
+// The DIIS bookkeeping of one iteration on one wave, all in registers: lane r
+// holds row r of the bordered Gram matrix B (0-based here; the reference's
+// b(r+1, c+1)), n1 = nmat + 1 <= 25 rows.
+//
+//   drop-10 rule   when nmat reached diis_max the 10 oldest vectors go: rows
+//                  and columns 11.. move to 1.. (jacobi_diis.f90:216-227)
+//   makeb          new row/column of dot products (jacobi_diis.f90:258-297)
+//   gjinv          Gauss-Jordan with full pivoting, right-hand side e_1
+//                  (jacobi_diis.f90:299-400).  Pivot: the largest |a(j,k)|
+//                  over rows j and columns k not yet pivoted (piv(j) != 1,
+//                  piv(k) == 0), first in row-major order on ties, as the
+//                  reference's nested loops pick it; the inverse's column
+//                  unscrambling does not touch the solution and is skipped.
+constexpr int kB = kDiisMax;      // register columns per row (n1 <= 25)
+
+// a[c] for a wave-uniform c, written as an exact sum (one term is non-zero)
+// so that the compiler keeps a[] in registers instead of indexing memory
+__device__ __forceinline__ double sel(const double (&a)[kB], int c) {
+  double v = 0.0;
+#pragma unroll
+  for (int k = 0; k < kB; ++k) v += (c == k) ? a[k] : 0.0;
+  return v;
+}
+
+// value of v in lane `src`; src must be wave-uniform (it is moved to an SGPR)
+__device__ __forceinline__ double lane_value(double v, int src) {
+  const int s = __builtin_amdgcn_readfirstlane(src);
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffll), s);
+  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), s);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+constexpr int kSmallThreads = 256;       // 4 workgroup partials per thread
+
+__global__ __launch_bounds__(kSmallThreads) void k_diis_small(int nblk, int diis_max,
+                                                              const double *__restrict__ part,
+                                                              DiisState *__restrict__ st,
+                                                              int *__restrict__ newslot_out) {
+#pragma clang fp contract(off)
+  if (st->done) return;
+  __shared__ double red[kSmallThreads / kWave][kPartStride];
+  const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+  int nmat = st->nmat;
+  const int off = (nmat >= diis_max) ? 10 : 0;
+  const int ndot = nmat - off - 1;
+  // Gram-row dot products: each thread sums 4 workgroup partials (loads in
+  // flight together), butterflies per wave, waves combined in a fixed order
+  constexpr int kPer = kMaxBlk / kSmallThreads;
+#pragma unroll
+  for (int q = 0; q < kPartStride; ++q) {
+    double v = 0.0;
+    if (q < ndot || q == kDiisMax) {
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const int b = tid + kSmallThreads * k;
+        if (b < nblk) v += part[(size_t)q * kMaxBlk + b];
+      }
+    }
+    v = wave_sum(v);
+    if (lane == 0) red[wave][q] = v;
+  }
   __syncthreads();
+  if (wave != 0) return;
+  double a[kB];
+#pragma unroll
+  for (int c = 0; c < kB; ++c) a[c] = (lane < kB) ? st->b[lane * kB + c] : 0.0;
+  int myslot = (lane < diis_max) ? st->slot[lane] : 0;
+  // lane i + 1 keeps dot i; every lane keeps the self dot
+  const int qi = (lane >= 1 && lane <= kDiisMax) ? lane - 1 : 0;
+  double mydot = 0.0, self = 0.0;
+  for (int w = 0; w < kSmallThreads / kWave; ++w) {
+    mydot += red[w][qi];
+    self += red[w][kDiisMax];
+  }
+  if (!(lane >= 1 && lane - 1 < ndot)) mydot = 0.0;
+  if (lane == 0) *newslot_out = st->slot[nmat - 1];
+  if (nmat >= diis_max) {
+    const int m = nmat - 10;
+    // rows 1..m-1 <- rows 11..m+9, then columns likewise
+    double moved[kB];
+#pragma unroll
+    for (int c = 0; c < kB; ++c) moved[c] = __shfl(a[c], min(lane + 10, kWave - 1), kWave);
+    if (lane >= 1 && lane <= m - 1) {
+#pragma unroll
+      for (int c = 0; c < kB; ++c) a[c] = moved[c];
+#pragma unroll
+      for (int c = 1; c + 10 < kB; ++c) a[c] = (c <= m - 1) ? a[c + 10] : a[c];
+    }
+    // slots: logical j (0-based) <- j + 10 for j < m, recycled ones after
+    int src = (lane < m) ? lane + 10 : lane - m;
+    myslot = __shfl(myslot, src < diis_max ? src : 0, kWave);
+    nmat = m;
+  }
+  // makeb: row / column nmat (0-based) of the new vector
+  if (nmat == 1) {
+    if (lane == 0) { a[0] = 0.0; a[1] = 1.0; }
+    if (lane == 1) { a[0] = 1.0; a[1] = self; }
+  } else {
+    double rowv[kB];
+#pragma unroll
+    for (int c = 0; c < kB; ++c) rowv[c] = lane_value(mydot, c);   // dot c-1
+#pragma unroll
+    for (int c = 0; c < kB; ++c) {
+      if (c == nmat) {
+        if (lane == 0) a[c] = 1.0;
+        else if (lane < nmat) a[c] = mydot;
+        else if (lane == nmat) a[c] = self;
+      }
+    }
+    if (lane == nmat) {
+      a[0] = 1.0;
+#pragma unroll
+      for (int c = 1; c < kB; ++c)
+        if (c < nmat) a[c] = rowv[c];
+    }
+  }
+  // persist B and the slot map
+  if (lane < kB) {
+#pragma unroll
+    for (int c = 0; c < kB; ++c) st->b[lane * kB + c] = a[c];
+  }
+  if (lane < diis_max) st->slot[lane] = myslot;
+
+  // Gauss-Jordan on a copy, n1 = nmat + 1; lane r holds row r
+  const int n1 = nmat + 1;
+  double g[kB];
+#pragma unroll
+  for (int c = 0; c < kB; ++c) g[c] = (lane < n1 && c < n1) ? a[c] : 0.0;
+  double bv = (lane == 0) ? 1.0 : 0.0;
+  unsigned int pivoted = 0u;   // columns already used as pivot
+  bool ok = true;
   for (int it = 0; it < n1; ++it) {
-    // parallel argmax over the linear index (j-major), first maximum wins
-    double big = 0.0;
+    double big = 0.0, bval = 0.0;
     int best = 0x7fffffff;
-    for (int idx = lane; idx < n1 * n1; idx += kWave) {
-      int j = idx / n1, k = idx % n1;
-      if (piv[j] != 1 && piv[k] == 0) {
-        double v = fabs(a[j + k * n1]);
-        if (v > big) { big = v; best = idx; }
+    if (lane < n1 && !((pivoted >> lane) & 1u)) {
+#pragma unroll
+      for (int c = 0; c < kB; ++c) {
+        if (c < n1 && !((pivoted >> c) & 1u)) {
+          double v = fabs(g[c]);
+          if (v > big) { big = v; bval = g[c]; best = lane * n1 + c; }
+        }
       }
     }
     for (int o = 32; o > 0; o >>= 1) {
       double ob = __shfl_xor(big, o, kWave);
+      double ov = __shfl_xor(bval, o, kWave);
       int oi = __shfl_xor(best, o, kWave);
-      if (ob > big || (ob == big && oi < best)) { big = ob; best = oi; }
+      if (ob > big || (ob == big && oi < best)) { big = ob; bval = ov; best = oi; }
     }
-    if (best == 0x7fffffff) return 0;   // nothing left that is non-zero
+    best = __builtin_amdgcn_readfirstlane(best);
+    if (best == 0x7fffffff) { ok = false; break; }
     const int irow = best / n1, icol = best % n1;
-    __syncthreads();
-    if (piv[icol] + 1 > 1) return 0;
-    __syncthreads();
-    if (lane == 0) piv[icol] += 1;
+    if ((pivoted >> icol) & 1u) { ok = false; break; }
+    pivoted |= 1u << icol;
     if (irow != icol) {
-      for (int c = lane; c < n1; c += kWave) {
-        double t = a[irow + c * n1];
-        a[irow + c * n1] = a[icol + c * n1];
-        a[icol + c * n1] = t;
+#pragma unroll
+      for (int c = 0; c < kB; ++c) {
+        const double ri = lane_value(g[c], irow), ci = lane_value(g[c], icol);
+        g[c] = (lane == irow) ? ci : ((lane == icol) ? ri : g[c]);
       }
-      if (lane == 0) { double t = bv[irow]; bv[irow] = bv[icol]; bv[icol] = t; }
+      const double rb = lane_value(bv, irow), cb = lane_value(bv, icol);
+      bv = (lane == irow) ? cb : ((lane == icol) ? rb : bv);
     }
-    __syncthreads();
-    double d = a[icol + icol * n1];
-    if (d == 0.0) return 0;
-    double pinv = 1.0 / d;
-    __syncthreads();
-    if (lane == 0) a[icol + icol * n1] = 1.0;
-    __syncthreads();
-    for (int c = lane; c < n1; c += kWave) a[icol + c * n1] *= pinv;
-    if (lane == 0) bv[icol] *= pinv;
-    __syncthreads();
-    for (int j = lane; j < n1; j += kWave) dj[j] = a[j + icol * n1];
-    __syncthreads();
-    for (int idx = lane; idx < n1 * n1; idx += kWave) {
-      int j = idx % n1, c = idx / n1;
-      if (j == icol) continue;
-      double aj = (c == icol) ? 0.0 : a[j + c * n1];
-      a[j + c * n1] = aj - a[icol + c * n1] * dj[j];
+    const double d = lane_value(bval, 0);   // the pivot, now in row icol
+    if (d == 0.0) { ok = false; break; }
+    const double pinv = 1.0 / d;
+    if (lane == icol) {
+#pragma unroll
+      for (int c = 0; c < kB; ++c) g[c] = ((c == icol) ? 1.0 : g[c]) * pinv;
+      bv = bv * pinv;
     }
-    __syncthreads();
-    for (int j = lane; j < n1; j += kWave)
-      if (j != icol) bv[j] = bv[j] - bv[icol] * dj[j];
-    __syncthreads();
+    // eliminate column icol from the other rows
+    const double dum = sel(g, icol);
+    const double bp = lane_value(bv, icol);
+#pragma unroll
+    for (int c = 0; c < kB; ++c) {
+      const double pc = lane_value(g[c], icol);
+      if (lane != icol) g[c] = ((c == icol) ? 0.0 : g[c]) - pc * dum;
+    }
+    if (lane != icol) bv = bv - bp * dum;
   }
-  return 1;
-}
-
-__global__ __launch_bounds__(kWave) void k_diis_small(int nblk, int diis_max,
-                                                      const double *__restrict__ part,
-                                                      DiisState *__restrict__ st,
-                                                      int *__restrict__ newslot_out) {
-  if (st->done) return;
-  constexpr int LB = kDiisMax + 1;
-  __shared__ double dots[kDiisMax + 1];
-  __shared__ double sb[LB * LB];          // B (1-based in the reference), 0-based here
-  __shared__ double a[LB * LB];
-  __shared__ double bv[LB];
-  __shared__ double dj[LB];
-  __shared__ int piv[LB];
-  __shared__ int slots[kDiisMax];
-  const int lane = threadIdx.x;
-  const int lenb = diis_max + 1;
-  int nmat = st->nmat;
-  const int off = (nmat >= diis_max) ? 10 : 0;
-  const int ndot = nmat - off - 1;
-  // Gram-row dot products from the per-block partial sums (fixed order)
-  if (lane <= kDiisMax && (lane < ndot || lane == kDiisMax)) {
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += part[(size_t)b * kPartStride + lane];
-    dots[lane] = s;
-  }
-  for (int idx = lane; idx < lenb * lenb; idx += kWave) sb[idx] = st->b[idx];
-  for (int j = lane; j < diis_max; j += kWave) slots[j] = st->slot[j];
-  __syncthreads();
-  if (lane == 0) *newslot_out = slots[nmat - 1];
-#define SB(r, c) sb[((r) - 1) + ((c) - 1) * lenb]
-  if (nmat >= diis_max) {
-    // drop the 10 oldest vectors (jacobi_diis.f90:216-227).  All sources of
-    // the B shift lie at indices >= 12 and are read before any of them is
-    // overwritten in the reference's loop order, so a gather-then-scatter
-    // copy is the same operation.
-    const int m = nmat - 10;
-    double v[8];
-    int cnt = 0;
-    for (int idx = lane; idx < (m - 1) * (m - 1); idx += kWave) {
-      int j = 2 + idx % (m - 1), k = 2 + idx / (m - 1);
-      v[cnt++] = SB(j + 10, k + 10);
-    }
-    __syncthreads();
-    cnt = 0;
-    for (int idx = lane; idx < (m - 1) * (m - 1); idx += kWave) {
-      int j = 2 + idx % (m - 1), k = 2 + idx / (m - 1);
-      SB(j, k) = v[cnt++];
-    }
-    int sl = 0;
-    if (lane < diis_max) {
-      int j = lane + 1;
-      sl = (j <= m) ? slots[j + 9] : slots[j - m - 1];
-    }
-    __syncthreads();
-    if (lane < diis_max) slots[lane] = sl;
-    nmat = m;
-    __syncthreads();
-  }
-  // makeb (jacobi_diis.f90:258-297)
-  if (nmat == 1) {
-    if (lane == 0) {
-      SB(1, 1) = 0.0;
-      SB(1, 2) = 1.0;
-      SB(2, 1) = 1.0;
-      SB(2, 2) = dots[kDiisMax];
-    }
-  } else {
-    if (lane == 0) {
-      SB(nmat + 1, 1) = 1.0;
-      SB(1, nmat + 1) = 1.0;
-      SB(nmat + 1, nmat + 1) = dots[kDiisMax];
-    }
-    for (int i = 1 + lane; i <= nmat - 1; i += kWave) {
-      SB(nmat + 1, i + 1) = dots[i - 1];
-      SB(i + 1, nmat + 1) = dots[i - 1];
-    }
-  }
-  __syncthreads();
-  const int n1 = nmat + 1;
-  for (int idx = lane; idx < n1 * n1; idx += kWave) {
-    int r = idx % n1, c = idx / n1;
-    a[idx] = SB(r + 1, c + 1);
-  }
-  for (int j = lane; j < n1; j += kWave) bv[j] = (j == 0) ? 1.0 : 0.0;
-#undef SB
-  __syncthreads();
-  int ok = gj_solve(n1, a, bv, piv, dj);
-  __syncthreads();
-  // write back B and the slot map
-  for (int idx = lane; idx < lenb * lenb; idx += kWave) st->b[idx] = sb[idx];
-  for (int j = lane; j < diis_max; j += kWave) st->slot[j] = slots[j];
   if (ok) {
-    for (int i = 1 + lane; i <= nmat; i += kWave) st->coef[i - 1] = bv[i];
+    if (lane >= 1 && lane <= nmat) st->coef[lane - 1] = bv;
   }
   if (lane == 0) {
     if (!ok) {
@@ -287,6 +320,8 @@ __global__ __launch_bounds__(kJThreads) void k_extrap(
   if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
+// rms of the per-sphere norms and the convergence test (rmsvec,
+// jacobi_diis.f90:405-438; jacobi_diis.f90:153-156)
 __global__ __launch_bounds__(kWave) void k_norm(int nblk, int nsph, double tol, int maxit,
                                                 const double *__restrict__ part,
                                                 DiisState *__restrict__ st) {
@@ -403,7 +438,7 @@ int build_rhs(ddx_handle *h, const double *d_phi, double *d_rhs) {
                      h->ui.p, h->cavidx.p, d_phi, h->P.p);
   DDX_HIP(hipGetLastError());
   // rhs(lm, i) = sum_n Y(lm, n) w_n g(n, i)   (intrhs per sphere, cosmo.f90:114-116)
-  return project(h, 1.0, h->P.p, nullptr, d_rhs, nullptr);
+  return project(h, 1.0, h->P.p, d_rhs, nullptr);
 }
 
 int energy_dot(ddx_handle *h, const double *a, const double *b, double scale, double *d_out) {
@@ -431,7 +466,7 @@ int solve_jacobi_diis(ddx_handle *h, int star, int diis_max, double tol, const d
   }
   hipStream_t s = h->stream;
   size_t nn = (size_t)h->nylm * h->n;
-  int nbj = grid_blocks(nn, kJThreads * 2, 1024);
+  int nbj = grid_blocks(nn, kJThreads, kMaxBlk);
   int nbe = grid_blocks(h->n, kJThreads / kWave, 1024);
   DDX_TRY(h->y.reserve(nn));
   DDX_TRY(h->xnew.reserve(nn));
@@ -439,7 +474,7 @@ int solve_jacobi_diis(ddx_handle *h, int star, int diis_max, double tol, const d
     DDX_TRY(h->xd.reserve(nn * diis_max));
     DDX_TRY(h->ed.reserve(nn * diis_max));
   }
-  DDX_TRY(h->part.reserve((size_t)1024 * kPartStride + 2048));
+  DDX_TRY(h->part.reserve((size_t)kMaxBlk * kPartStride + 2048));
   DDX_TRY(h->diis.reserve(2));
   DDX_TRY(h->misc.reserve(16));
   double *part_e = h->part.p + (size_t)1024 * kPartStride;
@@ -467,8 +502,8 @@ int solve_jacobi_diis(ddx_handle *h, int star, int diis_max, double tol, const d
                          h->facl.p, d_rhs, h->y.p, d_x, h->xd.p, h->ed.p, h->xnew.p, h->part.p, ds);
       DDX_HIP(hipGetLastError());
       if (diis_max > 0) {
-        hipLaunchKernelGGL(k_diis_small, dim3(1), dim3(kWave), 0, s, nbj, diis_max, h->part.p, ds,
-                           newslot);
+        hipLaunchKernelGGL(k_diis_small, dim3(1), dim3(kSmallThreads), 0, s, nbj, diis_max,
+                           h->part.p, ds, newslot);
         DDX_HIP(hipGetLastError());
       }
       hipLaunchKernelGGL(k_extrap, dim3(nbe), dim3(kJThreads), 0, s, h->n, h->nylm, nn, diis_max,

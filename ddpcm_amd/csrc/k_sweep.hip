@@ -115,12 +115,15 @@ __device__ __forceinline__ void accum_R(double cthe, double sthe, double cphi, d
 
 constexpr int kLThreads = 256;
 
-// r_t = R(t) . X_c for every triple t of container c = blockIdx.x
+// W_t R(t) . X_c for every triple t of container c = blockIdx.x, stored in
+// the triple's slot of its point's list, so that each point's terms are
+// contiguous for the gather.
 template <int LMAX>
 __global__ __launch_bounds__(kLThreads) void k_sweep_contract(
     const int *__restrict__ inl, const int *__restrict__ pairoff,
     const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
-    const double *__restrict__ X, const double *__restrict__ cf, double *__restrict__ r,
+    const int *__restrict__ tslot, const double *__restrict__ X,
+    const double *__restrict__ cf, double *__restrict__ rw,
     const DiisState *__restrict__ gate) {
   if (gate && gate->done) return;
   constexpr int NY = (LMAX + 1) * (LMAX + 1);
@@ -129,9 +132,26 @@ __global__ __launch_bounds__(kLThreads) void k_sweep_contract(
   const double *Xc = X + (size_t)c * NY;
   for (int k = beg + threadIdx.x; k < end; k += kLThreads) {
     double4 g = tgeo[k];
-    double t = ttw[k].x;
-    r[k] = contract_R<LMAX>(g.x, g.y, g.z, g.w, t, Xc, cf);
+    double2 tw = ttw[k];
+    rw[tslot[k]] = tw.y * contract_R<LMAX>(g.x, g.y, g.z, g.w, tw.x, Xc, cf);
   }
+}
+
+// P(n,o) = sum of the point's weighted terms, in neighbour order; zero where
+// ui = 1 (calcv skips those points, ddcosmo.f90:1698).
+__global__ __launch_bounds__(256) void k_gather_points(size_t np, const int *__restrict__ ptoff,
+                                                       const double *__restrict__ rw,
+                                                       const double *__restrict__ ui,
+                                                       double *__restrict__ P,
+                                                       const DiisState *__restrict__ gate) {
+  if (gate && gate->done) return;
+  size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  double s = 0.0;
+  const int b = ptoff[p], e = ptoff[p + 1];
+  if (b < e && ui[p] < 1.0)
+    for (int k = b; k < e; ++k) s += rw[k];
+  P[p] = s;
 }
 
 constexpr int kRedChunk = 32;
@@ -178,7 +198,7 @@ __global__ __launch_bounds__(kWave) void k_sweep_adjoint(
 template <int LMAX>
 int launch_contract(ddx_handle *h, const double *X, double *r, const DiisState *gate) {
   hipLaunchKernelGGL(k_sweep_contract<LMAX>, dim3(h->n), dim3(kLThreads), 0, h->stream,
-                     h->inl.p, h->pairoff.p, h->tgeo.p, h->ttw.p, X, h->cf.p, r, gate);
+                     h->inl.p, h->pairoff.p, h->tgeo.p, h->ttw.p, h->tslot.p, X, h->cf.p, r, gate);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
@@ -214,9 +234,13 @@ int sweep_contract(ddx_handle *h, const double *X, double *r, const DiisState *g
 
 int apply_lx(ddx_handle *h, const double *x, double *y, const DiisState *gate) {
   DDX_TRY(h->r.reserve(h->ntri + 1));
+  DDX_TRY(h->P.reserve(h->np));
   DDX_TRY(sweep_contract(h, x, h->r.p, gate));
-  // y(lm, o) = - sum_n Yw(lm, n) P(n, o), P gathered from r on the fly
-  return project(h, -1.0, nullptr, h->r.p, y, gate);
+  hipLaunchKernelGGL(k_gather_points, dim3((h->np + 255) / 256), dim3(256), 0, h->stream, h->np,
+                     h->ptoff.p, h->r.p, h->ui.p, h->P.p, gate);
+  DDX_HIP(hipGetLastError());
+  // y(lm, o) = - sum_n Yw(lm, n) P(n, o)
+  return project(h, -1.0, h->P.p, y, gate);
 }
 
 int apply_lstarx(ddx_handle *h, const double *x, double *y, const DiisState *gate) {
