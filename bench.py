@@ -31,8 +31,73 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "E+forces solves/s at conv 1e-8: mixed+learned vs plain FP64 GPU; HBM GB/s"
-HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64, vector and dense matrix alike
+KERNELS = ("sweep_l", "sweep_lstar", "project", "expand", "jacobi", "diis", "extrap", "field",
+           "potential", "forces")
+
+
+def diis_depths(niter, ndiis=25):
+    """(nmat after the drop-10 rule) per iteration of jacobi_diis.f90's DIIS:
+    nmat runs 1..25, then 16..25, ... (jacobi_diis.f90:216-248)."""
+    out, nmat = [], 1
+    for _ in range(niter):
+        if nmat >= ndiis:
+            nmat -= 10
+        out.append(nmat)
+        nmat += 1
+    return out
+
+
+def algorithmic_work(h, st, d):
+    """Per kernel class: (unit, work per solve, launches per solve).  Counting
+    convention: sweeps T (7 nylm + 40) flops per apply; MFMA products
+    2 nylm ngrid N flops; all-pairs 20 ncav N flops per kernel; Jacobi update
+    (4 + m) n 8 bytes (rhs, y, x, m - 1 history residuals read; x_new, e
+    written) and extrapolation (m + 2) n 8 bytes at DIIS depth m."""
+    n = h.nylm * h.nsph
+    T = st.ntriples
+    sweep = T * (7.0 * h.nylm + 40.0)
+    mm = 2.0 * h.nylm * h.ngrid * h.nsph
+    nd, na = st.niter_direct, st.niter_adjoint
+    depth = diis_depths(nd) + diis_depths(na)
+    jac = sum((4 + m) * n * 8.0 for m in depth)
+    ext = sum((m + 2) * n * 8.0 for m in depth)
+    pairs = 20.0 * h.ncav * h.nsph
+    return {
+        "sweep_l": ("flop", sweep * nd, nd),
+        "sweep_lstar": ("flop", sweep * na, na),
+        "project": ("flop", mm * (nd + 1), nd + 1),
+        "expand": ("flop", mm * (na + 2), na + 2),
+        "jacobi": ("byte", jac, nd + na),
+        "extrap": ("byte", ext, nd + na),
+        "field": ("flop", 2 * pairs, 2),
+        "potential": ("flop", pairs, 1),
+    }
+
+
+def kernel_table(h, st, d):
+    """One solve with every kernel class timed: per-class launches, time and
+    achieved rate against its roofline."""
+    for k in KERNELS:
+        h.kernel_timer(k, True)
+    h.run()
+    work = algorithmic_work(h, h.stats(), d)
+    table = {}
+    for k in KERNELS:
+        n, ms = h.kernel_timer_read(k)
+        h.kernel_timer(k, False)
+        row = {"launches": n, "ms": ms}
+        if k in work and ms > 0:
+            unit, w, _ = work[k]
+            if unit == "flop":
+                row["tflops"] = w / (ms * 1e-3) / 1e12
+                row["frac_fp64_peak"] = row["tflops"] / FP64_PEAK_TFLOPS
+            else:
+                row["gbs"] = w / (ms * 1e-3) / 1e9
+                row["frac_hbm_peak"] = row["gbs"] / HBM_PEAK_GBS
+        table[k] = row
+    return table
 
 
 def cpu_baseline(d, cores, timeout=300):
@@ -90,13 +155,38 @@ def main():
     h.upload(x, y, z, r, d.charge)
     for _ in range(args.warmup):
         h.run()
+    # which kernel class dominates one solve
+    table = kernel_table(h, h.stats(), d)
+    dominant = max(table, key=lambda k: table[k]["ms"])
+    # timed region; the dominant class is bracketed by HIP events on the
+    # library's stream, launch by launch
+    h.kernel_timer(dominant, True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         h.run()
     t1 = time.perf_counter()
+    launches, kms = h.kernel_timer_read(dominant)
+    h.kernel_timer(dominant, False)
     es, fx = h.result()
     st = h.stats()
     sec = (t1 - t0) / args.steps
+    work = algorithmic_work(h, st, d)
+    roofline = None
+    if dominant in work and launches > 0:
+        unit, w, per_solve = work[dominant]
+        per_launch = w / per_solve
+        avg_s = kms * 1e-3 / launches
+        if unit == "flop":
+            ach = per_launch / avg_s / 1e12
+            roofline = {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None}
+        else:
+            ach = per_launch / avg_s / 1e9
+            roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK_GBS, "traffic": None}
+        roofline.update({"kernel": dominant, "launches": launches,
+                         "avg_launch_us": avg_s * 1e6, "work_per_launch": per_launch,
+                         "work_unit": unit})
 
     parity = None
     gpath = os.path.join(ROOT, "tests", "golden", args.workload + ".npz")
@@ -121,6 +211,9 @@ def main():
                       "adjoint": st.ms_adjoint, "forces": st.ms_forces},
         "iterations": {"direct": st.niter_direct, "adjoint": st.niter_adjoint},
         "energy_hartree": es, "parity_vs_reference": parity,
+        "roofline": roofline,
+        "kernels": table,
+        "hbm_gbs": {k: table[k].get("gbs") for k in ("jacobi", "extrap")},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(d, args.cpu_cores)

@@ -246,6 +246,8 @@ int ddx_free(ddx_handle *h) {
   if (!h) return DDX_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->h_diis) (void)hipHostFree(h->h_diis);
+  for (auto &t : h->ktimer)
+    for (auto e : t.ev) (void)hipEventDestroy(e);
   for (auto &e : h->ev)
     if (e) (void)hipEventDestroy(e);
   hipStream_t s = h->stream;
@@ -490,6 +492,29 @@ int ddx_set_precision(ddx_handle *h, int prec) {
     return DDX_ERR_ARG;
   }
   h->prec = prec;
+  return DDX_OK;
+}
+
+int ddx_kernel_timer(ddx_handle *h, int kernel, int enable) {
+  if (!h || kernel < 0 || kernel >= DDX_K_COUNT) return DDX_ERR_ARG;
+  h->ktimer[kernel].on = enable != 0;
+  h->ktimer[kernel].used = 0;
+  return DDX_OK;
+}
+
+int ddx_kernel_timer_read(ddx_handle *h, int kernel, int *launches, double *total_ms) {
+  if (!h || kernel < 0 || kernel >= DDX_K_COUNT || !launches || !total_ms) return DDX_ERR_ARG;
+  KernelTimer &t = h->ktimer[kernel];
+  DDX_TRY(sync(h));
+  double tot = 0.0;
+  for (int k = 0; k < t.used; ++k) {
+    float ms = 0.f;
+    DDX_HIP(hipEventElapsedTime(&ms, t.ev[2 * k], t.ev[2 * k + 1]));
+    tot += ms;
+  }
+  *launches = t.used;
+  *total_ms = tot;
+  t.used = 0;
   return DDX_OK;
 }
 

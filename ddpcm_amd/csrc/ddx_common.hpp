@@ -72,12 +72,20 @@ struct DiisState {
   int done;                 // converged or failed: later kernels return at once
   int ok;                   // converged
   int singular;             // DIIS resets from a singular B
+  int pending_norm;         // the last k_extrap left increment-norm partials
   int slot[kDiisMax];       // logical history index -> physical slot
   int ncoef;                // coefficients valid in coef[] (0: keep Jacobi step)
   double coef[kDiisMax];    // extrapolation weights, by logical index
   double b[(kDiisMax + 1) * (kDiisMax + 1)];  // bordered Gram matrix, rows of 25, 0-based
   double norm;              // increment norm of the last iteration
   double hist[kMaxIter + 1];
+};
+
+// Event pairs around the launches of one kernel class (ddx_kernel_timer).
+struct KernelTimer {
+  bool on = false;
+  int used = 0;
+  std::vector<hipEvent_t> ev;   // begin/end pairs, reused across reads
 };
 
 }  // namespace ddx
@@ -139,4 +147,30 @@ struct ddx_handle {
 
   std::vector<double> hist_direct, hist_adjoint;
   ddx_stats stats{};
+  ddx::KernelTimer ktimer[DDX_K_COUNT];
 };
+
+namespace ddx {
+// Brackets one launch with events when the kernel class is being timed.
+struct TimedLaunch {
+  ddx_handle *h;
+  int id;
+  hipEvent_t end = nullptr;
+  TimedLaunch(ddx_handle *hh, int kid) : h(hh), id(kid) {
+    KernelTimer &t = h->ktimer[id];
+    if (!t.on) return;
+    if ((int)t.ev.size() < 2 * (t.used + 1)) {
+      hipEvent_t a = nullptr, b = nullptr;
+      if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+      t.ev.push_back(a);
+      t.ev.push_back(b);
+    }
+    (void)hipEventRecord(t.ev[2 * t.used], h->stream);
+    end = t.ev[2 * t.used + 1];
+    t.used += 1;
+  }
+  ~TimedLaunch() {
+    if (end) (void)hipEventRecord(end, h->stream);
+  }
+};
+}  // namespace ddx

@@ -121,6 +121,7 @@ __global__ void k_psi(int n, int nylm, double fac, const double *__restrict__ q,
 int coulomb_potential(ddx_handle *h, int nsrc, const double *q, const double4 *src, int ntrg,
                       const double *trg3, double *phi) {
   if (ntrg <= 0) return DDX_OK;
+  TimedLaunch tl(h, DDX_K_POTENTIAL);
   hipLaunchKernelGGL(k_potential, dim3((ntrg + kCT - 1) / kCT), dim3(kCT), 0, h->stream, nsrc, q,
                      src, ntrg, trg3, phi);
   DDX_HIP(hipGetLastError());
@@ -143,8 +144,11 @@ int coulomb_field(ddx_handle *h, int nsrc, const double *q, const double *src3, 
   nsplit = chunk > 0 ? (nsrc + chunk - 1) / chunk : 1;
   if (nsplit < 1) nsplit = 1;
   DDX_TRY(h->fpart.reserve((size_t)nsplit * 3 * ntrg));
-  hipLaunchKernelGGL(k_field, dim3(tb, nsplit), dim3(kCT), 0, h->stream, nsrc, q, src3, ntrg, trg3,
-                     chunk > 0 ? chunk : kCT, h->fpart.p);
+  {
+    TimedLaunch tl(h, DDX_K_FIELD);
+    hipLaunchKernelGGL(k_field, dim3(tb, nsplit), dim3(kCT), 0, h->stream, nsrc, q, src3, ntrg, trg3,
+                       chunk > 0 ? chunk : kCT, h->fpart.p);
+  }
   DDX_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_field_reduce, dim3((3 * ntrg + 255) / 256), dim3(256), 0, h->stream, nsplit,
                      ntrg, h->fpart.p, ef);

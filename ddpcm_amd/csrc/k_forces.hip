@@ -335,6 +335,7 @@ __global__ void k_sph3(int n, const double4 *__restrict__ sph, double *__restric
 
 template <int LMAX>
 int launch_triple(ddx_handle *h, const double *sigma, double4 *fb) {
+  TimedLaunch tl(h, DDX_K_FORCES);
   hipLaunchKernelGGL(k_force_triple<LMAX>, dim3(h->n), dim3(256), 0, h->stream, h->ngrid,
                      h->inl.p, h->pairoff.p, h->sph.p, h->grid.p, h->tgeo.p, h->ttw.p, h->tp.p,
                      sigma, h->facs.p, h->cf.p + 2 * h->nylm, fb);

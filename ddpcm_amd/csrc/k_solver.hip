@@ -110,34 +110,70 @@ __device__ __forceinline__ double lane_value(double v, int src) {
 
 constexpr int kSmallThreads = 256;       // 4 workgroup partials per thread
 
-__global__ __launch_bounds__(kSmallThreads) void k_diis_small(int nblk, int diis_max,
-                                                              const double *__restrict__ part,
-                                                              DiisState *__restrict__ st,
-                                                              int *__restrict__ newslot_out) {
+__global__ __launch_bounds__(kSmallThreads) void k_diis_small(
+    int nblk, int diis_max, const double *__restrict__ part, int nblk_e,
+    const double *__restrict__ part_e, int nsph, double tol, int maxit,
+    DiisState *__restrict__ st, int *__restrict__ newslot_out) {
 #pragma clang fp contract(off)
   if (st->done) return;
-  __shared__ double red[kSmallThreads / kWave][kPartStride];
+  __shared__ double red[kSmallThreads / kWave][kPartStride + 1];
+  __shared__ int stop;
   const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
   int nmat = st->nmat;
   const int off = (nmat >= diis_max) ? 10 : 0;
   const int ndot = nmat - off - 1;
-  // Gram-row dot products: each thread sums 4 workgroup partials (loads in
-  // flight together), butterflies per wave, waves combined in a fixed order
+  const bool pending = st->pending_norm != 0;
+  // Gram-row dot products (each thread sums 4 workgroup partials) and the
+  // increment norm of the previous iteration, as straight-line butterflies
   constexpr int kPer = kMaxBlk / kSmallThreads;
+  double v[kPartStride + 1];
 #pragma unroll
   for (int q = 0; q < kPartStride; ++q) {
-    double v = 0.0;
-    if (q < ndot || q == kDiisMax) {
+    const bool need = (q < ndot || q == kDiisMax);
+    double sum = 0.0;
 #pragma unroll
-      for (int k = 0; k < kPer; ++k) {
-        const int b = tid + kSmallThreads * k;
-        if (b < nblk) v += part[(size_t)q * kMaxBlk + b];
-      }
+    for (int k = 0; k < kPer; ++k) {
+      const int b = tid + kSmallThreads * k;
+      sum += (need && b < nblk) ? part[(size_t)q * kMaxBlk + b] : 0.0;
     }
-    v = wave_sum(v);
-    if (lane == 0) red[wave][q] = v;
+    v[q] = sum;
+  }
+  {
+    double sum = 0.0;
+    for (int b = tid; b < nblk_e; b += kSmallThreads) sum += pending ? part_e[b] : 0.0;
+    v[kPartStride] = sum;
+  }
+#pragma unroll
+  for (int off2 = 32; off2 > 0; off2 >>= 1)
+#pragma unroll
+    for (int q = 0; q <= kPartStride; ++q) v[q] += __shfl_xor(v[q], off2, kWave);
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q <= kPartStride; ++q) red[wave][q] = v[q];
   }
   __syncthreads();
+  if (tid == 0) {
+    stop = 0;
+    if (pending) {
+      // rms of the per-sphere norms and the convergence test of the previous
+      // iteration (rmsvec, jacobi_diis.f90:405-438; jacobi_diis.f90:153-156)
+      double ss = 0.0;
+      for (int w = 0; w < kSmallThreads / kWave; ++w) ss += red[w][kPartStride];
+      const double norm = sqrt(ss / (double)nsph);
+      const int it = st->it;
+      if (it <= kMaxIter) st->hist[it] = norm;
+      st->it = it + 1;
+      st->norm = norm;
+      st->ok = norm < tol;
+      st->pending_norm = 0;
+      if (norm < tol || it + 1 >= maxit) {
+        st->done = 1;
+        stop = 1;
+      }
+    }
+  }
+  __syncthreads();
+  if (stop) return;
   if (wave != 0) return;
   double a[kB];
 #pragma unroll
@@ -150,6 +186,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(int nblk, int diis
     mydot += red[w][qi];
     self += red[w][kDiisMax];
   }
+  if (lane == 0) st->pending_norm = 1;   // k_extrap leaves this iteration's norm
   if (!(lane >= 1 && lane - 1 < ndot)) mydot = 0.0;
   if (lane == 0) *newslot_out = st->slot[nmat - 1];
   if (nmat >= diis_max) {
@@ -219,7 +256,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(int nblk, int diis
         }
       }
     }
-    for (int o = 32; o > 0; o >>= 1) {
+    for (int o = 16; o > 0; o >>= 1) {   // candidates live in lanes 0..24
       double ob = __shfl_xor(big, o, kWave);
       double ov = __shfl_xor(bval, o, kWave);
       int oi = __shfl_xor(best, o, kWave);
@@ -487,35 +524,60 @@ int solve_jacobi_diis(ddx_handle *h, int star, int diis_max, double tol, const d
   hs->ok = 0;
   hs->singular = 0;
   hs->ncoef = 0;
+  hs->pending_norm = 0;
   for (int q = 0; q < kDiisMax; ++q) hs->slot[q] = q;
   hs->norm = 0.0;
   DiisState *ds = h->diis.p;
   DDX_HIP(hipMemcpyAsync(ds, hs, sizeof(DiisState), hipMemcpyHostToDevice, s));
 
   const double *invl1 = h->cf.p + h->nylm;   // 1/(l+1) table stored after cf
+  // kernel classes launched once per iteration: launches enqueued after
+  // convergence return at once and are dropped from their timers below
+  const int per_iter[5] = {star ? DDX_K_SWEEP_LSTAR : DDX_K_SWEEP_L,
+                           star ? DDX_K_EXPAND : DDX_K_PROJECT, DDX_K_JACOBI, DDX_K_DIIS,
+                           DDX_K_EXTRAP};
+  int used0[5];
+  for (int k = 0; k < 5; ++k) used0[k] = h->ktimer[per_iter[k]].used;
   int enqueued = 0, group = 3;
   while (true) {
-    for (int g = 0; g < group && enqueued < max_iter; ++g, ++enqueued) {
+    // with DIIS the norm of iteration k is evaluated by the DIIS kernel of
+    // iteration k + 1, so up to max_iter + 1 rounds are enqueued
+    const int rounds = diis_max > 0 ? max_iter + 1 : max_iter;
+    for (int g = 0; g < group && enqueued < rounds; ++g, ++enqueued) {
       if (star) DDX_TRY(apply_lstarx(h, d_x, h->y.p, ds));
       else DDX_TRY(apply_lx(h, d_x, h->y.p, ds));
-      hipLaunchKernelGGL(k_jacobi, dim3(nbj), dim3(kJThreads), 0, s, nn, h->nylm, diis_max,
+      {
+        TimedLaunch tl(h, DDX_K_JACOBI);
+        hipLaunchKernelGGL(k_jacobi, dim3(nbj), dim3(kJThreads), 0, s, nn, h->nylm, diis_max,
                          h->facl.p, d_rhs, h->y.p, d_x, h->xd.p, h->ed.p, h->xnew.p, h->part.p, ds);
+      }
       DDX_HIP(hipGetLastError());
       if (diis_max > 0) {
+        TimedLaunch tl(h, DDX_K_DIIS);
         hipLaunchKernelGGL(k_diis_small, dim3(1), dim3(kSmallThreads), 0, s, nbj, diis_max,
-                           h->part.p, ds, newslot);
+                           h->part.p, nbe, part_e, h->n, tol, max_iter, ds, newslot);
         DDX_HIP(hipGetLastError());
       }
-      hipLaunchKernelGGL(k_extrap, dim3(nbe), dim3(kJThreads), 0, s, h->n, h->nylm, nn, diis_max,
-                         h->xd.p, h->xnew.p, newslot, invl1, d_x, part_e, ds);
+      {
+        TimedLaunch tl(h, DDX_K_EXTRAP);
+        hipLaunchKernelGGL(k_extrap, dim3(nbe), dim3(kJThreads), 0, s, h->n, h->nylm, nn, diis_max,
+                           h->xd.p, h->xnew.p, newslot, invl1, d_x, part_e, ds);
+      }
       DDX_HIP(hipGetLastError());
-      hipLaunchKernelGGL(k_norm, dim3(1), dim3(kWave), 0, s, nbe, h->n, tol, max_iter, part_e, ds);
-      DDX_HIP(hipGetLastError());
+      if (diis_max == 0) {
+        hipLaunchKernelGGL(k_norm, dim3(1), dim3(kWave), 0, s, nbe, h->n, tol, max_iter, part_e,
+                           ds);
+        DDX_HIP(hipGetLastError());
+      }
     }
     DDX_HIP(hipMemcpyAsync(hs, ds, sizeof(DiisState), hipMemcpyDeviceToHost, s));
     DDX_HIP(hipStreamSynchronize(s));
-    if (hs->done || enqueued >= max_iter) break;
+    if (hs->done || enqueued >= (diis_max > 0 ? max_iter + 1 : max_iter)) break;
     group = 2;
+  }
+  for (int k = 0; k < 5; ++k) {
+    KernelTimer &t = h->ktimer[per_iter[k]];
+    if (t.on && t.used > used0[k] + hs->it) t.used = used0[k] + hs->it;
   }
   *n_iter = hs->ok ? hs->it : max_iter + 1;
   *ok = hs->ok;

@@ -197,6 +197,7 @@ __global__ __launch_bounds__(kWave) void k_sweep_adjoint(
 
 template <int LMAX>
 int launch_contract(ddx_handle *h, const double *X, double *r, const DiisState *gate) {
+  TimedLaunch tl(h, DDX_K_SWEEP_L);
   hipLaunchKernelGGL(k_sweep_contract<LMAX>, dim3(h->n), dim3(kLThreads), 0, h->stream,
                      h->inl.p, h->pairoff.p, h->tgeo.p, h->ttw.p, h->tslot.p, X, h->cf.p, r, gate);
   DDX_HIP(hipGetLastError());
@@ -205,6 +206,7 @@ int launch_contract(ddx_handle *h, const double *X, double *r, const DiisState *
 
 template <int LMAX>
 int launch_adjoint(ddx_handle *h, const double *xi, double *y, const DiisState *gate) {
+  TimedLaunch tl(h, DDX_K_SWEEP_LSTAR);
   hipLaunchKernelGGL(k_sweep_adjoint<LMAX>, dim3(h->n), dim3(kWave), 0, h->stream, h->inl.p,
                      h->pairoff.p, h->tgeo.p, h->ttw.p, h->tp.p, xi, h->cf.p, y, gate);
   DDX_HIP(hipGetLastError());

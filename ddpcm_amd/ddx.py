@@ -26,8 +26,10 @@ EXPORTS = (
     "ddx_get_neighbours", "ddx_mkrhs", "ddx_cosmo", "ddx_forces_dd", "ddx_mkzeta",
     "ddx_efld", "ddx_lx", "ddx_lstarx", "ddx_ldm1x", "ddx_hnorm", "ddx_jacobi_diis",
     "ddx_get_history", "ddx_energy_forces", "ddx_upload", "ddx_run", "ddx_result",
-    "ddx_set_precision", "ddx_get_stats",
+    "ddx_set_precision", "ddx_get_stats", "ddx_kernel_timer", "ddx_kernel_timer_read",
 )
+KERNELS = ("sweep_l", "sweep_lstar", "project", "expand", "jacobi", "diis", "extrap", "field",
+           "potential", "forces")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -85,6 +87,8 @@ def load_library(path: str = LIB_PATH):
     L.ddx_result.argtypes = [C.c_void_p, _dp, _dp]
     L.ddx_set_precision.argtypes = [C.c_void_p, C.c_int]
     L.ddx_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
+    L.ddx_kernel_timer.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.ddx_kernel_timer_read.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
     _LIB = L
     return L
 
@@ -276,6 +280,17 @@ class DDCosmo:
         fx = np.zeros((self.nsph, 3))
         self._check(self.L.ddx_result(self.h, C.cast(C.byref(es), _dp), _p(fx)))
         return es.value, fx
+
+    def kernel_timer(self, name, enable=True):
+        """Brackets every launch of one kernel class with HIP events."""
+        self._check(self.L.ddx_kernel_timer(self.h, KERNELS.index(name), int(enable)))
+
+    def kernel_timer_read(self, name):
+        """(launches, total ms) since the last read; resets the count."""
+        n, ms = C.c_int(), C.c_double()
+        self._check(self.L.ddx_kernel_timer_read(self.h, KERNELS.index(name), C.byref(n),
+                                                 C.byref(ms)))
+        return n.value, ms.value
 
     def set_precision(self, prec):
         self._check(self.L.ddx_set_precision(self.h, prec))

@@ -127,6 +127,24 @@ int ddx_result(ddx_handle *h, double *esolv, double *fx);
 int ddx_set_precision(ddx_handle *h, int prec);
 int ddx_get_stats(const ddx_handle *h, ddx_stats *st);
 
+/* Kernel timing (instrumentation; no counterpart in the reference).  While
+ * enabled for a kernel class, every launch of that class is bracketed by HIP
+ * events on the handle's stream; ddx_kernel_timer_read returns the number of
+ * launches and their summed duration since the last read and resets them. */
+#define DDX_K_SWEEP_L 0      /* L X sweep over the triples               */
+#define DDX_K_SWEEP_LSTAR 1  /* L* S sweep over the triples              */
+#define DDX_K_PROJECT 2      /* MFMA projection onto harmonics           */
+#define DDX_K_EXPAND 3       /* MFMA expansion on the grid (xi)          */
+#define DDX_K_JACOBI 4       /* Jacobi update + DIIS Gram row            */
+#define DDX_K_DIIS 5         /* DIIS bordered solve                      */
+#define DDX_K_EXTRAP 6       /* DIIS extrapolation + increment norm      */
+#define DDX_K_FIELD 7        /* all-pairs field                          */
+#define DDX_K_POTENTIAL 8    /* all-pairs potential                      */
+#define DDX_K_FORCES 9       /* per-triple forces terms                  */
+#define DDX_K_COUNT 10
+int ddx_kernel_timer(ddx_handle *h, int kernel, int enable);
+int ddx_kernel_timer_read(ddx_handle *h, int kernel, int *launches, double *total_ms);
+
 #ifdef __cplusplus
 }
 #endif
