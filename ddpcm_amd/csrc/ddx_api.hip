@@ -35,6 +35,7 @@ int download(double *dst, const double *src, size_t n, hipStream_t s) {
 
 int sync(ddx_handle *h) {
   DDX_HIP(hipStreamSynchronize(h->stream));
+  if (h->lane[1].stream) DDX_HIP(hipStreamSynchronize(h->lane[1].stream));
   return DDX_OK;
 }
 
@@ -62,34 +63,80 @@ int set_geometry(ddx_handle *h, const double *x, const double *y, const double *
   return sync(h);
 }
 
-// Direct solve L X = g from the rhs in h->rhs; X in h->sigma; E in energy[0].
-int run_direct(ddx_handle *h, const double *d_psi) {
-  int nit = 0, ok = 0;
-  DDX_TRY(device_ldm1x(h, h->rhs.p, h->sigma.p));   // guess D^-1 rhs (cosmo.f90:136-138)
-  DDX_TRY(solve_jacobi_diis(h, 0, kDiisMax, tol_of(h->iconv), h->rhs.p, h->sigma.p, kMaxIter,
-                            &nit, &ok, &h->hist_direct));
+// Start of the direct solve L X = g (rhs in h->rhs, X in h->sigma) on a lane,
+// from the guess D^-1 rhs (cosmo.f90:136-138).
+int begin_direct(ddx_handle *h, Lane &ln) {
+  DDX_TRY(device_ldm1x(h, ln.stream, h->rhs.p, h->sigma.p));
+  return solve_begin(h, ln, 0, kDiisMax, tol_of(h->iconv), h->rhs.p, h->sigma.p, kMaxIter);
+}
+
+// Start of the adjoint solve L* S = psi (S in h->s) from D^-1 psi
+// (cosmo.f90:174-176).
+int begin_adjoint(ddx_handle *h, Lane &ln, const double *d_psi) {
+  DDX_TRY(device_ldm1x(h, ln.stream, d_psi, h->s.p));
+  return solve_begin(h, ln, 1, kDiisMax, tol_of(h->iconv), d_psi, h->s.p, kMaxIter);
+}
+
+// End of the direct solve; the energy goes to energy[0] on the handle's stream.
+int end_direct(ddx_handle *h, Lane &ln, const double *d_psi, int *ok) {
+  int nit = 0;
+  DDX_TRY(solve_end(h, ln, &nit, ok, &h->hist_direct));
   h->stats.niter_direct = nit;
   DDX_TRY(energy_dot(h, h->sigma.p, d_psi, 0.5 * ((h->eps - 1.0) / h->eps), h->energy.p));
   h->have_sigma = true;
-  if (!ok) {
-    set_error("ddCOSMO did not converge");
-    return DDX_ERR_NOCONV;
+  return DDX_OK;
+}
+
+int end_adjoint(ddx_handle *h, Lane &ln, int *ok) {
+  int nit = 0;
+  DDX_TRY(solve_end(h, ln, &nit, ok, &h->hist_adjoint));
+  h->stats.niter_adjoint = nit;
+  h->have_s = true;
+  return DDX_OK;
+}
+
+int noconv(bool star) {
+  set_error(star ? "adjoint ddCOSMO did not converge" : "ddCOSMO did not converge");
+  return DDX_ERR_NOCONV;
+}
+
+// Runs the solves begun on the given lanes to their ends, enqueueing the
+// next group of iterations on every lane before waiting on any of them.
+int drive(ddx_handle *h, Lane *const *lanes, int count) {
+  bool left = true;
+  while (left) {
+    left = false;
+    for (int k = 0; k < count; ++k)
+      if (lanes[k]->active) DDX_TRY(solve_enqueue(h, *lanes[k]));
+    for (int k = 0; k < count; ++k) {
+      Lane &ln = *lanes[k];
+      if (!ln.active) continue;
+      bool fin = false;
+      DDX_TRY(solve_poll(h, ln, &fin));
+      if (fin) ln.active = false;   // solve_end reports it
+      else left = true;
+    }
   }
   return DDX_OK;
 }
 
+// Direct solve on lane 0 (the handle's stream); X in h->sigma, E in energy[0].
+int run_direct(ddx_handle *h, const double *d_psi) {
+  Lane *ln = &h->lane[0];
+  int ok = 0;
+  DDX_TRY(begin_direct(h, *ln));
+  DDX_TRY(drive(h, &ln, 1));
+  DDX_TRY(end_direct(h, *ln, d_psi, &ok));
+  return ok ? DDX_OK : noconv(false);
+}
+
 int run_adjoint(ddx_handle *h, const double *d_psi) {
-  int nit = 0, ok = 0;
-  DDX_TRY(device_ldm1x(h, d_psi, h->s.p));   // guess D^-1 psi (cosmo.f90:174-176)
-  DDX_TRY(solve_jacobi_diis(h, 1, kDiisMax, tol_of(h->iconv), d_psi, h->s.p, kMaxIter, &nit, &ok,
-                            &h->hist_adjoint));
-  h->stats.niter_adjoint = nit;
-  h->have_s = true;
-  if (!ok) {
-    set_error("adjoint ddCOSMO did not converge");
-    return DDX_ERR_NOCONV;
-  }
-  return DDX_OK;
+  Lane *ln = &h->lane[0];
+  int ok = 0;
+  DDX_TRY(begin_adjoint(h, *ln, d_psi));
+  DDX_TRY(drive(h, &ln, 1));
+  DDX_TRY(end_adjoint(h, *ln, &ok));
+  return ok ? DDX_OK : noconv(true);
 }
 
 int reserve_vectors(ddx_handle *h) {
@@ -111,31 +158,59 @@ int pipeline(ddx_handle *h) {
   DDX_TRY(reserve_vectors(h));
   DDX_TRY(h->phi.reserve((size_t)h->ncav + 1));
   DDX_TRY(h->zeta.reserve((size_t)h->ncav + 1));
+  if (h->igrad) DDX_TRY(h->ef.reserve(field_buffer_size(h)));
   DDX_HIP(hipEventRecord(h->ev[1], s));
-  DDX_TRY(coulomb_potential(h, h->n, h->charge.p, h->sph.p, h->ncav, h->ccav.p, h->phi.p));
+  // potential at the cavity points, with forces also the solute field there
+  DDX_TRY(coulomb_potential(h, h->n, h->charge.p, h->sph.p, h->ncav, h->ccav.p, h->phi.p,
+                            h->igrad ? h->ef.p : nullptr));
   DDX_TRY(make_psi(h, h->charge.p, h->psi.p));
   DDX_TRY(build_rhs(h, h->phi.p, h->rhs.p));
   DDX_HIP(hipEventRecord(h->ev[2], s));
-  DDX_TRY(run_direct(h, h->psi.p));
+  // the direct solve on lane 0 and, with forces, the adjoint solve on lane 1
+  // at the same time: neither depends on the other (cosmo.f90:136-196)
+  Lane *lanes[2] = {&h->lane[0], &h->lane[1]};
+  const int nl = h->igrad ? 2 : 1;
+  hipStream_t s1 = h->lane[1].stream;
+  if (h->igrad) {
+    DDX_HIP(hipStreamWaitEvent(s1, h->ev[2], 0));
+    DDX_TRY(begin_adjoint(h, h->lane[1], h->psi.p));
+  }
+  DDX_TRY(begin_direct(h, h->lane[0]));
+  DDX_TRY(drive(h, lanes, nl));
+  int ok_d = 0, ok_a = 1;
+  DDX_TRY(end_direct(h, h->lane[0], h->psi.p, &ok_d));
   DDX_HIP(hipEventRecord(h->ev[3], s));
   if (h->igrad) {
-    DDX_TRY(run_adjoint(h, h->psi.p));
-    DDX_HIP(hipEventRecord(h->ev[4], s));
+    DDX_TRY(end_adjoint(h, h->lane[1], &ok_a));
+    DDX_HIP(hipEventRecord(h->ev[4], s1));
+  }
+  if (!ok_d) {
+    DDX_TRY(sync(h));
+    return noconv(false);
+  }
+  if (!ok_a) {
+    DDX_TRY(sync(h));
+    return noconv(true);
+  }
+  if (h->igrad) {
+    DDX_HIP(hipStreamWaitEvent(s, h->ev[4], 0));
+    DDX_HIP(hipEventRecord(h->ev[7], s));
     DDX_TRY(forces_dd(h, h->phi.p, h->sigma.p, h->s.p, h->fx.p));
     DDX_TRY(make_zeta(h, h->s.p, h->zeta.p));
-    DDX_TRY(field_forces(h, h->charge.p, h->zeta.p, h->fx.p));
+    DDX_TRY(field_forces(h, h->charge.p, h->zeta.p, h->fx.p, true));
     DDX_HIP(hipEventRecord(h->ev[5], s));
   }
   DDX_TRY(sync(h));
+  // the two solves overlap: each is timed from the end of the RHS
   float ms = 0.f;
   DDX_HIP(hipEventElapsedTime(&ms, h->ev[1], h->ev[2]));
   h->stats.ms_rhs = ms;
   DDX_HIP(hipEventElapsedTime(&ms, h->ev[2], h->ev[3]));
   h->stats.ms_direct = ms;
   if (h->igrad) {
-    DDX_HIP(hipEventElapsedTime(&ms, h->ev[3], h->ev[4]));
+    DDX_HIP(hipEventElapsedTime(&ms, h->ev[2], h->ev[4]));
     h->stats.ms_adjoint = ms;
-    DDX_HIP(hipEventElapsedTime(&ms, h->ev[4], h->ev[5]));
+    DDX_HIP(hipEventElapsedTime(&ms, h->ev[7], h->ev[5]));
     h->stats.ms_forces = ms;
   }
   return DDX_OK;
@@ -191,19 +266,23 @@ int ddx_init(ddx_handle **out, int n, const double *x, const double *y, const do
     ddx_free(h);
     return code;
   };
-  if (hipStreamCreate(&h->stream) != hipSuccess) {
+  if (hipStreamCreate(&h->stream) != hipSuccess ||
+      hipStreamCreate(&h->lane[1].stream) != hipSuccess) {
     set_error("hipStreamCreate failed");
     return fail(DDX_ERR_HIP);
   }
+  h->lane[0].stream = h->stream;
   for (auto &e : h->ev)
     if (hipEventCreate(&e) != hipSuccess) {
       set_error("hipEventCreate failed");
       return fail(DDX_ERR_HIP);
     }
-  if (hipHostMalloc((void **)&h->h_diis, sizeof(DiisState), hipHostMallocDefault) != hipSuccess) {
-    set_error("hipHostMalloc failed");
-    return fail(DDX_ERR_ALLOC);
-  }
+  for (Lane &ln : h->lane)
+    if (hipHostMalloc((void **)&ln.h_diis, sizeof(DiisState), hipHostMallocDefault) !=
+        hipSuccess) {
+      set_error("hipHostMalloc failed");
+      return fail(DDX_ERR_ALLOC);
+    }
   // one-time tables (ddcosmo.f90:225-267)
   if (lebedev_grid(h->ngrid, h->h_grid, h->h_w) != 0) {
     set_error("no Lebedev grid of %d points", h->ngrid);
@@ -245,14 +324,22 @@ int ddx_init(ddx_handle **out, int n, const double *x, const double *y, const do
 int ddx_free(ddx_handle *h) {
   if (!h) return DDX_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  if (h->h_diis) (void)hipHostFree(h->h_diis);
-  for (auto &t : h->ktimer)
-    for (auto e : t.ev) (void)hipEventDestroy(e);
+  if (h->lane[1].stream) (void)hipStreamSynchronize(h->lane[1].stream);
+  for (Lane &ln : h->lane)
+    if (ln.h_diis) (void)hipHostFree(ln.h_diis);
+  for (auto &t : h->ktimer) {
+    for (auto &r : t.recs) {
+      (void)hipEventDestroy(r.b);
+      (void)hipEventDestroy(r.e);
+    }
+    for (auto e : t.pool) (void)hipEventDestroy(e);
+  }
   for (auto &e : h->ev)
     if (e) (void)hipEventDestroy(e);
-  hipStream_t s = h->stream;
+  hipStream_t s = h->stream, s1 = h->lane[1].stream;
   delete h;   // device arrays release themselves
   if (s) (void)hipStreamDestroy(s);
+  if (s1) (void)hipStreamDestroy(s1);
   return DDX_OK;
 }
 
@@ -321,7 +408,8 @@ int ddx_mkrhs(ddx_handle *h, const double *charge, double *phi, double *psi) {
   DDX_TRY(reserve_vectors(h));
   DDX_TRY(upload(h->charge, charge, h->n, h->stream));
   DDX_TRY(h->phi.reserve((size_t)h->ncav + 1));
-  DDX_TRY(coulomb_potential(h, h->n, h->charge.p, h->sph.p, h->ncav, h->ccav.p, h->phi.p));
+  DDX_TRY(coulomb_potential(h, h->n, h->charge.p, h->sph.p, h->ncav, h->ccav.p, h->phi.p,
+                            nullptr));
   DDX_TRY(make_psi(h, h->charge.p, h->psi.p));
   DDX_TRY(download(phi, h->phi.p, h->ncav, h->stream));
   DDX_TRY(download(psi, h->psi.p, (size_t)h->nylm * h->n, h->stream));
@@ -406,9 +494,9 @@ static int op_apply(ddx_handle *h, int which, const double *x, double *y) {
   DDX_TRY(reserve_vectors(h));
   DDX_TRY(h->y.reserve(nn));
   DDX_TRY(upload(h->x, x, nn, s));
-  if (which == 0) DDX_TRY(apply_lx(h, h->x.p, h->y.p, nullptr));
-  else if (which == 1) DDX_TRY(apply_lstarx(h, h->x.p, h->y.p, nullptr));
-  else DDX_TRY(device_ldm1x(h, h->x.p, h->y.p));
+  if (which == 0) DDX_TRY(apply_lx(h, h->lane[0], h->x.p, h->y.p, nullptr));
+  else if (which == 1) DDX_TRY(apply_lstarx(h, h->lane[0], h->x.p, h->y.p, nullptr));
+  else DDX_TRY(device_ldm1x(h, s, h->x.p, h->y.p));
   DDX_TRY(download(y, h->y.p, nn, s));
   return sync(h);
 }
@@ -433,8 +521,8 @@ int ddx_jacobi_diis(ddx_handle *h, int star, int diis_max, double tol, const dou
   DDX_TRY(reserve_vectors(h));
   DDX_TRY(upload(h->rhs, rhs, nn, s));
   DDX_TRY(upload(h->x, x, nn, s));
-  DDX_TRY(solve_jacobi_diis(h, star ? 1 : 0, diis_max, tol, h->rhs.p, h->x.p, *n_iter, n_iter, ok,
-                            star ? &h->hist_adjoint : &h->hist_direct));
+  DDX_TRY(solve_jacobi_diis(h, h->lane[0], star ? 1 : 0, diis_max, tol, h->rhs.p, h->x.p, *n_iter,
+                            n_iter, ok, star ? &h->hist_adjoint : &h->hist_direct));
   DDX_TRY(download(x, h->x.p, nn, s));
   return sync(h);
 }
@@ -497,8 +585,14 @@ int ddx_set_precision(ddx_handle *h, int prec) {
 
 int ddx_kernel_timer(ddx_handle *h, int kernel, int enable) {
   if (!h || kernel < 0 || kernel >= DDX_K_COUNT) return DDX_ERR_ARG;
-  h->ktimer[kernel].on = enable != 0;
-  h->ktimer[kernel].used = 0;
+  KernelTimer &t = h->ktimer[kernel];
+  DDX_TRY(sync(h));
+  for (auto &r : t.recs) {
+    t.pool.push_back(r.b);
+    t.pool.push_back(r.e);
+  }
+  t.recs.clear();
+  t.on = enable != 0;
   return DDX_OK;
 }
 
@@ -507,14 +601,16 @@ int ddx_kernel_timer_read(ddx_handle *h, int kernel, int *launches, double *tota
   KernelTimer &t = h->ktimer[kernel];
   DDX_TRY(sync(h));
   double tot = 0.0;
-  for (int k = 0; k < t.used; ++k) {
+  for (auto &r : t.recs) {
     float ms = 0.f;
-    DDX_HIP(hipEventElapsedTime(&ms, t.ev[2 * k], t.ev[2 * k + 1]));
+    DDX_HIP(hipEventElapsedTime(&ms, r.b, r.e));
     tot += ms;
+    t.pool.push_back(r.b);
+    t.pool.push_back(r.e);
   }
-  *launches = t.used;
+  *launches = (int)t.recs.size();
   *total_ms = tot;
-  t.used = 0;
+  t.recs.clear();
   return DDX_OK;
 }
 

@@ -82,10 +82,35 @@ struct DiisState {
 };
 
 // Event pairs around the launches of one kernel class (ddx_kernel_timer).
+// A launch made for a Jacobi/DIIS solve carries the solve's serial number
+// and round, so that rounds enqueued after convergence can be dropped.
 struct KernelTimer {
+  struct Rec {
+    hipEvent_t b, e;
+    int solve;                  // serial number of the solve, -1 outside solves
+    int round;
+  };
   bool on = false;
-  int used = 0;
-  std::vector<hipEvent_t> ev;   // begin/end pairs, reused across reads
+  std::vector<Rec> recs;        // launches since the last read
+  std::vector<hipEvent_t> pool; // events free for reuse
+};
+
+// Workspace and progress of one Jacobi/DIIS solve.  Each lane owns a stream;
+// the direct and adjoint solves of one energy + forces evaluation run on two
+// lanes at once, so that the latency-bound DIIS steps of one overlap the
+// sweeps of the other.
+struct Lane {
+  hipStream_t stream = nullptr;
+  DevArray<double> y, xnew, P, xiw, rw, xd, ed, part;
+  DevArray<DiisState> diis;
+  DevArray<int> misc;
+  DiisState *h_diis = nullptr;  // pinned host mirror of diis
+  // the solve in flight (solve_begin .. solve_end)
+  int serial = -1, star = 0, diis_max = 0, max_iter = 0, rounds = 0, enqueued = 0, group = 3;
+  double tol = 0.0;
+  const double *rhs = nullptr;
+  double *x = nullptr;
+  bool active = false;
 };
 
 }  // namespace ddx
@@ -106,7 +131,7 @@ struct ddx_handle {
   bool have_setup = false, have_sigma = false, have_s = false, have_charge = false;
 
   hipStream_t stream = nullptr;
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[10] = {};
 
   // tables
   ddx::DevArray<double4> grid;    // (x, y, z, w) per Lebedev point
@@ -135,11 +160,11 @@ struct ddx_handle {
   ddx::DevArray<int> pttri;       // per point: its triples in neighbour order
   ddx::DevArray<int> tslot;       // per triple: its slot in pttri
 
-  // solver workspace
-  ddx::DevArray<double> x, y, rhs, xnew, P, xiw, r, xd, ed, part;
-  ddx::DevArray<int> counter;     // arrival counters of the last-workgroup reductions
-  ddx::DevArray<ddx::DiisState> diis;
-  ddx::DiisState *h_diis = nullptr;  // pinned host mirror
+  // solver workspace: lane 0 runs on `stream`, lane 1 on a stream of its own
+  ddx::Lane lane[2];
+  ddx::DevArray<double> x, y, rhs, r, part;
+  int solve_serial = 0;
+  int tag_solve = -1, tag_round = 0;  // attached to timed launches
 
   // problem data / results
   ddx::DevArray<double> phi, psi, sigma, s, xi, fx, zeta, ef, energy, fpart;
@@ -155,22 +180,31 @@ namespace ddx {
 struct TimedLaunch {
   ddx_handle *h;
   int id;
+  hipStream_t s;
   hipEvent_t end = nullptr;
-  TimedLaunch(ddx_handle *hh, int kid) : h(hh), id(kid) {
+  TimedLaunch(ddx_handle *hh, int kid, hipStream_t stream) : h(hh), id(kid), s(stream) {
     KernelTimer &t = h->ktimer[id];
     if (!t.on) return;
-    if ((int)t.ev.size() < 2 * (t.used + 1)) {
-      hipEvent_t a = nullptr, b = nullptr;
-      if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
-      t.ev.push_back(a);
-      t.ev.push_back(b);
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    for (auto &e : ev) {
+      if (!t.pool.empty()) {
+        e = t.pool.back();
+        t.pool.pop_back();
+      } else if (hipEventCreate(&e) != hipSuccess) {
+        e = nullptr;
+      }
     }
-    (void)hipEventRecord(t.ev[2 * t.used], h->stream);
-    end = t.ev[2 * t.used + 1];
-    t.used += 1;
+    if (!ev[0] || !ev[1]) {
+      for (auto e : ev)
+        if (e) t.pool.push_back(e);
+      return;
+    }
+    (void)hipEventRecord(ev[0], s);
+    t.recs.push_back({ev[0], ev[1], h->tag_solve, h->tag_round});
+    end = ev[1];
   }
   ~TimedLaunch() {
-    if (end) (void)hipEventRecord(end, h->stream);
+    if (end) (void)hipEventRecord(end, s);
   }
 };
 }  // namespace ddx

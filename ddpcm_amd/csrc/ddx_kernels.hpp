@@ -14,29 +14,43 @@ int run_setup(ddx_handle *h);
 
 // k_gemm.hip — FP64 MFMA products with the harmonic tables
 // y(lm, i) = alpha sum_n Yw(lm, n) P(n, i)
-int project(ddx_handle *h, double alpha, const double *P, double *y, const DiisState *gate);
+int project(ddx_handle *h, hipStream_t s, double alpha, const double *P, double *y,
+            const DiisState *gate);
 // xi(n, i) = [w_n] sum_lm Y(lm, n) S(lm, i)
-int expand(ddx_handle *h, const double *S, double *xi, bool wscale, const DiisState *gate);
+int expand(ddx_handle *h, hipStream_t s, const double *S, double *xi, bool wscale,
+           const DiisState *gate);
 
 // k_sweep.hip — off-diagonal operators (matvec.f90:94-235)
-int apply_lx(ddx_handle *h, const double *x, double *y, const DiisState *gate);
-int apply_lstarx(ddx_handle *h, const double *x, double *y, const DiisState *gate);
+// (scratch and stream of the lane)
+int apply_lx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate);
+int apply_lstarx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate);
 // per-triple W_t sum_lm R_lm(t) X_{c,lm}, stored in point-slot order
-int sweep_contract(ddx_handle *h, const double *X, double *rw, const DiisState *gate);
+int sweep_contract(ddx_handle *h, hipStream_t s, const double *X, double *rw,
+                   const DiisState *gate);
 
 // k_solver.hip — Jacobi/DIIS (jacobi_diis.f90:43-438), RHS, norms
-int solve_jacobi_diis(ddx_handle *h, int star, int diis_max, double tol,
+// A solve on a lane: solve_begin sets it up (x holds the guess), each
+// solve_enqueue queues the next group of iterations and the copy-back of the
+// DIIS state, solve_poll waits for that copy and says whether the solve has
+// ended, solve_end reports it.  solve_jacobi_diis runs all four.
+int solve_begin(ddx_handle *h, Lane &ln, int star, int diis_max, double tol,
+                const double *d_rhs, double *d_x, int max_iter);
+int solve_enqueue(ddx_handle *h, Lane &ln);
+int solve_poll(ddx_handle *h, Lane &ln, bool *finished);
+int solve_end(ddx_handle *h, Lane &ln, int *n_iter, int *ok, std::vector<double> *hist);
+int solve_jacobi_diis(ddx_handle *h, Lane &ln, int star, int diis_max, double tol,
                       const double *d_rhs, double *d_x, int max_iter,
                       int *n_iter, int *ok, std::vector<double> *hist);
-int device_ldm1x(ddx_handle *h, const double *x, double *y);
+int device_ldm1x(ddx_handle *h, hipStream_t s, const double *x, double *y);
 int device_hnorm(ddx_handle *h, const double *x, double *norm_host);
 int build_rhs(ddx_handle *h, const double *d_phi, double *d_rhs);
 int energy_dot(ddx_handle *h, const double *a, const double *b, double scale,
                double *d_out);
 
 // k_coulomb.hip — all-pairs potential and field (mkrhs.f90, efld.f90)
+// phi(ntrg); with ef != NULL also the field ef(3, ntrg) of the same sources
 int coulomb_potential(ddx_handle *h, int nsrc, const double *q, const double4 *src,
-                      int ntrg, const double *trg3, double *phi);
+                      int ntrg, const double *trg3, double *phi, double *ef);
 int coulomb_field(ddx_handle *h, int nsrc, const double *q, const double *src3,
                   int ntrg, const double *trg3, double *ef);
 int make_psi(ddx_handle *h, const double *charge, double *psi);
@@ -45,7 +59,11 @@ int make_psi(ddx_handle *h, const double *charge, double *psi);
 int forces_dd(ddx_handle *h, const double *d_phi, const double *d_sigma,
               const double *d_s, double *d_fx);
 int make_zeta(ddx_handle *h, const double *d_s, double *d_zeta);
+// solute_field: h->ef already holds the solute field at the cavity points
+// (from coulomb_potential)
 int field_forces(ddx_handle *h, const double *d_charge, const double *d_zeta,
-                 double *d_fx);
+                 double *d_fx, bool solute_field);
+// size of h->ef that field_forces uses
+size_t field_buffer_size(const ddx_handle *h);
 
 }  // namespace ddx

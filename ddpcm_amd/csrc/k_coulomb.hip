@@ -2,6 +2,9 @@
 //   potential  phi(j) = sum_i q_i / |r_j - r_i|            (mkrhs.f90:102-114)
 //   field      E(j)   = sum_i q_i (r_j - r_i) / |r_j - r_i|^3  (efld.f90:98-114)
 // Each thread owns one target; sources stream through LDS in tiles of 256.
+// The potential kernel can return the field of the same sources as well
+// (the solute field at the cavity points that the forces contract with
+// zeta), which costs one all-pairs pass less than a separate field kernel.
 // 1/r comes from the hardware reciprocal square root refined by two Newton
 // steps (relative error a few ulp), which replaces an IEEE sqrt and divide
 // per pair.
@@ -25,10 +28,14 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
   return y;
 }
 
+// phi(j) and, with FIELD, the field ef(:, j) of the same sources, from one
+// evaluation of 1/r per pair
+template <bool FIELD>
 __global__ __launch_bounds__(kCT) void k_potential(int nsrc, const double *__restrict__ q,
                                                    const double4 *__restrict__ src, int ntrg,
                                                    const double *__restrict__ trg,
-                                                   double *__restrict__ phi) {
+                                                   double *__restrict__ phi,
+                                                   double *__restrict__ ef) {
   __shared__ double4 tile[kCT];
   int j = blockIdx.x * kCT + threadIdx.x;
   double tx = 0.0, ty = 0.0, tz = 0.0;
@@ -37,7 +44,7 @@ __global__ __launch_bounds__(kCT) void k_potential(int nsrc, const double *__res
     ty = trg[3 * (size_t)j + 1];
     tz = trg[3 * (size_t)j + 2];
   }
-  double v = 0.0;
+  double v = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0;
   for (int i0 = 0; i0 < nsrc; i0 += kCT) {
     int i = i0 + threadIdx.x;
     if (i < nsrc) {
@@ -50,11 +57,26 @@ __global__ __launch_bounds__(kCT) void k_potential(int nsrc, const double *__res
       double4 s = tile[k];
       double dx = tx - s.x, dy = ty - s.y, dz = tz - s.z;
       double d2 = dx * dx + dy * dy + dz * dz;
-      v = v + s.w * rsqrt_nr(d2);
+      double y = rsqrt_nr(d2);
+      double qy = s.w * y;
+      v = v + qy;
+      if (FIELD) {
+        double f = qy * (y * y);
+        e0 = e0 + f * dx;
+        e1 = e1 + f * dy;
+        e2 = e2 + f * dz;
+      }
     }
     __syncthreads();
   }
-  if (j < ntrg) phi[j] = v;
+  if (j < ntrg) {
+    phi[j] = v;
+    if (FIELD) {
+      ef[3 * (size_t)j] = e0;
+      ef[3 * (size_t)j + 1] = e1;
+      ef[3 * (size_t)j + 2] = e2;
+    }
+  }
 }
 
 // partial fields of sources [split range) at targets; out[(split * ntrg + j) * 3]
@@ -119,11 +141,15 @@ __global__ void k_psi(int n, int nylm, double fac, const double *__restrict__ q,
 }  // namespace
 
 int coulomb_potential(ddx_handle *h, int nsrc, const double *q, const double4 *src, int ntrg,
-                      const double *trg3, double *phi) {
+                      const double *trg3, double *phi, double *ef) {
   if (ntrg <= 0) return DDX_OK;
-  TimedLaunch tl(h, DDX_K_POTENTIAL);
-  hipLaunchKernelGGL(k_potential, dim3((ntrg + kCT - 1) / kCT), dim3(kCT), 0, h->stream, nsrc, q,
-                     src, ntrg, trg3, phi);
+  TimedLaunch tl(h, DDX_K_POTENTIAL, h->stream);
+  if (ef)
+    hipLaunchKernelGGL(k_potential<true>, dim3((ntrg + kCT - 1) / kCT), dim3(kCT), 0, h->stream,
+                       nsrc, q, src, ntrg, trg3, phi, ef);
+  else
+    hipLaunchKernelGGL(k_potential<false>, dim3((ntrg + kCT - 1) / kCT), dim3(kCT), 0, h->stream,
+                       nsrc, q, src, ntrg, trg3, phi, nullptr);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
@@ -145,7 +171,7 @@ int coulomb_field(ddx_handle *h, int nsrc, const double *q, const double *src3, 
   if (nsplit < 1) nsplit = 1;
   DDX_TRY(h->fpart.reserve((size_t)nsplit * 3 * ntrg));
   {
-    TimedLaunch tl(h, DDX_K_FIELD);
+    TimedLaunch tl(h, DDX_K_FIELD, h->stream);
     hipLaunchKernelGGL(k_field, dim3(tb, nsplit), dim3(kCT), 0, h->stream, nsrc, q, src3, ntrg, trg3,
                        chunk > 0 ? chunk : kCT, h->fpart.p);
   }

@@ -335,7 +335,7 @@ __global__ void k_sph3(int n, const double4 *__restrict__ sph, double *__restric
 
 template <int LMAX>
 int launch_triple(ddx_handle *h, const double *sigma, double4 *fb) {
-  TimedLaunch tl(h, DDX_K_FORCES);
+  TimedLaunch tl(h, DDX_K_FORCES, h->stream);
   hipLaunchKernelGGL(k_force_triple<LMAX>, dim3(h->n), dim3(256), 0, h->stream, h->ngrid,
                      h->inl.p, h->pairoff.p, h->sph.p, h->grid.p, h->tgeo.p, h->ttw.p, h->tp.p,
                      sigma, h->facs.p, h->cf.p + 2 * h->nylm, fb);
@@ -379,7 +379,7 @@ int forces_dd(ddx_handle *h, const double *d_phi, const double *d_sigma, const d
   hipStream_t s = h->stream;
   int n = h->n;
   DDX_TRY(h->xi.reserve(h->np));
-  DDX_TRY(expand(h, d_s, h->xi.p, false, nullptr));
+  DDX_TRY(expand(h, h->stream, d_s, h->xi.p, false, nullptr));
   DDX_TRY(h->r.reserve(4 * (size_t)h->ntri + 4));
   double4 *fb = reinterpret_cast<double4 *>(h->r.p);
   DDX_TRY(h->fpart.reserve(6 * (size_t)n + 6));
@@ -405,7 +405,7 @@ int forces_dd(ddx_handle *h, const double *d_phi, const double *d_sigma, const d
 int make_zeta(ddx_handle *h, const double *d_s, double *d_zeta) {
   hipStream_t s = h->stream;
   DDX_TRY(h->xi.reserve(h->np));
-  DDX_TRY(expand(h, d_s, h->xi.p, false, nullptr));
+  DDX_TRY(expand(h, h->stream, d_s, h->xi.p, false, nullptr));
   double scale = 0.5 * ((h->eps - 1.0) / h->eps);
   hipLaunchKernelGGL(k_zeta, dim3((h->np + 255) / 256), dim3(256), 0, s, h->np, h->ngrid, scale,
                      h->grid.p, h->ui.p, h->cavidx.p, h->xi.p, d_zeta);
@@ -413,17 +413,23 @@ int make_zeta(ddx_handle *h, const double *d_s, double *d_zeta) {
   return DDX_OK;
 }
 
-int field_forces(ddx_handle *h, const double *d_charge, const double *d_zeta, double *d_fx) {
+size_t field_buffer_size(const ddx_handle *h) {
+  return 3 * ((size_t)h->ncav + h->n) + 6 + 3 * (size_t)h->n;
+}
+
+int field_forces(ddx_handle *h, const double *d_charge, const double *d_zeta, double *d_fx,
+                 bool solute_field) {
   hipStream_t s = h->stream;
   int n = h->n;
   size_t nef = 3 * ((size_t)h->ncav + n) + 6;
-  DDX_TRY(h->ef.reserve(nef + 3 * (size_t)n));
+  DDX_TRY(h->ef.reserve(field_buffer_size(h)));
   double *c3 = h->ef.p + nef;
   double *ef = h->ef.p, *ez = h->ef.p + 3 * (size_t)h->ncav + 3;
   hipLaunchKernelGGL(k_sph3, dim3((n + 255) / 256), dim3(256), 0, s, n, h->sph.p, c3);
   DDX_HIP(hipGetLastError());
-  // solute field at the cavity points, and the field of zeta at the nuclei
-  DDX_TRY(coulomb_field(h, n, d_charge, c3, h->ncav, h->ccav.p, ef));
+  // solute field at the cavity points (unless the potential kernel made
+  // it), and the field of zeta at the nuclei
+  if (!solute_field) DDX_TRY(coulomb_field(h, n, d_charge, c3, h->ncav, h->ccav.p, ef));
   DDX_TRY(coulomb_field(h, h->ncav, d_zeta, h->ccav.p, n, c3, ez));
   hipLaunchKernelGGL(k_field_contract, dim3(n), dim3(256), 0, s, h->ngrid, h->cavidx.p, d_zeta, ef,
                      d_charge, ez, d_fx);

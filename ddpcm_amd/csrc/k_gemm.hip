@@ -130,20 +130,22 @@ __global__ __launch_bounds__(256) void k_expand(int nylm, int ngrid, int nsph,
 
 }  // namespace
 
-int project(ddx_handle *h, double alpha, const double *P, double *y, const DiisState *gate) {
+int project(ddx_handle *h, hipStream_t s, double alpha, const double *P, double *y,
+            const DiisState *gate) {
   if (h->n <= 0) return DDX_OK;
-  TimedLaunch tl(h, DDX_K_PROJECT);
-  hipLaunchKernelGGL(k_project, dim3((h->n + 15) / 16), dim3(256), 0, h->stream, h->nylm,
+  TimedLaunch tl(h, DDX_K_PROJECT, s);
+  hipLaunchKernelGGL(k_project, dim3((h->n + 15) / 16), dim3(256), 0, s, h->nylm,
                      h->ngrid, h->n, alpha, h->Yw.p, P, y, gate);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
 
-int expand(ddx_handle *h, const double *S, double *xi, bool wscale, const DiisState *gate) {
+int expand(ddx_handle *h, hipStream_t s, const double *S, double *xi, bool wscale,
+           const DiisState *gate) {
   if (h->n <= 0) return DDX_OK;
   dim3 g((h->ngrid + 63) / 64, (h->n + 15) / 16);
-  TimedLaunch tl(h, DDX_K_EXPAND);
-  hipLaunchKernelGGL(k_expand, g, dim3(256), 0, h->stream, h->nylm, h->ngrid, h->n, h->Y.p, S,
+  TimedLaunch tl(h, DDX_K_EXPAND, s);
+  hipLaunchKernelGGL(k_expand, g, dim3(256), 0, s, h->nylm, h->ngrid, h->n, h->Y.p, S,
                      h->grid.p, wscale ? 1 : 0, xi, gate);
   DDX_HIP(hipGetLastError());
   return DDX_OK;

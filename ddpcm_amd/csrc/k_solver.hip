@@ -236,24 +236,31 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(
   }
   if (lane < diis_max) st->slot[lane] = myslot;
 
-  // Gauss-Jordan on a copy, n1 = nmat + 1; lane r holds row r
+  // Gauss-Jordan on a copy, n1 = nmat + 1.  Lane r holds the row that started
+  // as row r and `pos` is its current row index: the reference's row
+  // exchange becomes an exchange of two pos values, the arithmetic on every
+  // element stays the reference's.  Columns already pivoted hold the
+  // in-place inverse, which neither the later pivots nor the solution read,
+  // so only live columns are updated.
   const int n1 = nmat + 1;
   double g[kB];
 #pragma unroll
   for (int c = 0; c < kB; ++c) g[c] = (lane < n1 && c < n1) ? a[c] : 0.0;
   double bv = (lane == 0) ? 1.0 : 0.0;
-  unsigned int pivoted = 0u;   // columns already used as pivot
+  int pos = lane;
+  unsigned int live = (n1 >= 32) ? 0xffffffffu : ((1u << n1) - 1u);   // columns not pivoted
   bool ok = true;
   for (int it = 0; it < n1; ++it) {
+    // pivot: largest |a(j, k)| over rows and columns not pivoted, first in
+    // row-major order of the current row indices on ties
     double big = 0.0, bval = 0.0;
     int best = 0x7fffffff;
-    if (lane < n1 && !((pivoted >> lane) & 1u)) {
+    const bool rowlive = lane < n1 && ((live >> pos) & 1u);
 #pragma unroll
-      for (int c = 0; c < kB; ++c) {
-        if (c < n1 && !((pivoted >> c) & 1u)) {
-          double v = fabs(g[c]);
-          if (v > big) { big = v; bval = g[c]; best = lane * n1 + c; }
-        }
+    for (int c = 0; c < kB; ++c) {
+      if ((live >> c) & 1u) {
+        const double v = fabs(g[c]);
+        if (rowlive && v > big) { big = v; bval = g[c]; best = pos * n1 + c; }
       }
     }
     for (int o = 16; o > 0; o >>= 1) {   // candidates live in lanes 0..24
@@ -265,37 +272,44 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(
     best = __builtin_amdgcn_readfirstlane(best);
     if (best == 0x7fffffff) { ok = false; break; }
     const int irow = best / n1, icol = best % n1;
-    if ((pivoted >> icol) & 1u) { ok = false; break; }
-    pivoted |= 1u << icol;
+    if (!((live >> icol) & 1u)) { ok = false; break; }
+    // the lane holding row irow becomes pivot row icol
+    const int lr = __ffsll((unsigned long long)__ballot(lane < n1 && pos == irow)) - 1;
+    if (lr < 0) { ok = false; break; }
     if (irow != icol) {
-#pragma unroll
-      for (int c = 0; c < kB; ++c) {
-        const double ri = lane_value(g[c], irow), ci = lane_value(g[c], icol);
-        g[c] = (lane == irow) ? ci : ((lane == icol) ? ri : g[c]);
-      }
-      const double rb = lane_value(bv, irow), cb = lane_value(bv, icol);
-      bv = (lane == irow) ? cb : ((lane == icol) ? rb : bv);
+      int np = pos;
+      if (lane < n1 && pos == icol) np = irow;
+      if (lane == lr) np = icol;
+      pos = np;
     }
-    const double d = lane_value(bval, 0);   // the pivot, now in row icol
+    const double d = lane_value(bval, 0);
     if (d == 0.0) { ok = false; break; }
-    const double pinv = 1.0 / d;
-    if (lane == icol) {
+    // each row's multiplier a(j, icol), read before the pivot row is scaled
+    double dum = 0.0;
 #pragma unroll
-      for (int c = 0; c < kB; ++c) g[c] = ((c == icol) ? 1.0 : g[c]) * pinv;
+    for (int c = 0; c < kB; ++c)
+      if ((live >> c) & 1u) dum += (c == icol) ? g[c] : 0.0;
+    live &= ~(1u << icol);
+    const double pinv = 1.0 / d;
+    if (lane == lr) {
+#pragma unroll
+      for (int c = 0; c < kB; ++c)
+        if ((live >> c) & 1u) g[c] = g[c] * pinv;
       bv = bv * pinv;
     }
     // eliminate column icol from the other rows
-    const double dum = sel(g, icol);
-    const double bp = lane_value(bv, icol);
+    const double bp = lane_value(bv, lr);
 #pragma unroll
     for (int c = 0; c < kB; ++c) {
-      const double pc = lane_value(g[c], icol);
-      if (lane != icol) g[c] = ((c == icol) ? 0.0 : g[c]) - pc * dum;
+      if ((live >> c) & 1u) {
+        const double pc = lane_value(g[c], lr);
+        if (lane != lr) g[c] = g[c] - pc * dum;
+      }
     }
-    if (lane != icol) bv = bv - bp * dum;
+    if (lane != lr) bv = bv - bp * dum;
   }
   if (ok) {
-    if (lane >= 1 && lane <= nmat) st->coef[lane - 1] = bv;
+    if (lane < n1 && pos >= 1) st->coef[pos - 1] = bv;
   }
   if (lane == 0) {
     if (!ok) {
@@ -444,9 +458,9 @@ int grid_blocks(size_t work, int per_block, int cap) {
 
 }  // namespace
 
-int device_ldm1x(ddx_handle *h, const double *x, double *y) {
+int device_ldm1x(ddx_handle *h, hipStream_t s, const double *x, double *y) {
   size_t nn = (size_t)h->nylm * h->n;
-  hipLaunchKernelGGL(k_ldm1x, dim3((nn + 255) / 256), dim3(256), 0, h->stream, nn, h->nylm,
+  hipLaunchKernelGGL(k_ldm1x, dim3((nn + 255) / 256), dim3(256), 0, s, nn, h->nylm,
                      h->facl.p, x, y);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
@@ -470,12 +484,13 @@ int device_hnorm(ddx_handle *h, const double *x, double *norm_host) {
 }
 
 int build_rhs(ddx_handle *h, const double *d_phi, double *d_rhs) {
-  DDX_TRY(h->P.reserve(h->np));
+  Lane &ln = h->lane[0];   // lane 0 runs on the handle's stream
+  DDX_TRY(ln.P.reserve(h->np));
   hipLaunchKernelGGL(k_wghpot, dim3((h->np + 255) / 256), dim3(256), 0, h->stream, h->np,
-                     h->ui.p, h->cavidx.p, d_phi, h->P.p);
+                     h->ui.p, h->cavidx.p, d_phi, ln.P.p);
   DDX_HIP(hipGetLastError());
   // rhs(lm, i) = sum_n Y(lm, n) w_n g(n, i)   (intrhs per sphere, cosmo.f90:114-116)
-  return project(h, 1.0, h->P.p, d_rhs, nullptr);
+  return project(h, h->stream, 1.0, ln.P.p, d_rhs, nullptr);
 }
 
 int energy_dot(ddx_handle *h, const double *a, const double *b, double scale, double *d_out) {
@@ -490,9 +505,19 @@ int energy_dot(ddx_handle *h, const double *a, const double *b, double scale, do
   return DDX_OK;
 }
 
-int solve_jacobi_diis(ddx_handle *h, int star, int diis_max, double tol, const double *d_rhs,
-                      double *d_x, int max_iter, int *n_iter, int *ok,
-                      std::vector<double> *hist) {
+namespace {
+// kernel classes launched once per iteration of a solve
+void per_iter_classes(int star, int cls[5]) {
+  cls[0] = star ? DDX_K_SWEEP_LSTAR : DDX_K_SWEEP_L;
+  cls[1] = star ? DDX_K_EXPAND : DDX_K_PROJECT;
+  cls[2] = DDX_K_JACOBI;
+  cls[3] = DDX_K_DIIS;
+  cls[4] = DDX_K_EXTRAP;
+}
+}  // namespace
+
+int solve_begin(ddx_handle *h, Lane &ln, int star, int diis_max, double tol, const double *d_rhs,
+                double *d_x, int max_iter) {
   if (!(diis_max == 0 || (diis_max > 10 && diis_max <= kDiisMax))) {
     set_error("diis_max must be 0 or in 11..%d", kDiisMax);
     return DDX_ERR_ARG;
@@ -501,23 +526,31 @@ int solve_jacobi_diis(ddx_handle *h, int star, int diis_max, double tol, const d
     set_error("n_iter must be in 1..%d", kMaxIter);
     return DDX_ERR_ARG;
   }
-  hipStream_t s = h->stream;
   size_t nn = (size_t)h->nylm * h->n;
-  int nbj = grid_blocks(nn, kJThreads, kMaxBlk);
-  int nbe = grid_blocks(h->n, kJThreads / kWave, 1024);
-  DDX_TRY(h->y.reserve(nn));
-  DDX_TRY(h->xnew.reserve(nn));
+  DDX_TRY(ln.y.reserve(nn));
+  DDX_TRY(ln.xnew.reserve(nn));
   if (diis_max > 0) {
-    DDX_TRY(h->xd.reserve(nn * diis_max));
-    DDX_TRY(h->ed.reserve(nn * diis_max));
+    DDX_TRY(ln.xd.reserve(nn * diis_max));
+    DDX_TRY(ln.ed.reserve(nn * diis_max));
   }
-  DDX_TRY(h->part.reserve((size_t)kMaxBlk * kPartStride + 2048));
-  DDX_TRY(h->diis.reserve(2));
-  DDX_TRY(h->misc.reserve(16));
-  double *part_e = h->part.p + (size_t)1024 * kPartStride;
-  int *newslot = h->misc.p;
+  DDX_TRY(ln.part.reserve((size_t)kMaxBlk * kPartStride + 2048));
+  DDX_TRY(ln.diis.reserve(2));
+  DDX_TRY(ln.misc.reserve(16));
+  ln.serial = h->solve_serial++;
+  ln.star = star;
+  ln.diis_max = diis_max;
+  ln.max_iter = max_iter;
+  ln.tol = tol;
+  ln.rhs = d_rhs;
+  ln.x = d_x;
+  // with DIIS the norm of iteration k is evaluated by the DIIS kernel of
+  // iteration k + 1, so up to max_iter + 1 rounds are enqueued
+  ln.rounds = diis_max > 0 ? max_iter + 1 : max_iter;
+  ln.enqueued = 0;
+  ln.group = 3;
+  ln.active = true;
 
-  DiisState *hs = h->h_diis;
+  DiisState *hs = ln.h_diis;
   hs->nmat = 1;
   hs->it = 0;
   hs->done = 0;
@@ -527,64 +560,101 @@ int solve_jacobi_diis(ddx_handle *h, int star, int diis_max, double tol, const d
   hs->pending_norm = 0;
   for (int q = 0; q < kDiisMax; ++q) hs->slot[q] = q;
   hs->norm = 0.0;
-  DiisState *ds = h->diis.p;
-  DDX_HIP(hipMemcpyAsync(ds, hs, sizeof(DiisState), hipMemcpyHostToDevice, s));
+  DDX_HIP(hipMemcpyAsync(ln.diis.p, hs, sizeof(DiisState), hipMemcpyHostToDevice, ln.stream));
+  return DDX_OK;
+}
 
+int solve_enqueue(ddx_handle *h, Lane &ln) {
+  hipStream_t s = ln.stream;
+  size_t nn = (size_t)h->nylm * h->n;
+  int nbj = grid_blocks(nn, kJThreads, kMaxBlk);
+  int nbe = grid_blocks(h->n, kJThreads / kWave, 1024);
+  double *part_e = ln.part.p + (size_t)1024 * kPartStride;
+  int *newslot = ln.misc.p;
+  DiisState *ds = ln.diis.p;
   const double *invl1 = h->cf.p + h->nylm;   // 1/(l+1) table stored after cf
-  // kernel classes launched once per iteration: launches enqueued after
-  // convergence return at once and are dropped from their timers below
-  const int per_iter[5] = {star ? DDX_K_SWEEP_LSTAR : DDX_K_SWEEP_L,
-                           star ? DDX_K_EXPAND : DDX_K_PROJECT, DDX_K_JACOBI, DDX_K_DIIS,
-                           DDX_K_EXTRAP};
-  int used0[5];
-  for (int k = 0; k < 5; ++k) used0[k] = h->ktimer[per_iter[k]].used;
-  int enqueued = 0, group = 3;
-  while (true) {
-    // with DIIS the norm of iteration k is evaluated by the DIIS kernel of
-    // iteration k + 1, so up to max_iter + 1 rounds are enqueued
-    const int rounds = diis_max > 0 ? max_iter + 1 : max_iter;
-    for (int g = 0; g < group && enqueued < rounds; ++g, ++enqueued) {
-      if (star) DDX_TRY(apply_lstarx(h, d_x, h->y.p, ds));
-      else DDX_TRY(apply_lx(h, d_x, h->y.p, ds));
-      {
-        TimedLaunch tl(h, DDX_K_JACOBI);
-        hipLaunchKernelGGL(k_jacobi, dim3(nbj), dim3(kJThreads), 0, s, nn, h->nylm, diis_max,
-                         h->facl.p, d_rhs, h->y.p, d_x, h->xd.p, h->ed.p, h->xnew.p, h->part.p, ds);
-      }
+  const int diis_max = ln.diis_max;
+  h->tag_solve = ln.serial;
+  for (int g = 0; g < ln.group && ln.enqueued < ln.rounds; ++g, ++ln.enqueued) {
+    h->tag_round = ln.enqueued;
+    if (ln.star) DDX_TRY(apply_lstarx(h, ln, ln.x, ln.y.p, ds));
+    else DDX_TRY(apply_lx(h, ln, ln.x, ln.y.p, ds));
+    {
+      TimedLaunch tl(h, DDX_K_JACOBI, s);
+      hipLaunchKernelGGL(k_jacobi, dim3(nbj), dim3(kJThreads), 0, s, nn, h->nylm, diis_max,
+                         h->facl.p, ln.rhs, ln.y.p, ln.x, ln.xd.p, ln.ed.p, ln.xnew.p, ln.part.p,
+                         ds);
+    }
+    DDX_HIP(hipGetLastError());
+    if (diis_max > 0) {
+      TimedLaunch tl(h, DDX_K_DIIS, s);
+      hipLaunchKernelGGL(k_diis_small, dim3(1), dim3(kSmallThreads), 0, s, nbj, diis_max,
+                         ln.part.p, nbe, part_e, h->n, ln.tol, ln.max_iter, ds, newslot);
       DDX_HIP(hipGetLastError());
-      if (diis_max > 0) {
-        TimedLaunch tl(h, DDX_K_DIIS);
-        hipLaunchKernelGGL(k_diis_small, dim3(1), dim3(kSmallThreads), 0, s, nbj, diis_max,
-                           h->part.p, nbe, part_e, h->n, tol, max_iter, ds, newslot);
-        DDX_HIP(hipGetLastError());
-      }
-      {
-        TimedLaunch tl(h, DDX_K_EXTRAP);
-        hipLaunchKernelGGL(k_extrap, dim3(nbe), dim3(kJThreads), 0, s, h->n, h->nylm, nn, diis_max,
-                           h->xd.p, h->xnew.p, newslot, invl1, d_x, part_e, ds);
-      }
+    }
+    {
+      TimedLaunch tl(h, DDX_K_EXTRAP, s);
+      hipLaunchKernelGGL(k_extrap, dim3(nbe), dim3(kJThreads), 0, s, h->n, h->nylm, nn, diis_max,
+                         ln.xd.p, ln.xnew.p, newslot, invl1, ln.x, part_e, ds);
+    }
+    DDX_HIP(hipGetLastError());
+    if (diis_max == 0) {
+      hipLaunchKernelGGL(k_norm, dim3(1), dim3(kWave), 0, s, nbe, h->n, ln.tol, ln.max_iter,
+                         part_e, ds);
       DDX_HIP(hipGetLastError());
-      if (diis_max == 0) {
-        hipLaunchKernelGGL(k_norm, dim3(1), dim3(kWave), 0, s, nbe, h->n, tol, max_iter, part_e,
-                           ds);
-        DDX_HIP(hipGetLastError());
+    }
+  }
+  h->tag_solve = -1;
+  h->tag_round = 0;
+  ln.group = 2;
+  DDX_HIP(hipMemcpyAsync(ln.h_diis, ds, sizeof(DiisState), hipMemcpyDeviceToHost, s));
+  return DDX_OK;
+}
+
+int solve_poll(ddx_handle *h, Lane &ln, bool *finished) {
+  (void)h;
+  DDX_HIP(hipStreamSynchronize(ln.stream));
+  *finished = ln.h_diis->done || ln.enqueued >= ln.rounds;
+  return DDX_OK;
+}
+
+int solve_end(ddx_handle *h, Lane &ln, int *n_iter, int *ok, std::vector<double> *hist) {
+  const DiisState *hs = ln.h_diis;
+  ln.active = false;
+  // launches of rounds after the last iteration returned at once: drop
+  // them from the timers
+  int cls[5];
+  per_iter_classes(ln.star, cls);
+  for (int c : cls) {
+    KernelTimer &t = h->ktimer[c];
+    size_t keep = 0;
+    for (size_t k = 0; k < t.recs.size(); ++k) {
+      const KernelTimer::Rec &r = t.recs[k];
+      if (r.solve == ln.serial && r.round >= hs->it) {
+        t.pool.push_back(r.b);
+        t.pool.push_back(r.e);
+      } else {
+        t.recs[keep++] = r;
       }
     }
-    DDX_HIP(hipMemcpyAsync(hs, ds, sizeof(DiisState), hipMemcpyDeviceToHost, s));
-    DDX_HIP(hipStreamSynchronize(s));
-    if (hs->done || enqueued >= (diis_max > 0 ? max_iter + 1 : max_iter)) break;
-    group = 2;
+    t.recs.resize(keep);
   }
-  for (int k = 0; k < 5; ++k) {
-    KernelTimer &t = h->ktimer[per_iter[k]];
-    if (t.on && t.used > used0[k] + hs->it) t.used = used0[k] + hs->it;
-  }
-  *n_iter = hs->ok ? hs->it : max_iter + 1;
+  *n_iter = hs->ok ? hs->it : ln.max_iter + 1;
   *ok = hs->ok;
-  if (hist) {
-    hist->assign(hs->hist, hs->hist + (hs->it < kMaxIter ? hs->it : kMaxIter));
-  }
+  if (hist) hist->assign(hs->hist, hs->hist + (hs->it < kMaxIter ? hs->it : kMaxIter));
   return DDX_OK;
+}
+
+int solve_jacobi_diis(ddx_handle *h, Lane &ln, int star, int diis_max, double tol,
+                      const double *d_rhs, double *d_x, int max_iter, int *n_iter, int *ok,
+                      std::vector<double> *hist) {
+  DDX_TRY(solve_begin(h, ln, star, diis_max, tol, d_rhs, d_x, max_iter));
+  bool finished = false;
+  while (!finished) {
+    DDX_TRY(solve_enqueue(h, ln));
+    DDX_TRY(solve_poll(h, ln, &finished));
+  }
+  return solve_end(h, ln, n_iter, ok, hist);
 }
 
 }  // namespace ddx

@@ -196,18 +196,20 @@ __global__ __launch_bounds__(kWave) void k_sweep_adjoint(
 }
 
 template <int LMAX>
-int launch_contract(ddx_handle *h, const double *X, double *r, const DiisState *gate) {
-  TimedLaunch tl(h, DDX_K_SWEEP_L);
-  hipLaunchKernelGGL(k_sweep_contract<LMAX>, dim3(h->n), dim3(kLThreads), 0, h->stream,
+int launch_contract(ddx_handle *h, hipStream_t s, const double *X, double *r,
+                    const DiisState *gate) {
+  TimedLaunch tl(h, DDX_K_SWEEP_L, s);
+  hipLaunchKernelGGL(k_sweep_contract<LMAX>, dim3(h->n), dim3(kLThreads), 0, s,
                      h->inl.p, h->pairoff.p, h->tgeo.p, h->ttw.p, h->tslot.p, X, h->cf.p, r, gate);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
 
 template <int LMAX>
-int launch_adjoint(ddx_handle *h, const double *xi, double *y, const DiisState *gate) {
-  TimedLaunch tl(h, DDX_K_SWEEP_LSTAR);
-  hipLaunchKernelGGL(k_sweep_adjoint<LMAX>, dim3(h->n), dim3(kWave), 0, h->stream, h->inl.p,
+int launch_adjoint(ddx_handle *h, hipStream_t s, const double *xi, double *y,
+                   const DiisState *gate) {
+  TimedLaunch tl(h, DDX_K_SWEEP_LSTAR, s);
+  hipLaunchKernelGGL(k_sweep_adjoint<LMAX>, dim3(h->n), dim3(kWave), 0, s, h->inl.p,
                      h->pairoff.p, h->tgeo.p, h->ttw.p, h->tp.p, xi, h->cf.p, y, gate);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
@@ -229,28 +231,31 @@ int launch_adjoint(ddx_handle *h, const double *xi, double *y, const DiisState *
 
 }  // namespace
 
-int sweep_contract(ddx_handle *h, const double *X, double *r, const DiisState *gate) {
+int sweep_contract(ddx_handle *h, hipStream_t s, const double *X, double *r,
+                   const DiisState *gate) {
   if (h->n == 0) return DDX_OK;
-  DDX_LMAX_CASES(launch_contract, h, X, r, gate);
+  DDX_LMAX_CASES(launch_contract, h, s, X, r, gate);
 }
 
-int apply_lx(ddx_handle *h, const double *x, double *y, const DiisState *gate) {
-  DDX_TRY(h->r.reserve(h->ntri + 1));
-  DDX_TRY(h->P.reserve(h->np));
-  DDX_TRY(sweep_contract(h, x, h->r.p, gate));
-  hipLaunchKernelGGL(k_gather_points, dim3((h->np + 255) / 256), dim3(256), 0, h->stream, h->np,
-                     h->ptoff.p, h->r.p, h->ui.p, h->P.p, gate);
+int apply_lx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate) {
+  hipStream_t s = ln.stream;
+  DDX_TRY(ln.rw.reserve(h->ntri + 1));
+  DDX_TRY(ln.P.reserve(h->np));
+  DDX_TRY(sweep_contract(h, s, x, ln.rw.p, gate));
+  hipLaunchKernelGGL(k_gather_points, dim3((h->np + 255) / 256), dim3(256), 0, s, h->np,
+                     h->ptoff.p, ln.rw.p, h->ui.p, ln.P.p, gate);
   DDX_HIP(hipGetLastError());
   // y(lm, o) = - sum_n Yw(lm, n) P(n, o)
-  return project(h, -1.0, h->P.p, y, gate);
+  return project(h, s, -1.0, ln.P.p, y, gate);
 }
 
-int apply_lstarx(ddx_handle *h, const double *x, double *y, const DiisState *gate) {
-  DDX_TRY(h->xiw.reserve(h->np));
+int apply_lstarx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate) {
+  hipStream_t s = ln.stream;
+  DDX_TRY(ln.xiw.reserve(h->np));
   // w_n xi(n, o) = w_n sum_lm Y(lm, n) x(lm, o)
-  DDX_TRY(expand(h, x, h->xiw.p, true, gate));
+  DDX_TRY(expand(h, s, x, ln.xiw.p, true, gate));
   if (h->n == 0) return DDX_OK;
-  DDX_LMAX_CASES(launch_adjoint, h, h->xiw.p, y, gate);
+  DDX_LMAX_CASES(launch_adjoint, h, s, ln.xiw.p, y, gate);
 }
 
 }  // namespace ddx
