@@ -34,7 +34,7 @@ METRIC = "E+forces solves/s at conv 1e-8: mixed+learned vs plain FP64 GPU; HBM G
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6      # MI355X FP64, vector and dense matrix alike
 KERNELS = ("sweep_l", "sweep_lstar", "project", "expand", "jacobi", "diis", "extrap", "field",
-           "potential", "forces")
+           "potential", "forces", "sweep_pair")
 
 
 def diis_depths(niter, ndiis=25):
@@ -50,11 +50,13 @@ def diis_depths(niter, ndiis=25):
 
 
 def algorithmic_work(h, st, d):
-    """Per kernel class: (unit, work per solve, launches per solve).  Counting
-    convention: sweeps T (7 nylm + 40) flops per apply; MFMA products
-    2 nylm ngrid N flops; all-pairs 20 ncav N flops per kernel; Jacobi update
-    (4 + m) n 8 bytes (rhs, y, x, m - 1 history residuals read; x_new, e
-    written) and extrapolation (m + 2) n 8 bytes at DIIS depth m."""
+    """Per kernel class: (unit, work per solve, launches per solve) for an
+    energy + forces solve.  Counting convention: sweeps T (7 nylm + 40) flops
+    per operator apply; MFMA products 2 nylm ngrid N flops; all-pairs
+    20 ncav N flops per pass (the potential pass also returns the field, the
+    field kernel is the zeta field at the nuclei); Jacobi update (4 + m) n 8
+    bytes (rhs, y, x, m - 1 history residuals read; x_new, e written) and
+    extrapolation (m + 2) n 8 bytes at DIIS depth m."""
     n = h.nylm * h.nsph
     T = st.ntriples
     sweep = T * (7.0 * h.nylm + 40.0)
@@ -64,14 +66,16 @@ def algorithmic_work(h, st, d):
     jac = sum((4 + m) * n * 8.0 for m in depth)
     ext = sum((m + 2) * n * 8.0 for m in depth)
     pairs = 20.0 * h.ncav * h.nsph
+    # with forces the two solves iterate in lockstep: one sweep, Jacobi, DIIS
+    # and extrapolation launch per round serves both
+    rounds = max(nd, na)
     return {
-        "sweep_l": ("flop", sweep * nd, nd),
-        "sweep_lstar": ("flop", sweep * na, na),
+        "sweep_pair": ("flop", sweep * (nd + na), rounds),
         "project": ("flop", mm * (nd + 1), nd + 1),
         "expand": ("flop", mm * (na + 2), na + 2),
-        "jacobi": ("byte", jac, nd + na),
-        "extrap": ("byte", ext, nd + na),
-        "field": ("flop", 2 * pairs, 2),
+        "jacobi": ("byte", jac, rounds),
+        "extrap": ("byte", ext, rounds),
+        "field": ("flop", pairs, 1),
         "potential": ("flop", pairs, 1),
     }
 
@@ -207,8 +211,8 @@ def main():
                                    args.workload, d.n, d.lmax, h.ngrid, d.eps, d.eta, d.iconv),
                    "precision": args.precision, "nsph": d.n, "ncav": h.ncav,
                    "ntriples": st.ntriples},
-        "stages_ms": {"setup": st.ms_setup, "rhs": st.ms_rhs, "direct": st.ms_direct,
-                      "adjoint": st.ms_adjoint, "forces": st.ms_forces},
+        "stages_ms": {"setup": st.ms_setup, "rhs": st.ms_rhs,
+                      "solves": st.ms_direct + st.ms_adjoint, "forces": st.ms_forces},
         "iterations": {"direct": st.niter_direct, "adjoint": st.niter_adjoint},
         "energy_hartree": es, "parity_vs_reference": parity,
         "roofline": roofline,

@@ -35,7 +35,6 @@ int download(double *dst, const double *src, size_t n, hipStream_t s) {
 
 int sync(ddx_handle *h) {
   DDX_HIP(hipStreamSynchronize(h->stream));
-  if (h->lane[1].stream) DDX_HIP(hipStreamSynchronize(h->lane[1].stream));
   return DDX_OK;
 }
 
@@ -100,32 +99,12 @@ int noconv(bool star) {
   return DDX_ERR_NOCONV;
 }
 
-// Runs the solves begun on the given lanes to their ends, enqueueing the
-// next group of iterations on every lane before waiting on any of them.
-int drive(ddx_handle *h, Lane *const *lanes, int count) {
-  bool left = true;
-  while (left) {
-    left = false;
-    for (int k = 0; k < count; ++k)
-      if (lanes[k]->active) DDX_TRY(solve_enqueue(h, *lanes[k]));
-    for (int k = 0; k < count; ++k) {
-      Lane &ln = *lanes[k];
-      if (!ln.active) continue;
-      bool fin = false;
-      DDX_TRY(solve_poll(h, ln, &fin));
-      if (fin) ln.active = false;   // solve_end reports it
-      else left = true;
-    }
-  }
-  return DDX_OK;
-}
-
-// Direct solve on lane 0 (the handle's stream); X in h->sigma, E in energy[0].
+// Direct solve on lane 0; X in h->sigma, E in energy[0].
 int run_direct(ddx_handle *h, const double *d_psi) {
   Lane *ln = &h->lane[0];
   int ok = 0;
   DDX_TRY(begin_direct(h, *ln));
-  DDX_TRY(drive(h, &ln, 1));
+  DDX_TRY(solve_run(h, &ln, 1));
   DDX_TRY(end_direct(h, *ln, d_psi, &ok));
   return ok ? DDX_OK : noconv(false);
 }
@@ -134,7 +113,7 @@ int run_adjoint(ddx_handle *h, const double *d_psi) {
   Lane *ln = &h->lane[0];
   int ok = 0;
   DDX_TRY(begin_adjoint(h, *ln, d_psi));
-  DDX_TRY(drive(h, &ln, 1));
+  DDX_TRY(solve_run(h, &ln, 1));
   DDX_TRY(end_adjoint(h, *ln, &ok));
   return ok ? DDX_OK : noconv(true);
 }
@@ -166,24 +145,23 @@ int pipeline(ddx_handle *h) {
   DDX_TRY(make_psi(h, h->charge.p, h->psi.p));
   DDX_TRY(build_rhs(h, h->phi.p, h->rhs.p));
   DDX_HIP(hipEventRecord(h->ev[2], s));
-  // the direct solve on lane 0 and, with forces, the adjoint solve on lane 1
-  // at the same time: neither depends on the other (cosmo.f90:136-196)
-  Lane *lanes[2] = {&h->lane[0], &h->lane[1]};
-  const int nl = h->igrad ? 2 : 1;
-  hipStream_t s1 = h->lane[1].stream;
-  if (h->igrad) {
-    DDX_HIP(hipStreamWaitEvent(s1, h->ev[2], 0));
-    DDX_TRY(begin_adjoint(h, h->lane[1], h->psi.p));
-  }
-  DDX_TRY(begin_direct(h, h->lane[0]));
-  DDX_TRY(drive(h, lanes, nl));
   int ok_d = 0, ok_a = 1;
-  DDX_TRY(end_direct(h, h->lane[0], h->psi.p, &ok_d));
-  DDX_HIP(hipEventRecord(h->ev[3], s));
   if (h->igrad) {
+    // the direct and the adjoint solve in lockstep: neither depends on the
+    // other (cosmo.f90:136-196), and each sweep builds the basis for both
+    Lane *lanes[2] = {&h->lane[0], &h->lane[1]};
+    DDX_TRY(begin_direct(h, h->lane[0]));
+    DDX_TRY(begin_adjoint(h, h->lane[1], h->psi.p));
+    DDX_TRY(solve_run(h, lanes, 2));
+    DDX_TRY(end_direct(h, h->lane[0], h->psi.p, &ok_d));
     DDX_TRY(end_adjoint(h, h->lane[1], &ok_a));
-    DDX_HIP(hipEventRecord(h->ev[4], s1));
+  } else {
+    Lane *ln = &h->lane[0];
+    DDX_TRY(begin_direct(h, *ln));
+    DDX_TRY(solve_run(h, &ln, 1));
+    DDX_TRY(end_direct(h, *ln, h->psi.p, &ok_d));
   }
+  DDX_HIP(hipEventRecord(h->ev[3], s));
   if (!ok_d) {
     DDX_TRY(sync(h));
     return noconv(false);
@@ -193,24 +171,22 @@ int pipeline(ddx_handle *h) {
     return noconv(true);
   }
   if (h->igrad) {
-    DDX_HIP(hipStreamWaitEvent(s, h->ev[4], 0));
-    DDX_HIP(hipEventRecord(h->ev[7], s));
     DDX_TRY(forces_dd(h, h->phi.p, h->sigma.p, h->s.p, h->fx.p));
     DDX_TRY(make_zeta(h, h->s.p, h->zeta.p));
     DDX_TRY(field_forces(h, h->charge.p, h->zeta.p, h->fx.p, true));
     DDX_HIP(hipEventRecord(h->ev[5], s));
   }
   DDX_TRY(sync(h));
-  // the two solves overlap: each is timed from the end of the RHS
+  // with forces the two solves run together: ms_direct covers both and
+  // ms_adjoint is zero
   float ms = 0.f;
   DDX_HIP(hipEventElapsedTime(&ms, h->ev[1], h->ev[2]));
   h->stats.ms_rhs = ms;
   DDX_HIP(hipEventElapsedTime(&ms, h->ev[2], h->ev[3]));
   h->stats.ms_direct = ms;
+  h->stats.ms_adjoint = 0.0;
   if (h->igrad) {
-    DDX_HIP(hipEventElapsedTime(&ms, h->ev[2], h->ev[4]));
-    h->stats.ms_adjoint = ms;
-    DDX_HIP(hipEventElapsedTime(&ms, h->ev[7], h->ev[5]));
+    DDX_HIP(hipEventElapsedTime(&ms, h->ev[3], h->ev[5]));
     h->stats.ms_forces = ms;
   }
   return DDX_OK;
@@ -266,12 +242,11 @@ int ddx_init(ddx_handle **out, int n, const double *x, const double *y, const do
     ddx_free(h);
     return code;
   };
-  if (hipStreamCreate(&h->stream) != hipSuccess ||
-      hipStreamCreate(&h->lane[1].stream) != hipSuccess) {
+  if (hipStreamCreate(&h->stream) != hipSuccess) {
     set_error("hipStreamCreate failed");
     return fail(DDX_ERR_HIP);
   }
-  h->lane[0].stream = h->stream;
+  for (Lane &ln : h->lane) ln.stream = h->stream;
   for (auto &e : h->ev)
     if (hipEventCreate(&e) != hipSuccess) {
       set_error("hipEventCreate failed");
@@ -324,7 +299,6 @@ int ddx_init(ddx_handle **out, int n, const double *x, const double *y, const do
 int ddx_free(ddx_handle *h) {
   if (!h) return DDX_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  if (h->lane[1].stream) (void)hipStreamSynchronize(h->lane[1].stream);
   for (Lane &ln : h->lane)
     if (ln.h_diis) (void)hipHostFree(ln.h_diis);
   for (auto &t : h->ktimer) {
@@ -336,10 +310,9 @@ int ddx_free(ddx_handle *h) {
   }
   for (auto &e : h->ev)
     if (e) (void)hipEventDestroy(e);
-  hipStream_t s = h->stream, s1 = h->lane[1].stream;
+  hipStream_t s = h->stream;
   delete h;   // device arrays release themselves
   if (s) (void)hipStreamDestroy(s);
-  if (s1) (void)hipStreamDestroy(s1);
   return DDX_OK;
 }
 
@@ -494,7 +467,7 @@ static int op_apply(ddx_handle *h, int which, const double *x, double *y) {
   DDX_TRY(reserve_vectors(h));
   DDX_TRY(h->y.reserve(nn));
   DDX_TRY(upload(h->x, x, nn, s));
-  if (which == 0) DDX_TRY(apply_lx(h, h->lane[0], h->x.p, h->y.p, nullptr));
+  if (which == 0) DDX_TRY(apply_lx(h, h->lane[0], h->x.p, h->y.p, nullptr, false));
   else if (which == 1) DDX_TRY(apply_lstarx(h, h->lane[0], h->x.p, h->y.p, nullptr));
   else DDX_TRY(device_ldm1x(h, s, h->x.p, h->y.p));
   DDX_TRY(download(y, h->y.p, nn, s));

@@ -16,6 +16,7 @@ constexpr int kMaxLmax = 16;       // template instantiations of the sweeps
 constexpr int kMaxDegree = 64;     // neighbours per sphere (64-bit point masks)
 constexpr int kDiisMax = 25;       // ndiis of the reference (ddcosmo.f90:87)
 constexpr int kMaxIter = 200;      // n_iter of cosmo (cosmo.f90:81)
+constexpr int kTripleChunk = 512;  // triples per sweep work unit (8 per lane)
 
 void set_error(const char *fmt, ...);
 
@@ -101,7 +102,7 @@ struct KernelTimer {
 // sweeps of the other.
 struct Lane {
   hipStream_t stream = nullptr;
-  DevArray<double> y, xnew, P, xiw, rw, xd, ed, part;
+  DevArray<double> y, xnew, xs, P, xiw, rw, yp, xd, ed, part;
   DevArray<DiisState> diis;
   DevArray<int> misc;
   DiisState *h_diis = nullptr;  // pinned host mirror of diis
@@ -154,6 +155,9 @@ struct ddx_handle {
 
   // triples grouped by the sphere that contains the point
   ddx::DevArray<int> paircnt, pairoff;
+  ddx::DevArray<int> chcnt, choff;  // sweep chunks per container and their offsets
+  ddx::DevArray<int2> chunk;        // (container, first triple) per chunk
+  int nchunk = 0;
   ddx::DevArray<double4> tgeo;    // (cos theta, sin theta, cos phi, sin phi)
   ddx::DevArray<double2> ttw;     // (t, W)
   ddx::DevArray<int> tp;          // owner point index i*ngrid + n

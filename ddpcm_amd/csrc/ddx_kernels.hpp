@@ -20,23 +20,27 @@ int project(ddx_handle *h, hipStream_t s, double alpha, const double *P, double 
 int expand(ddx_handle *h, hipStream_t s, const double *S, double *xi, bool wscale,
            const DiisState *gate);
 
-// k_sweep.hip — off-diagonal operators (matvec.f90:94-235)
-// (scratch and stream of the lane)
-int apply_lx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate);
+// k_sweep.hip — off-diagonal operators (matvec.f90:94-235), on the scratch
+// and stream of a lane.  L X reads X' = cf X from ln.xs: apply_lx makes it
+// unless xs_ready says that ln.xs already holds cf x.
+int apply_lx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate,
+             bool xs_ready);
 int apply_lstarx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate);
-// per-triple W_t sum_lm R_lm(t) X_{c,lm}, stored in point-slot order
-int sweep_contract(ddx_handle *h, hipStream_t s, const double *X, double *rw,
-                   const DiisState *gate);
+// yd = L xd (ld.xs holds cf xd) and ya = L* xa in one pass over the
+// triples, gated by the DIIS states of the two lanes
+int apply_pair(ddx_handle *h, Lane &ld, double *yd, Lane &la, const double *xa, double *ya);
+// xs = cf x
+int cf_scale(ddx_handle *h, hipStream_t s, const double *x, double *xs);
 
 // k_solver.hip — Jacobi/DIIS (jacobi_diis.f90:43-438), RHS, norms
-// A solve on a lane: solve_begin sets it up (x holds the guess), each
-// solve_enqueue queues the next group of iterations and the copy-back of the
-// DIIS state, solve_poll waits for that copy and says whether the solve has
-// ended, solve_end reports it.  solve_jacobi_diis runs all four.
+// Solves on lanes: solve_begin sets one up (x holds the guess); solve_run
+// iterates one solve, or a direct (star 0) and an adjoint (star 1) solve in
+// lockstep on one stream, enqueueing groups of iterations and reading the
+// DIIS states back between groups; solve_end reports each.
+// solve_jacobi_diis runs the three for one solve.
 int solve_begin(ddx_handle *h, Lane &ln, int star, int diis_max, double tol,
                 const double *d_rhs, double *d_x, int max_iter);
-int solve_enqueue(ddx_handle *h, Lane &ln);
-int solve_poll(ddx_handle *h, Lane &ln, bool *finished);
+int solve_run(ddx_handle *h, Lane *const *lanes, int count);
 int solve_end(ddx_handle *h, Lane &ln, int *n_iter, int *ok, std::vector<double> *hist);
 int solve_jacobi_diis(ddx_handle *h, Lane &ln, int star, int diis_max, double tol,
                       const double *d_rhs, double *d_x, int max_iter,

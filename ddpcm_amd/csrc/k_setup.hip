@@ -210,6 +210,25 @@ __global__ __launch_bounds__(kSetupThreads) void k_pair_fill(
   }
 }
 
+// Chunks of at most kTripleChunk triples of one container, the work unit of
+// the sweeps: number of chunks per container ...
+__global__ void k_chunk_count(int n, const int *__restrict__ inl, const int *__restrict__ pairoff,
+                              int *__restrict__ cnt) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  int t = pairoff[inl[c + 1]] - pairoff[inl[c]];
+  cnt[c] = (t + kTripleChunk - 1) / kTripleChunk;
+}
+
+// ... and (container, first triple) of each chunk
+__global__ void k_chunk_fill(int n, const int *__restrict__ inl, const int *__restrict__ pairoff,
+                             const int *__restrict__ choff, int2 *__restrict__ chunk) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  int beg = pairoff[inl[c]];
+  for (int j = choff[c]; j < choff[c + 1]; ++j, beg += kTripleChunk) chunk[j] = make_int2(c, beg);
+}
+
 }  // namespace
 
 int run_setup(ddx_handle *h) {
@@ -281,9 +300,22 @@ int run_setup(ddx_handle *h) {
   } else {
     DDX_HIP(hipMemsetAsync(h->pairoff.p, 0, sizeof(int), s));
   }
+  DDX_TRY(h->chcnt.reserve(n + 1));
+  DDX_TRY(h->choff.reserve(n + 1));
+  hipLaunchKernelGGL(k_chunk_count, dim3((n + 255) / 256), dim3(256), 0, s, n, h->inl.p,
+                     h->pairoff.p, h->chcnt.p);
+  DDX_HIP(hipGetLastError());
+  DDX_TRY(scan_exclusive(h->chcnt.p, h->choff.p, n, h->scan_tmp, s));
+  int nchunk = 0;
+  DDX_HIP(hipMemcpyAsync(&nchunk, h->choff.p + n, sizeof(int), hipMemcpyDeviceToHost, s));
   DDX_HIP(hipStreamSynchronize(s));
   h->ntri = ntri;
   h->ncav = ncav;
+  h->nchunk = nchunk;
+  DDX_TRY(h->chunk.reserve(nchunk + 1));
+  hipLaunchKernelGGL(k_chunk_fill, dim3((n + 255) / 256), dim3(256), 0, s, n, h->inl.p,
+                     h->pairoff.p, h->choff.p, h->chunk.p);
+  DDX_HIP(hipGetLastError());
 
   DDX_TRY(h->tgeo.reserve(ntri + 1));
   DDX_TRY(h->ttw.reserve(ntri + 1));

@@ -23,12 +23,34 @@ constexpr int kJThreads = 256;
 constexpr int kPartStride = kDiisMax + 1;   // 25 dots + self dot per block
 constexpr int kMaxBlk = 1024;               // workgroups of k_jacobi (partials per dot)
 
-__global__ __launch_bounds__(kJThreads) void k_jacobi(
-    size_t nn, int nylm, int diis_max, const double *__restrict__ facl,
-    const double *__restrict__ rhs, const double *__restrict__ y, const double *__restrict__ x,
-    double *__restrict__ xd, double *__restrict__ ed, double *__restrict__ xnew,
-    double *__restrict__ part, const DiisState *__restrict__ st) {
+// Pointers of one system's Jacobi/DIIS iteration; the kernels below take
+// one or two systems (the paired direct and adjoint solves), selected by
+// blockIdx.y (blockIdx.x for the single-workgroup kernels).
+struct SysArgs {
+  const double *rhs, *y;
+  double *x, *xs;            // xs = cf x, kept for the L X sweep (null for L*)
+  double *xd, *ed, *xnew;    // DIIS history and the Jacobi step
+  double *part, *part_e;     // Gram-row and increment-norm partial sums
+  int *newslot;
+  DiisState *st;
+};
+struct Systems {
+  SysArgs s[2];
+};
+
+__global__ __launch_bounds__(kJThreads) void k_jacobi(size_t nn, int nylm, int diis_max,
+                                                      const double *__restrict__ facl,
+                                                      Systems sys) {
+  const SysArgs &q0 = sys.s[blockIdx.y];
+  const DiisState *__restrict__ st = q0.st;
   if (st->done) return;
+  const double *__restrict__ rhs = q0.rhs;
+  const double *__restrict__ y = q0.y;
+  const double *__restrict__ x = q0.x;
+  double *__restrict__ xd = q0.xd;
+  double *__restrict__ ed = q0.ed;
+  double *__restrict__ xnew = q0.xnew;
+  double *__restrict__ part = q0.part;
   __shared__ double wpart[kJThreads / kWave][kPartStride];
   const int nmat = st->nmat;
   const int off = (diis_max > 0 && nmat >= diis_max) ? 10 : 0;
@@ -110,12 +132,16 @@ __device__ __forceinline__ double lane_value(double v, int src) {
 
 constexpr int kSmallThreads = 256;       // 4 workgroup partials per thread
 
-__global__ __launch_bounds__(kSmallThreads) void k_diis_small(
-    int nblk, int diis_max, const double *__restrict__ part, int nblk_e,
-    const double *__restrict__ part_e, int nsph, double tol, int maxit,
-    DiisState *__restrict__ st, int *__restrict__ newslot_out) {
+__global__ __launch_bounds__(kSmallThreads) void k_diis_small(int nblk, int diis_max, int nblk_e,
+                                                              int nsph, double tol, int maxit,
+                                                              Systems sys) {
 #pragma clang fp contract(off)
+  const SysArgs &q0 = sys.s[blockIdx.x];
+  DiisState *__restrict__ st = q0.st;
   if (st->done) return;
+  const double *__restrict__ part = q0.part;
+  const double *__restrict__ part_e = q0.part_e;
+  int *__restrict__ newslot_out = q0.newslot;
   __shared__ double red[kSmallThreads / kWave][kPartStride + 1];
   __shared__ int stop;
   const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
@@ -325,12 +351,20 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(
 
 // x_new = sum_i c_i x_i (or the plain Jacobi step), increment norm partials
 // per sphere (hsnorm, ddcosmo.f90:1233-1271), x <- x_new.  One wave per sphere.
-__global__ __launch_bounds__(kJThreads) void k_extrap(
-    int nsph, int nylm, size_t nn, int diis_max, const double *__restrict__ xd,
-    const double *__restrict__ xnew, const int *__restrict__ newslot,
-    const double *__restrict__ invl1, double *__restrict__ x, double *__restrict__ part,
-    const DiisState *__restrict__ st) {
+// ... and xs = cf x_new where the system keeps it.
+__global__ __launch_bounds__(kJThreads) void k_extrap(int nsph, int nylm, size_t nn, int diis_max,
+                                                      const double *__restrict__ invl1,
+                                                      const double *__restrict__ cf,
+                                                      Systems sys) {
+  const SysArgs &q0 = sys.s[blockIdx.y];
+  const DiisState *__restrict__ st = q0.st;
   if (st->done) return;
+  const double *__restrict__ xd = q0.xd;
+  const double *__restrict__ xnew = q0.xnew;
+  const int *__restrict__ newslot = q0.newslot;
+  double *__restrict__ x = q0.x;
+  double *__restrict__ xsc = q0.xs;
+  double *__restrict__ part = q0.part_e;
   __shared__ double lds[kJThreads / kWave];
   const int ncoef = diis_max > 0 ? st->ncoef : 0;
   const double *xs[kDiisMax];
@@ -359,6 +393,7 @@ __global__ __launch_bounds__(kJThreads) void k_extrap(
       }
       double inc = xn - x[k];
       x[k] = xn;
+      if (xsc) xsc[k] = cf[lm] * xn;
       s += invl1[lm] * inc * inc;
     }
     s = wave_sum(s);
@@ -374,9 +409,10 @@ __global__ __launch_bounds__(kJThreads) void k_extrap(
 // rms of the per-sphere norms and the convergence test (rmsvec,
 // jacobi_diis.f90:405-438; jacobi_diis.f90:153-156)
 __global__ __launch_bounds__(kWave) void k_norm(int nblk, int nsph, double tol, int maxit,
-                                                const double *__restrict__ part,
-                                                DiisState *__restrict__ st) {
+                                                Systems sys) {
+  DiisState *__restrict__ st = sys.s[blockIdx.x].st;
   if (st->done) return;
+  const double *__restrict__ part = sys.s[blockIdx.x].part_e;
   double s = 0.0;
   for (int b = threadIdx.x; b < nblk; b += kWave) s += part[b];
   s = wave_sum(s);
@@ -506,13 +542,37 @@ int energy_dot(ddx_handle *h, const double *a, const double *b, double scale, do
 }
 
 namespace {
-// kernel classes launched once per iteration of a solve
-void per_iter_classes(int star, int cls[5]) {
-  cls[0] = star ? DDX_K_SWEEP_LSTAR : DDX_K_SWEEP_L;
-  cls[1] = star ? DDX_K_EXPAND : DDX_K_PROJECT;
-  cls[2] = DDX_K_JACOBI;
-  cls[3] = DDX_K_DIIS;
-  cls[4] = DDX_K_EXTRAP;
+SysArgs sys_args(const ddx_handle *h, Lane &ln) {
+  SysArgs a;
+  a.rhs = ln.rhs;
+  a.y = ln.y.p;
+  a.x = ln.x;
+  a.xs = ln.star ? nullptr : ln.xs.p;
+  a.xd = ln.xd.p;
+  a.ed = ln.ed.p;
+  a.xnew = ln.xnew.p;
+  a.part = ln.part.p;
+  a.part_e = ln.part.p + (size_t)1024 * kPartStride;
+  a.newslot = ln.misc.p;
+  a.st = ln.diis.p;
+  (void)h;
+  return a;
+}
+
+// drops the timer records of one kernel class for rounds >= keep of a group
+void drop_rounds(ddx_handle *h, int cls, int serial, int keep) {
+  KernelTimer &t = h->ktimer[cls];
+  size_t n = 0;
+  for (size_t k = 0; k < t.recs.size(); ++k) {
+    const KernelTimer::Rec &r = t.recs[k];
+    if (r.solve == serial && r.round >= keep) {
+      t.pool.push_back(r.b);
+      t.pool.push_back(r.e);
+    } else {
+      t.recs[n++] = r;
+    }
+  }
+  t.recs.resize(n);
 }
 }  // namespace
 
@@ -529,6 +589,7 @@ int solve_begin(ddx_handle *h, Lane &ln, int star, int diis_max, double tol, con
   size_t nn = (size_t)h->nylm * h->n;
   DDX_TRY(ln.y.reserve(nn));
   DDX_TRY(ln.xnew.reserve(nn));
+  DDX_TRY(ln.xs.reserve(nn + 1));
   if (diis_max > 0) {
     DDX_TRY(ln.xd.reserve(nn * diis_max));
     DDX_TRY(ln.ed.reserve(nn * diis_max));
@@ -536,7 +597,6 @@ int solve_begin(ddx_handle *h, Lane &ln, int star, int diis_max, double tol, con
   DDX_TRY(ln.part.reserve((size_t)kMaxBlk * kPartStride + 2048));
   DDX_TRY(ln.diis.reserve(2));
   DDX_TRY(ln.misc.reserve(16));
-  ln.serial = h->solve_serial++;
   ln.star = star;
   ln.diis_max = diis_max;
   ln.max_iter = max_iter;
@@ -561,87 +621,115 @@ int solve_begin(ddx_handle *h, Lane &ln, int star, int diis_max, double tol, con
   for (int q = 0; q < kDiisMax; ++q) hs->slot[q] = q;
   hs->norm = 0.0;
   DDX_HIP(hipMemcpyAsync(ln.diis.p, hs, sizeof(DiisState), hipMemcpyHostToDevice, ln.stream));
+  // the L X sweep reads cf x: from here on k_extrap keeps it
+  if (!star) DDX_TRY(cf_scale(h, ln.stream, d_x, ln.xs.p));
   return DDX_OK;
 }
 
-int solve_enqueue(ddx_handle *h, Lane &ln) {
-  hipStream_t s = ln.stream;
+int solve_enqueue(ddx_handle *h, Lane *const *lanes, int count) {
+  Lane &l0 = *lanes[0];
+  const bool pair = count == 2;
+  if (pair && (l0.star != 0 || lanes[1]->star != 1 || lanes[1]->stream != l0.stream ||
+               lanes[1]->diis_max != l0.diis_max || lanes[1]->rounds != l0.rounds)) {
+    set_error("paired solves need a direct and an adjoint solve with equal settings");
+    return DDX_ERR_ARG;
+  }
+  hipStream_t s = l0.stream;
   size_t nn = (size_t)h->nylm * h->n;
   int nbj = grid_blocks(nn, kJThreads, kMaxBlk);
   int nbe = grid_blocks(h->n, kJThreads / kWave, 1024);
-  double *part_e = ln.part.p + (size_t)1024 * kPartStride;
-  int *newslot = ln.misc.p;
-  DiisState *ds = ln.diis.p;
   const double *invl1 = h->cf.p + h->nylm;   // 1/(l+1) table stored after cf
-  const int diis_max = ln.diis_max;
-  h->tag_solve = ln.serial;
-  for (int g = 0; g < ln.group && ln.enqueued < ln.rounds; ++g, ++ln.enqueued) {
-    h->tag_round = ln.enqueued;
-    if (ln.star) DDX_TRY(apply_lstarx(h, ln, ln.x, ln.y.p, ds));
-    else DDX_TRY(apply_lx(h, ln, ln.x, ln.y.p, ds));
+  const int diis_max = l0.diis_max;
+  Systems sys{};
+  for (int k = 0; k < count; ++k) sys.s[k] = sys_args(h, *lanes[k]);
+  h->tag_solve = l0.serial;
+  for (int g = 0; g < l0.group && l0.enqueued < l0.rounds; ++g, ++l0.enqueued) {
+    h->tag_round = l0.enqueued;
+    if (pair) {
+      DDX_TRY(apply_pair(h, l0, l0.y.p, *lanes[1], lanes[1]->x, lanes[1]->y.p));
+    } else if (l0.star) {
+      DDX_TRY(apply_lstarx(h, l0, l0.x, l0.y.p, l0.diis.p));
+    } else {
+      DDX_TRY(apply_lx(h, l0, l0.x, l0.y.p, l0.diis.p, true));
+    }
     {
       TimedLaunch tl(h, DDX_K_JACOBI, s);
-      hipLaunchKernelGGL(k_jacobi, dim3(nbj), dim3(kJThreads), 0, s, nn, h->nylm, diis_max,
-                         h->facl.p, ln.rhs, ln.y.p, ln.x, ln.xd.p, ln.ed.p, ln.xnew.p, ln.part.p,
-                         ds);
+      hipLaunchKernelGGL(k_jacobi, dim3(nbj, count), dim3(kJThreads), 0, s, nn, h->nylm, diis_max,
+                         h->facl.p, sys);
     }
     DDX_HIP(hipGetLastError());
     if (diis_max > 0) {
       TimedLaunch tl(h, DDX_K_DIIS, s);
-      hipLaunchKernelGGL(k_diis_small, dim3(1), dim3(kSmallThreads), 0, s, nbj, diis_max,
-                         ln.part.p, nbe, part_e, h->n, ln.tol, ln.max_iter, ds, newslot);
+      hipLaunchKernelGGL(k_diis_small, dim3(count), dim3(kSmallThreads), 0, s, nbj, diis_max, nbe,
+                         h->n, l0.tol, l0.max_iter, sys);
       DDX_HIP(hipGetLastError());
     }
     {
       TimedLaunch tl(h, DDX_K_EXTRAP, s);
-      hipLaunchKernelGGL(k_extrap, dim3(nbe), dim3(kJThreads), 0, s, h->n, h->nylm, nn, diis_max,
-                         ln.xd.p, ln.xnew.p, newslot, invl1, ln.x, part_e, ds);
+      hipLaunchKernelGGL(k_extrap, dim3(nbe, count), dim3(kJThreads), 0, s, h->n, h->nylm, nn,
+                         diis_max, invl1, h->cf.p, sys);
     }
     DDX_HIP(hipGetLastError());
     if (diis_max == 0) {
-      hipLaunchKernelGGL(k_norm, dim3(1), dim3(kWave), 0, s, nbe, h->n, ln.tol, ln.max_iter,
-                         part_e, ds);
+      hipLaunchKernelGGL(k_norm, dim3(count), dim3(kWave), 0, s, nbe, h->n, l0.tol, l0.max_iter,
+                         sys);
       DDX_HIP(hipGetLastError());
     }
   }
   h->tag_solve = -1;
   h->tag_round = 0;
-  ln.group = 2;
-  DDX_HIP(hipMemcpyAsync(ln.h_diis, ds, sizeof(DiisState), hipMemcpyDeviceToHost, s));
+  l0.group = 2;
+  for (int k = 0; k < count; ++k)
+    DDX_HIP(hipMemcpyAsync(lanes[k]->h_diis, lanes[k]->diis.p, sizeof(DiisState),
+                           hipMemcpyDeviceToHost, s));
   return DDX_OK;
 }
 
-int solve_poll(ddx_handle *h, Lane &ln, bool *finished) {
+int solve_poll(ddx_handle *h, Lane *const *lanes, int count, bool *finished) {
   (void)h;
-  DDX_HIP(hipStreamSynchronize(ln.stream));
-  *finished = ln.h_diis->done || ln.enqueued >= ln.rounds;
+  DDX_HIP(hipStreamSynchronize(lanes[0]->stream));
+  bool all = true;
+  for (int k = 0; k < count; ++k) all = all && lanes[k]->h_diis->done;
+  *finished = all || lanes[0]->enqueued >= lanes[0]->rounds;
   return DDX_OK;
 }
 
 int solve_end(ddx_handle *h, Lane &ln, int *n_iter, int *ok, std::vector<double> *hist) {
+  (void)h;
   const DiisState *hs = ln.h_diis;
   ln.active = false;
-  // launches of rounds after the last iteration returned at once: drop
-  // them from the timers
-  int cls[5];
-  per_iter_classes(ln.star, cls);
-  for (int c : cls) {
-    KernelTimer &t = h->ktimer[c];
-    size_t keep = 0;
-    for (size_t k = 0; k < t.recs.size(); ++k) {
-      const KernelTimer::Rec &r = t.recs[k];
-      if (r.solve == ln.serial && r.round >= hs->it) {
-        t.pool.push_back(r.b);
-        t.pool.push_back(r.e);
-      } else {
-        t.recs[keep++] = r;
-      }
-    }
-    t.recs.resize(keep);
-  }
   *n_iter = hs->ok ? hs->it : ln.max_iter + 1;
   *ok = hs->ok;
   if (hist) hist->assign(hs->hist, hs->hist + (hs->it < kMaxIter ? hs->it : kMaxIter));
+  return DDX_OK;
+}
+
+int solve_run(ddx_handle *h, Lane *const *lanes, int count) {
+  const int serial = h->solve_serial++;
+  for (int k = 0; k < count; ++k) lanes[k]->serial = serial;
+  bool finished = false;
+  while (!finished) {
+    DDX_TRY(solve_enqueue(h, lanes, count));
+    DDX_TRY(solve_poll(h, lanes, count, &finished));
+  }
+  // launches of rounds after a solve's last iteration returned at once:
+  // drop them from the timers
+  int it[2] = {0, 0}, itmax = 0;
+  for (int k = 0; k < count; ++k) {
+    it[k] = lanes[k]->h_diis->it;
+    if (it[k] > itmax) itmax = it[k];
+  }
+  const int shared[3] = {DDX_K_JACOBI, DDX_K_DIIS, DDX_K_EXTRAP};
+  for (int c : shared) drop_rounds(h, c, serial, itmax);
+  if (count == 2) {
+    drop_rounds(h, DDX_K_SWEEP_PAIR, serial, itmax);
+    drop_rounds(h, DDX_K_PROJECT, serial, it[0]);
+    drop_rounds(h, DDX_K_EXPAND, serial, it[1]);
+  } else {
+    const bool star = lanes[0]->star;
+    drop_rounds(h, star ? DDX_K_SWEEP_LSTAR : DDX_K_SWEEP_L, serial, it[0]);
+    drop_rounds(h, star ? DDX_K_EXPAND : DDX_K_PROJECT, serial, it[0]);
+  }
   return DDX_OK;
 }
 
@@ -649,11 +737,8 @@ int solve_jacobi_diis(ddx_handle *h, Lane &ln, int star, int diis_max, double to
                       const double *d_rhs, double *d_x, int max_iter, int *n_iter, int *ok,
                       std::vector<double> *hist) {
   DDX_TRY(solve_begin(h, ln, star, diis_max, tol, d_rhs, d_x, max_iter));
-  bool finished = false;
-  while (!finished) {
-    DDX_TRY(solve_enqueue(h, ln));
-    DDX_TRY(solve_poll(h, ln, &finished));
-  }
+  Lane *lp = &ln;
+  DDX_TRY(solve_run(h, &lp, 1));
   return solve_end(h, ln, n_iter, ok, hist);
 }
 

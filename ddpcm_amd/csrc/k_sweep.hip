@@ -10,14 +10,20 @@
 //     L* S:  xi(n,o) = Y(:,n) . S_o                           (matvec.f90:196-204)
 //            (L*S)_c = - sum_{t in c} w_n xi(n,o) W_t R(t)    (adjrhs, matvec.f90:213-223)
 //
-// Both sweeps walk the triples grouped by container c, so X_c / the output
-// vector of c is uniform across the workgroup: X_c is read through the
-// scalar cache, and the L* output is accumulated in registers per lane and
-// reduced across lanes once per container through an LDS transpose.
-// Y_lm follows ylmbas (ddcosmo.f90:830-895): Legendre recurrence of polleg
-// (m outer, l inner, ddcosmo.f90:1056-1094), Chebyshev cos/sin(m phi) of
-// trgev (ddcosmo.f90:1103-1125), normalisation facs; the per-(l,m) constant
-// cf = facs / facl folds facs with 4pi/(2l+1).
+// One sweep kernel serves both operators, and both at once: the triples are
+// grouped by container c, one wave per container.  The per-(l, m) constant
+// cf = facs * 4pi/(2l+1) is applied outside the triple loop: L X contracts
+// with X' = cf X (uniform across the wave, read through the scalar cache),
+// L* S accumulates sum_t coef_t p~ trig in registers per lane and multiplies
+// by cf after one LDS transpose reduction per container.  When the direct
+// and the adjoint solve run in lockstep a single pass over the triples
+// builds each basis once for both.
+// The basis follows ylmbas (ddcosmo.f90:830-895) with t^l folded into the
+// Legendre recurrence of polleg (m outer, l inner, ddcosmo.f90:1056-1094):
+//     p~(m, m)   = t^m P_m^m,   p~(m+1, m) = (2m+1) (t cos) p~(m, m)
+//     p~(l, m)   = [(2l-1) (t cos) p~(l-1, m) - (l+m-1) t^2 p~(l-2, m)] / (l-m)
+// and cos/sin(m phi) by the Chebyshev recurrence of trgev
+// (ddcosmo.f90:1103-1125).
 #include "ddx_device.hpp"
 #include "ddx_kernels.hpp"
 
@@ -26,114 +32,148 @@ namespace {
 
 __host__ __device__ constexpr int lmi(int l, int m) { return l * l + l + m; }
 
-// sum_lm R_lm(t) X_lm for one triple (X uniform across the wave).
-template <int LMAX>
-__device__ __forceinline__ double contract_R(double cthe, double sthe, double cphi, double sphi,
-                                             double t, const double *__restrict__ X,
-                                             const double *__restrict__ cf) {
-  double acc = 0.0;
-  double pmm = 1.0, tm = 1.0;
+// One triple: returns sum_lm p~_lm trig_m Xc'_lm when DIR and adds
+// coef p~_lm trig_m to acc_lm when ADJ (trig_m = cos(m phi) for m >= 0,
+// sin(|m| phi) for m < 0).
+template <int LMAX, bool DIR, bool ADJ>
+__device__ __forceinline__ double sweep_triple(double cthe, double sthe, double cphi, double sphi,
+                                               double t, const double *Xc,
+                                               double coef, double *acc) {
+  const double ct = cthe * t, st = sthe * t, t2 = t * t;
+  double s = 0.0;
+  double pmm = 1.0;   // t^m P_m^m
   double cm = 1.0, sm = 0.0, cprev = 0.0, sprev = 0.0;
 #pragma unroll
   for (int m = 0; m <= LMAX; ++m) {
-    double a, b = 0.0;
-    double p0 = pmm, p1 = 0.0, tl = tm;
-    {
-      double q = tl * p0;
-      a = q * (cf[lmi(m, m)] * X[lmi(m, m)]);
-      if (m > 0) b = q * (cf[lmi(m, m)] * X[lmi(m, -m)]);
+    double qc = 0.0, qs = 0.0;
+    if (ADJ) {
+      qc = (m == 0) ? coef : coef * cm;
+      qs = (m == 0) ? 0.0 : coef * sm;
     }
+    double a = 0.0, b = 0.0;
+    // X' of this m is read from LDS only after the previous m's sum: the
+    // compiler would otherwise keep all nylm values in registers
+    const double *Xm = Xc;
+    if (DIR) {
+      int off = 0;
+      asm volatile("" : "+v"(off) : "v"(s));
+      Xm = Xc + off;
+    }
+    auto visit = [&](int l, double p) {
+      if (DIR) {
+        a += p * Xm[lmi(l, m)];
+        if (m > 0) b += p * Xm[lmi(l, -m)];
+      }
+      if (ADJ) {
+        acc[lmi(l, m)] += p * qc;
+        if (m > 0) acc[lmi(l, -m)] += p * qs;
+      }
+    };
+    double p0 = pmm;
+    visit(m, p0);
     if (m < LMAX) {
-      p1 = cthe * (2.0 * m + 1.0) * p0;
-      tl *= t;
-      double q = tl * p1;
-      a += q * (cf[lmi(m + 1, m)] * X[lmi(m + 1, m)]);
-      if (m > 0) b += q * (cf[lmi(m + 1, m)] * X[lmi(m + 1, -m)]);
+      double p1 = ((double)(2 * m + 1) * ct) * p0;
+      visit(m + 1, p1);
 #pragma unroll
       for (int l = m + 2; l <= LMAX; ++l) {
-        double p2 = (cthe * (2.0 * l - 1.0) * p1 - (double)(l + m - 1) * p0) * (1.0 / (double)(l - m));
-        tl *= t;
-        q = tl * p2;
-        a += q * (cf[lmi(l, m)] * X[lmi(l, m)]);
-        if (m > 0) b += q * (cf[lmi(l, m)] * X[lmi(l, -m)]);
+        constexpr double one = 1.0;
+        const double ca = (double)(2 * l - 1) * (one / (double)(l - m));
+        const double cb = (double)(l + m - 1) * (one / (double)(l - m));
+        const double p2 = ca * (ct * p1) - cb * (t2 * p0);
+        visit(l, p2);
         p0 = p1;
         p1 = p2;
       }
     }
-    acc += (m == 0) ? a : (cm * a + sm * b);
-    pmm = -pmm * (2.0 * m + 1.0) * sthe;
-    tm *= t;
+    if (DIR) s += (m == 0) ? a : (cm * a + sm * b);
+    pmm = (-(double)(2 * m + 1) * st) * pmm;
     double cn, sn;
     if (m == 0) { cn = cphi; sn = sphi; }
     else { cn = 2.0 * cphi * cm - cprev; sn = 2.0 * cphi * sm - sprev; }
     cprev = cm; sprev = sm; cm = cn; sm = sn;
   }
-  return acc;
+  return s;
 }
 
-// acc_lm += coef * R_lm(t) for one triple (acc in registers).
-template <int LMAX>
-__device__ __forceinline__ void accum_R(double cthe, double sthe, double cphi, double sphi,
-                                        double t, double coef, const double *__restrict__ cf,
-                                        double *acc) {
-  double pmm = 1.0, tm = 1.0;
-  double cm = 1.0, sm = 0.0, cprev = 0.0, sprev = 0.0;
-#pragma unroll
-  for (int m = 0; m <= LMAX; ++m) {
-    double qc = coef * cm, qs = coef * sm;
-    double p0 = pmm, p1 = 0.0, tl = tm;
-    {
-      double z = tl * p0 * cf[lmi(m, m)];
-      if (m == 0) acc[lmi(m, 0)] += z * coef;
-      else { acc[lmi(m, m)] += z * qc; acc[lmi(m, -m)] += z * qs; }
-    }
-    if (m < LMAX) {
-      p1 = cthe * (2.0 * m + 1.0) * p0;
-      tl *= t;
-      double z = tl * p1 * cf[lmi(m + 1, m)];
-      if (m == 0) acc[lmi(m + 1, 0)] += z * coef;
-      else { acc[lmi(m + 1, m)] += z * qc; acc[lmi(m + 1, -m)] += z * qs; }
-#pragma unroll
-      for (int l = m + 2; l <= LMAX; ++l) {
-        double p2 = (cthe * (2.0 * l - 1.0) * p1 - (double)(l + m - 1) * p0) * (1.0 / (double)(l - m));
-        tl *= t;
-        z = tl * p2 * cf[lmi(l, m)];
-        if (m == 0) acc[lmi(l, 0)] += z * coef;
-        else { acc[lmi(l, m)] += z * qc; acc[lmi(l, -m)] += z * qs; }
-        p0 = p1;
-        p1 = p2;
-      }
-    }
-    pmm = -pmm * (2.0 * m + 1.0) * sthe;
-    tm *= t;
-    double cn, sn;
-    if (m == 0) { cn = cphi; sn = sphi; }
-    else { cn = 2.0 * cphi * cm - cprev; sn = 2.0 * cphi * sm - sprev; }
-    cprev = cm; sprev = sm; cm = cn; sm = sn;
-  }
-}
+constexpr int kRedChunk = 32;
 
-constexpr int kLThreads = 256;
-
-// W_t R(t) . X_c for every triple t of container c = blockIdx.x, stored in
-// the triple's slot of its point's list, so that each point's terms are
-// contiguous for the gather.
-template <int LMAX>
-__global__ __launch_bounds__(kLThreads) void k_sweep_contract(
-    const int *__restrict__ inl, const int *__restrict__ pairoff,
-    const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
-    const int *__restrict__ tslot, const double *__restrict__ X,
-    const double *__restrict__ cf, double *__restrict__ rw,
-    const DiisState *__restrict__ gate) {
-  if (gate && gate->done) return;
+// Triples beg..end-1 of container c: rw[tslot] = W (R . X_c) when DIR, and
+// when ADJ the chunk's share of (L*S)_c, -cf * sum_t w_n xi(n, o) W p~ trig,
+// into yp.
+template <int LMAX, bool DIR, bool ADJ>
+__device__ __forceinline__ void sweep_chunk(
+    int c, int beg, int end, const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
+    const int *__restrict__ tslot, const int *__restrict__ tp, const double *__restrict__ Xs,
+    double *__restrict__ rw, const double *__restrict__ xiw, const double *__restrict__ cf,
+    double *__restrict__ yp, double (*red)[kRedChunk + 1]) {
   constexpr int NY = (LMAX + 1) * (LMAX + 1);
-  const int c = blockIdx.x;
-  const int beg = pairoff[inl[c]], end = pairoff[inl[c + 1]];
-  const double *Xc = X + (size_t)c * NY;
-  for (int k = beg + threadIdx.x; k < end; k += kLThreads) {
-    double4 g = tgeo[k];
-    double2 tw = ttw[k];
-    rw[tslot[k]] = tw.y * contract_R<LMAX>(g.x, g.y, g.z, g.w, tw.x, Xc, cf);
+  const int lane = threadIdx.x;
+  // X'_c in LDS: every lane reads the same word (broadcast)
+  __shared__ double xsh[DIR ? NY : 1];
+  if (DIR) {
+    for (int k = lane; k < NY; k += kWave) xsh[k] = Xs[(size_t)c * NY + k];
+    __syncthreads();
+  }
+  double acc[ADJ ? NY : 1];
+#pragma unroll
+  for (int k = 0; k < (ADJ ? NY : 1); ++k) acc[k] = 0.0;
+  for (int k = beg + lane; k < end; k += kWave) {
+    const double4 g = tgeo[k];
+    const double2 tw = ttw[k];
+    const double coef = ADJ ? xiw[tp[k]] * tw.y : 0.0;
+    const double v = sweep_triple<LMAX, DIR, ADJ>(g.x, g.y, g.z, g.w, tw.x, xsh, coef, acc);
+    if (DIR) rw[tslot[k]] = tw.y * v;
+  }
+  if (ADJ) {
+    double *yc = yp;
+#pragma unroll
+    for (int c0 = 0; c0 < NY; c0 += kRedChunk) {
+#pragma unroll
+      for (int k = 0; k < kRedChunk; ++k)
+        if (c0 + k < NY) red[lane][k] = acc[c0 + k];
+      __syncthreads();
+      if (lane < kRedChunk && c0 + lane < NY) {
+        double r = 0.0;
+        for (int q = 0; q < kWave; ++q) r += red[q][lane];
+        yc[c0 + lane] = -(cf[c0 + lane] * r);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// One wave per chunk of triples (blockIdx.x).  DIR: L X terms (rw, slot
+// order); ADJ: the chunk's partial L* S (yp, per chunk).  A side whose solve
+// has ended (gate done) is skipped.
+template <int LMAX, bool DIR, bool ADJ>
+__global__ __launch_bounds__(kWave) void k_sweep(
+    const int2 *__restrict__ chunk, const int *__restrict__ inl, const int *__restrict__ pairoff,
+    const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
+    const int *__restrict__ tslot, const int *__restrict__ tp, const double *__restrict__ Xs,
+    double *__restrict__ rw, const double *__restrict__ xiw, const double *__restrict__ cf,
+    double *__restrict__ yp, const DiisState *__restrict__ gdir,
+    const DiisState *__restrict__ gadj) {
+  __shared__ double red[ADJ ? kWave : 1][kRedChunk + 1];
+  const bool dir = DIR && !(gdir && gdir->done);
+  const bool adj = ADJ && !(gadj && gadj->done);
+  const int2 ch = chunk[blockIdx.x];
+  const int c = ch.x, beg = ch.y;
+  const int end = min(beg + kTripleChunk, pairoff[inl[c + 1]]);
+  constexpr int NY = (LMAX + 1) * (LMAX + 1);
+  yp += (size_t)blockIdx.x * NY;
+  if constexpr (DIR && ADJ) {
+    if (dir && adj)
+      sweep_chunk<LMAX, true, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, yp, red);
+    else if (dir)
+      sweep_chunk<LMAX, true, false>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, yp, red);
+    else if (adj)
+      sweep_chunk<LMAX, false, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, yp, red);
+  } else if constexpr (DIR) {
+    if (dir)
+      sweep_chunk<LMAX, true, false>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, yp, red);
+  } else {
+    if (adj)
+      sweep_chunk<LMAX, false, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, yp, red);
   }
 }
 
@@ -154,94 +194,75 @@ __global__ __launch_bounds__(256) void k_gather_points(size_t np, const int *__r
   P[p] = s;
 }
 
-constexpr int kRedChunk = 32;
-
-// (L*S)_c for container c = blockIdx.x: one wave, lanes stride over the
-// container's triples, registers accumulate, one LDS transpose reduces.
-template <int LMAX>
-__global__ __launch_bounds__(kWave) void k_sweep_adjoint(
-    const int *__restrict__ inl, const int *__restrict__ pairoff,
-    const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
-    const int *__restrict__ tp, const double *__restrict__ xiw,
-    const double *__restrict__ cf, double *__restrict__ y,
-    const DiisState *__restrict__ gate) {
+// (L*S)_c = sum of its chunks' shares, in chunk order
+__global__ void k_chunk_sum(int n, int nylm, const int *__restrict__ choff,
+                            const double *__restrict__ yp, double *__restrict__ y,
+                            const DiisState *__restrict__ gate) {
   if (gate && gate->done) return;
-  constexpr int NY = (LMAX + 1) * (LMAX + 1);
-  __shared__ double red[kWave][kRedChunk + 1];
-  const int c = blockIdx.x, lane = threadIdx.x;
-  const int beg = pairoff[inl[c]], end = pairoff[inl[c + 1]];
-  double acc[NY];
-#pragma unroll
-  for (int k = 0; k < NY; ++k) acc[k] = 0.0;
-  for (int k = beg + lane; k < end; k += kWave) {
-    double4 g = tgeo[k];
-    double2 tw = ttw[k];
-    double coef = xiw[tp[k]] * tw.y;
-    accum_R<LMAX>(g.x, g.y, g.z, g.w, tw.x, coef, cf, acc);
-  }
-  double *yc = y + (size_t)c * NY;
-#pragma unroll
-  for (int c0 = 0; c0 < NY; c0 += kRedChunk) {
-#pragma unroll
-    for (int k = 0; k < kRedChunk; ++k)
-      if (c0 + k < NY) red[lane][k] = acc[c0 + k];
-    __syncthreads();
-    if (lane < kRedChunk && c0 + lane < NY) {
-      double s = 0.0;
-      for (int q = 0; q < kWave; ++q) s += red[q][lane];
-      yc[c0 + lane] = -s;
-    }
-    __syncthreads();
-  }
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)n * nylm) return;
+  const int c = (int)(k / nylm), lm = (int)(k % nylm);
+  double r = 0.0;
+  for (int j = choff[c]; j < choff[c + 1]; ++j) r += yp[(size_t)j * nylm + lm];
+  y[k] = r;
 }
 
-template <int LMAX>
-int launch_contract(ddx_handle *h, hipStream_t s, const double *X, double *r,
-                    const DiisState *gate) {
-  TimedLaunch tl(h, DDX_K_SWEEP_L, s);
-  hipLaunchKernelGGL(k_sweep_contract<LMAX>, dim3(h->n), dim3(kLThreads), 0, s,
-                     h->inl.p, h->pairoff.p, h->tgeo.p, h->ttw.p, h->tslot.p, X, h->cf.p, r, gate);
+// xs = cf * x, elementwise over (nylm, nsph)
+__global__ void k_cf_scale(size_t nn, int nylm, const double *__restrict__ cf,
+                           const double *__restrict__ x, double *__restrict__ xs) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nn) xs[k] = cf[k % nylm] * x[k];
+}
+
+template <int LMAX, bool DIR, bool ADJ>
+int launch_sweep(ddx_handle *h, hipStream_t s, const double *Xs, double *rw, const double *xiw,
+                 double *yp, const DiisState *gdir, const DiisState *gadj) {
+  if (h->nchunk == 0) return DDX_OK;
+  TimedLaunch tl(h, (DIR && ADJ) ? DDX_K_SWEEP_PAIR : (DIR ? DDX_K_SWEEP_L : DDX_K_SWEEP_LSTAR), s);
+  hipLaunchKernelGGL((k_sweep<LMAX, DIR, ADJ>), dim3(h->nchunk), dim3(kWave), 0, s, h->chunk.p,
+                     h->inl.p, h->pairoff.p, h->tgeo.p, h->ttw.p, h->tslot.p, h->tp.p, Xs, rw,
+                     xiw, h->cf.p, yp, gdir, gadj);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
 
 template <int LMAX>
-int launch_adjoint(ddx_handle *h, hipStream_t s, const double *xi, double *y,
-                   const DiisState *gate) {
-  TimedLaunch tl(h, DDX_K_SWEEP_LSTAR, s);
-  hipLaunchKernelGGL(k_sweep_adjoint<LMAX>, dim3(h->n), dim3(kWave), 0, s, h->inl.p,
-                     h->pairoff.p, h->tgeo.p, h->ttw.p, h->tp.p, xi, h->cf.p, y, gate);
-  DDX_HIP(hipGetLastError());
-  return DDX_OK;
+int sweep_lmax(ddx_handle *h, hipStream_t s, int mode, const double *Xs, double *rw,
+               const double *xiw, double *y, const DiisState *gdir, const DiisState *gadj) {
+  if (mode == 3) return launch_sweep<LMAX, true, true>(h, s, Xs, rw, xiw, y, gdir, gadj);
+  if (mode == 1) return launch_sweep<LMAX, true, false>(h, s, Xs, rw, xiw, y, gdir, gadj);
+  return launch_sweep<LMAX, false, true>(h, s, Xs, rw, xiw, y, gdir, gadj);
 }
 
-#define DDX_LMAX_CASES(F, ...)                                                        \
-  switch (h->lmax) {                                                                  \
-    case 0: return F<0>(__VA_ARGS__);  case 1: return F<1>(__VA_ARGS__);              \
-    case 2: return F<2>(__VA_ARGS__);  case 3: return F<3>(__VA_ARGS__);              \
-    case 4: return F<4>(__VA_ARGS__);  case 5: return F<5>(__VA_ARGS__);              \
-    case 6: return F<6>(__VA_ARGS__);  case 7: return F<7>(__VA_ARGS__);              \
-    case 8: return F<8>(__VA_ARGS__);  case 9: return F<9>(__VA_ARGS__);              \
-    case 10: return F<10>(__VA_ARGS__); case 11: return F<11>(__VA_ARGS__);           \
-    case 12: return F<12>(__VA_ARGS__); case 13: return F<13>(__VA_ARGS__);           \
-    case 14: return F<14>(__VA_ARGS__); case 15: return F<15>(__VA_ARGS__);           \
-    case 16: return F<16>(__VA_ARGS__);                                               \
-    default: set_error("lmax %d not supported (0..16)", h->lmax); return DDX_ERR_ARG; \
-  }
-
-}  // namespace
-
-int sweep_contract(ddx_handle *h, hipStream_t s, const double *X, double *r,
-                   const DiisState *gate) {
+// mode: 1 = L X terms, 2 = L* S shares per chunk, 3 = both
+int sweep(ddx_handle *h, hipStream_t s, int mode, const double *Xs, double *rw,
+          const double *xiw, double *y, const DiisState *gdir, const DiisState *gadj) {
   if (h->n == 0) return DDX_OK;
-  DDX_LMAX_CASES(launch_contract, h, s, X, r, gate);
+  switch (h->lmax) {
+    case 0: return sweep_lmax<0>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 1: return sweep_lmax<1>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 2: return sweep_lmax<2>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 3: return sweep_lmax<3>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 4: return sweep_lmax<4>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 5: return sweep_lmax<5>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 6: return sweep_lmax<6>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 7: return sweep_lmax<7>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 8: return sweep_lmax<8>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 9: return sweep_lmax<9>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 10: return sweep_lmax<10>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 11: return sweep_lmax<11>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 12: return sweep_lmax<12>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 13: return sweep_lmax<13>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 14: return sweep_lmax<14>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 15: return sweep_lmax<15>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 16: return sweep_lmax<16>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    default: set_error("lmax %d not supported (0..16)", h->lmax); return DDX_ERR_ARG;
+  }
 }
 
-int apply_lx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate) {
+// P = gathered L X terms of each point; y = - Yw^T-projection of P
+int finish_lx(ddx_handle *h, Lane &ln, double *y, const DiisState *gate) {
   hipStream_t s = ln.stream;
-  DDX_TRY(ln.rw.reserve(h->ntri + 1));
-  DDX_TRY(ln.P.reserve(h->np));
-  DDX_TRY(sweep_contract(h, s, x, ln.rw.p, gate));
   hipLaunchKernelGGL(k_gather_points, dim3((h->np + 255) / 256), dim3(256), 0, s, h->np,
                      h->ptoff.p, ln.rw.p, h->ui.p, ln.P.p, gate);
   DDX_HIP(hipGetLastError());
@@ -249,13 +270,59 @@ int apply_lx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisStat
   return project(h, s, -1.0, ln.P.p, y, gate);
 }
 
+}  // namespace
+
+int cf_scale(ddx_handle *h, hipStream_t s, const double *x, double *xs) {
+  size_t nn = (size_t)h->nylm * h->n;
+  if (nn == 0) return DDX_OK;
+  hipLaunchKernelGGL(k_cf_scale, dim3((nn + 255) / 256), dim3(256), 0, s, nn, h->nylm, h->cf.p, x,
+                     xs);
+  DDX_HIP(hipGetLastError());
+  return DDX_OK;
+}
+
+int apply_lx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate,
+             bool xs_ready) {
+  hipStream_t s = ln.stream;
+  size_t nn = (size_t)h->nylm * h->n;
+  DDX_TRY(ln.rw.reserve(h->ntri + 1));
+  DDX_TRY(ln.P.reserve(h->np));
+  DDX_TRY(ln.xs.reserve(nn + 1));
+  if (!xs_ready) DDX_TRY(cf_scale(h, s, x, ln.xs.p));
+  DDX_TRY(sweep(h, s, 1, ln.xs.p, ln.rw.p, nullptr, nullptr, gate, nullptr));
+  return finish_lx(h, ln, y, gate);
+}
+
+int chunk_sum(ddx_handle *h, hipStream_t s, const double *yp, double *y, const DiisState *gate) {
+  size_t nn = (size_t)h->nylm * h->n;
+  if (nn == 0) return DDX_OK;
+  hipLaunchKernelGGL(k_chunk_sum, dim3((nn + 255) / 256), dim3(256), 0, s, h->n, h->nylm,
+                     h->choff.p, yp, y, gate);
+  DDX_HIP(hipGetLastError());
+  return DDX_OK;
+}
+
 int apply_lstarx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate) {
   hipStream_t s = ln.stream;
   DDX_TRY(ln.xiw.reserve(h->np));
+  DDX_TRY(ln.yp.reserve((size_t)h->nchunk * h->nylm + 1));
   // w_n xi(n, o) = w_n sum_lm Y(lm, n) x(lm, o)
   DDX_TRY(expand(h, s, x, ln.xiw.p, true, gate));
-  if (h->n == 0) return DDX_OK;
-  DDX_LMAX_CASES(launch_adjoint, h, s, ln.xiw.p, y, gate);
+  DDX_TRY(sweep(h, s, 2, nullptr, nullptr, ln.xiw.p, ln.yp.p, nullptr, gate));
+  return chunk_sum(h, s, ln.yp.p, y, gate);
+}
+
+int apply_pair(ddx_handle *h, Lane &ld, double *yd, Lane &la, const double *xa, double *ya) {
+  hipStream_t s = ld.stream;
+  DDX_TRY(ld.rw.reserve(h->ntri + 1));
+  DDX_TRY(ld.P.reserve(h->np));
+  DDX_TRY(la.xiw.reserve(h->np));
+  DDX_TRY(la.yp.reserve((size_t)h->nchunk * h->nylm + 1));
+  const DiisState *gd = ld.diis.p, *ga = la.diis.p;
+  DDX_TRY(expand(h, s, xa, la.xiw.p, true, ga));
+  DDX_TRY(sweep(h, s, 3, ld.xs.p, ld.rw.p, la.xiw.p, la.yp.p, gd, ga));
+  DDX_TRY(chunk_sum(h, s, la.yp.p, ya, ga));
+  return finish_lx(h, ld, yd, gd);
 }
 
 }  // namespace ddx

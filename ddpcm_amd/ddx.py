@@ -29,7 +29,7 @@ EXPORTS = (
     "ddx_set_precision", "ddx_get_stats", "ddx_kernel_timer", "ddx_kernel_timer_read",
 )
 KERNELS = ("sweep_l", "sweep_lstar", "project", "expand", "jacobi", "diis", "extrap", "field",
-           "potential", "forces")
+           "potential", "forces", "sweep_pair")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

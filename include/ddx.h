@@ -42,7 +42,9 @@ typedef struct ddx_stats {
   int nnl;                  /* neighbour pairs                                   */
   int ncav;                 /* exposed cavity points                             */
   int max_degree;           /* largest neighbour count                           */
-  double ms_setup, ms_rhs, ms_direct, ms_adjoint, ms_forces; /* last run, HIP events */
+  /* last run, HIP events; where the direct and adjoint solves run in
+   * lockstep (energy + forces) ms_direct covers both and ms_adjoint is 0 */
+  double ms_setup, ms_rhs, ms_direct, ms_adjoint, ms_forces;
 } ddx_stats;
 
 const char *ddx_last_error(void);
@@ -141,7 +143,8 @@ int ddx_get_stats(const ddx_handle *h, ddx_stats *st);
 #define DDX_K_FIELD 7        /* all-pairs field                          */
 #define DDX_K_POTENTIAL 8    /* all-pairs potential                      */
 #define DDX_K_FORCES 9       /* per-triple forces terms                  */
-#define DDX_K_COUNT 10
+#define DDX_K_SWEEP_PAIR 10  /* L X and L* S in one sweep (paired solves) */
+#define DDX_K_COUNT 11
 int ddx_kernel_timer(ddx_handle *h, int kernel, int enable);
 int ddx_kernel_timer_read(ddx_handle *h, int kernel, int *launches, double *total_ms);
 
