@@ -247,6 +247,15 @@ int ddx_init(ddx_handle **out, int n, const double *x, const double *y, const do
     return fail(DDX_ERR_HIP);
   }
   for (Lane &ln : h->lane) ln.stream = h->stream;
+  {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        h->ncu < 1) {
+      set_error("cannot query the compute units of the device");
+      return fail(DDX_ERR_HIP);
+    }
+  }
   for (auto &e : h->ev)
     if (hipEventCreate(&e) != hipSuccess) {
       set_error("hipEventCreate failed");
@@ -467,7 +476,7 @@ static int op_apply(ddx_handle *h, int which, const double *x, double *y) {
   DDX_TRY(reserve_vectors(h));
   DDX_TRY(h->y.reserve(nn));
   DDX_TRY(upload(h->x, x, nn, s));
-  if (which == 0) DDX_TRY(apply_lx(h, h->lane[0], h->x.p, h->y.p, nullptr, false));
+  if (which == 0) DDX_TRY(apply_lx(h, h->lane[0], h->x.p, h->y.p, nullptr, false, 1));
   else if (which == 1) DDX_TRY(apply_lstarx(h, h->lane[0], h->x.p, h->y.p, nullptr));
   else DDX_TRY(device_ldm1x(h, s, h->x.p, h->y.p));
   DDX_TRY(download(y, h->y.p, nn, s));

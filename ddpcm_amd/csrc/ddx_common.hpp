@@ -16,7 +16,7 @@ constexpr int kMaxLmax = 16;       // template instantiations of the sweeps
 constexpr int kMaxDegree = 64;     // neighbours per sphere (64-bit point masks)
 constexpr int kDiisMax = 25;       // ndiis of the reference (ddcosmo.f90:87)
 constexpr int kMaxIter = 200;      // n_iter of cosmo (cosmo.f90:81)
-constexpr int kTripleChunk = 512;  // triples per sweep work unit (8 per lane)
+constexpr int kSweepWorkersPerCU = 8;  // one-wave sweep workers per compute unit
 
 void set_error(const char *fmt, ...);
 
@@ -155,9 +155,10 @@ struct ddx_handle {
 
   // triples grouped by the sphere that contains the point
   ddx::DevArray<int> paircnt, pairoff;
-  ddx::DevArray<int> chcnt, choff;  // sweep chunks per container and their offsets
-  ddx::DevArray<int2> chunk;        // (container, first triple) per chunk
-  int nchunk = 0;
+  // sweep work: units of 64 triples per container, segments per container
+  ddx::DevArray<int> ucnt, uoff, segcnt, segoff;
+  int nunit = 0, nworker = 0, nseg_cap = 0;
+  int ncu = 0;                      // compute units of the device
   ddx::DevArray<double4> tgeo;    // (cos theta, sin theta, cos phi, sin phi)
   ddx::DevArray<double2> ttw;     // (t, W)
   ddx::DevArray<int> tp;          // owner point index i*ngrid + n

@@ -60,17 +60,60 @@ __device__ __forceinline__ bool spheres_overlap(const double4 &a, const double4 
   return d2 <= r2;
 }
 
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+// Value of this lane's partner at stage S of a six-stage cross-lane
+// combine, without LDS: the lane 32 apart and the lane 16 apart (gfx950
+// row swaps), then the mirror lane within 16 and within 8 and the quad
+// swaps by 2 and by 1 (DPP).  Every stage pairs each lane with a distinct
+// partner, so combining over the six stages covers all 64 lanes.
+template <int S>
+__device__ __forceinline__ unsigned partner_u32(unsigned v) {
+  if constexpr (S == 0) {
+    auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (lane_id() < 32) ? r[1] : r[0];
+  } else if constexpr (S == 1) {
+    auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (lane_id() & 16) ? r[0] : r[1];
+  } else {
+    constexpr int ctrl = S == 2 ? 0x140 : S == 3 ? 0x141 : S == 4 ? 0x4E : 0xB1;
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, ctrl, 0xf, 0xf, false);
+  }
+}
+
+template <int S>
+__device__ __forceinline__ double partner(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = partner_u32<S>((unsigned)b), hi = partner_u32<S>((unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+template <int S>
+__device__ __forceinline__ int partner(int v) {
+  return (int)partner_u32<S>((unsigned)v);
+}
+
+// Wave-wide sum; every lane receives the same bits (each stage adds two
+// values that the pair of lanes holds alike, and addition commutes).
 __device__ __forceinline__ double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  v += partner<0>(v);
+  v += partner<1>(v);
+  v += partner<2>(v);
+  v += partner<3>(v);
+  v += partner<4>(v);
+  v += partner<5>(v);
   return v;
 }
 
 __device__ __forceinline__ int wave_sum_int(int v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  v += partner<0>(v);
+  v += partner<1>(v);
+  v += partner<2>(v);
+  v += partner<3>(v);
+  v += partner<4>(v);
+  v += partner<5>(v);
   return v;
 }
-
-__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
 // number of set ballot bits below this lane
 __device__ __forceinline__ int ballot_prefix(unsigned long long b) {

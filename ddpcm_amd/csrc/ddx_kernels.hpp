@@ -13,9 +13,11 @@ int scan_exclusive(const int *in, int *out, size_t n, DevArray<int> &tmp,
 int run_setup(ddx_handle *h);
 
 // k_gemm.hip — FP64 MFMA products with the harmonic tables
-// y(lm, i) = alpha sum_n Yw(lm, n) P(n, i)
+// y(lm, i) = alpha sum_n Yw(lm, n) P(n, i), as ysplit partial sums over
+// disjoint point sets at y + j nylm nsph (j < ysplit)
 int project(ddx_handle *h, hipStream_t s, double alpha, const double *P, double *y,
-            const DiisState *gate);
+            const DiisState *gate, int ysplit);
+constexpr int kProjSplit = 4;   // partial projections of L X inside the solver
 // xi(n, i) = [w_n] sum_lm Y(lm, n) S(lm, i)
 int expand(ddx_handle *h, hipStream_t s, const double *S, double *xi, bool wscale,
            const DiisState *gate);
@@ -24,11 +26,12 @@ int expand(ddx_handle *h, hipStream_t s, const double *S, double *xi, bool wscal
 // and stream of a lane.  L X reads X' = cf X from ln.xs: apply_lx makes it
 // unless xs_ready says that ln.xs already holds cf x.
 int apply_lx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate,
-             bool xs_ready);
+             bool xs_ready, int ysplit);
 int apply_lstarx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate);
 // yd = L xd (ld.xs holds cf xd) and ya = L* xa in one pass over the
 // triples, gated by the DIIS states of the two lanes
-int apply_pair(ddx_handle *h, Lane &ld, double *yd, Lane &la, const double *xa, double *ya);
+int apply_pair(ddx_handle *h, Lane &ld, double *yd, Lane &la, const double *xa, double *ya,
+               int ysplit);
 // xs = cf x
 int cf_scale(ddx_handle *h, hipStream_t s, const double *x, double *xs);
 

@@ -36,6 +36,10 @@ __global__ __launch_bounds__(256) void k_project(int nylm, int ngrid, int nsph, 
   if (gate && gate->done) return;
   __shared__ v4d red[3][kMaxTiles][kWave];
   const int lane = lane_id(), wave = threadIdx.x / kWave;
+  // blockIdx.y splits the points: wave w of split ks takes k-steps
+  // 4 ks + w, + 4 gridDim.y, ... and the split's sum goes to y + ks nylm nsph
+  const int nw = 4 * gridDim.y, gw = 4 * blockIdx.y + wave;
+  y += (size_t)blockIdx.y * nylm * nsph;
   const int i0 = blockIdx.x * 16;
   const int col = lane & 15, krow = lane >> 4;
   const int sph = i0 + col;
@@ -46,13 +50,13 @@ __global__ __launch_bounds__(256) void k_project(int nylm, int ngrid, int nsph, 
     v4d acc[kMaxTiles];
 #pragma unroll
     for (int t = 0; t < kMaxTiles; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
-    // wave w takes k-steps w, w+4, w+8, ... (4 points each)
-    for (int kb = 4 * wave; kb < ngrid; kb += 16 * kUnroll) {
+    // k-steps of 4 points each
+    for (int kb = 4 * gw; kb < ngrid; kb += 4 * nw * kUnroll) {
       double b[kUnroll];
       double a[kUnroll][kMaxTiles];
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
-        const int n = kb + 16 * u + krow;
+        const int n = kb + 4 * nw * u + krow;
         const bool okn = n < ngrid;
         b[u] = (okn && sph < nsph) ? Pc[n] : 0.0;
 #pragma unroll
@@ -131,10 +135,10 @@ __global__ __launch_bounds__(256) void k_expand(int nylm, int ngrid, int nsph,
 }  // namespace
 
 int project(ddx_handle *h, hipStream_t s, double alpha, const double *P, double *y,
-            const DiisState *gate) {
+            const DiisState *gate, int ysplit) {
   if (h->n <= 0) return DDX_OK;
   TimedLaunch tl(h, DDX_K_PROJECT, s);
-  hipLaunchKernelGGL(k_project, dim3((h->n + 15) / 16), dim3(256), 0, s, h->nylm,
+  hipLaunchKernelGGL(k_project, dim3((h->n + 15) / 16, ysplit), dim3(256), 0, s, h->nylm,
                      h->ngrid, h->n, alpha, h->Yw.p, P, y, gate);
   DDX_HIP(hipGetLastError());
   return DDX_OK;

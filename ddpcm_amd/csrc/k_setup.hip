@@ -210,23 +210,32 @@ __global__ __launch_bounds__(kSetupThreads) void k_pair_fill(
   }
 }
 
-// Chunks of at most kTripleChunk triples of one container, the work unit of
-// the sweeps: number of chunks per container ...
-__global__ void k_chunk_count(int n, const int *__restrict__ inl, const int *__restrict__ pairoff,
-                              int *__restrict__ cnt) {
+// Work units of the sweeps: runs of kWave triples of one container.
+// A launch of the sweep hands contiguous ranges of units to a fixed number
+// of workers; a segment is the part of one worker's range inside one
+// container, and each segment leaves one partial L* S vector.
+__global__ void k_unit_count(int n, const int *__restrict__ inl, const int *__restrict__ pairoff,
+                             int *__restrict__ ucnt) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   int t = pairoff[inl[c + 1]] - pairoff[inl[c]];
-  cnt[c] = (t + kTripleChunk - 1) / kTripleChunk;
+  ucnt[c] = (t + kWave - 1) / kWave;
 }
 
-// ... and (container, first triple) of each chunk
-__global__ void k_chunk_fill(int n, const int *__restrict__ inl, const int *__restrict__ pairoff,
-                             const int *__restrict__ choff, int2 *__restrict__ chunk) {
+// segments per container: its first unit starts one, and so does every
+// worker whose range begins inside the container (worker w starts at unit
+// floor(w U / G))
+__global__ void k_seg_count(int n, const int *__restrict__ uoff, long long U, long long G,
+                            int *__restrict__ segcnt) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  int beg = pairoff[inl[c]];
-  for (int j = choff[c]; j < choff[c + 1]; ++j, beg += kTripleChunk) chunk[j] = make_int2(c, beg);
+  const long long a = uoff[c], b = uoff[c + 1];
+  int cnt = 0;
+  if (b > a) {
+    const long long hi = (b * G + U - 1) / U, lo = ((a + 1) * G + U - 1) / U;
+    cnt = 1 + (int)(hi > lo ? hi - lo : 0);
+  }
+  segcnt[c] = cnt;
 }
 
 }  // namespace
@@ -300,22 +309,30 @@ int run_setup(ddx_handle *h) {
   } else {
     DDX_HIP(hipMemsetAsync(h->pairoff.p, 0, sizeof(int), s));
   }
-  DDX_TRY(h->chcnt.reserve(n + 1));
-  DDX_TRY(h->choff.reserve(n + 1));
-  hipLaunchKernelGGL(k_chunk_count, dim3((n + 255) / 256), dim3(256), 0, s, n, h->inl.p,
-                     h->pairoff.p, h->chcnt.p);
+  DDX_TRY(h->ucnt.reserve(n + 1));
+  DDX_TRY(h->uoff.reserve(n + 1));
+  hipLaunchKernelGGL(k_unit_count, dim3((n + 255) / 256), dim3(256), 0, s, n, h->inl.p,
+                     h->pairoff.p, h->ucnt.p);
   DDX_HIP(hipGetLastError());
-  DDX_TRY(scan_exclusive(h->chcnt.p, h->choff.p, n, h->scan_tmp, s));
-  int nchunk = 0;
-  DDX_HIP(hipMemcpyAsync(&nchunk, h->choff.p + n, sizeof(int), hipMemcpyDeviceToHost, s));
+  DDX_TRY(scan_exclusive(h->ucnt.p, h->uoff.p, n, h->scan_tmp, s));
+  int nunit = 0;
+  DDX_HIP(hipMemcpyAsync(&nunit, h->uoff.p + n, sizeof(int), hipMemcpyDeviceToHost, s));
   DDX_HIP(hipStreamSynchronize(s));
   h->ntri = ntri;
   h->ncav = ncav;
-  h->nchunk = nchunk;
-  DDX_TRY(h->chunk.reserve(nchunk + 1));
-  hipLaunchKernelGGL(k_chunk_fill, dim3((n + 255) / 256), dim3(256), 0, s, n, h->inl.p,
-                     h->pairoff.p, h->choff.p, h->chunk.p);
+  h->nunit = nunit;
+  // workers: enough one-wave workgroups to fill every SIMD twice
+  long long G = (long long)kSweepWorkersPerCU * h->ncu;
+  if (G > nunit) G = nunit;
+  h->nworker = (int)G;
+  DDX_TRY(h->segcnt.reserve(n + 1));
+  DDX_TRY(h->segoff.reserve(n + 1));
+  // (a container with units implies U > 0 and G > 0)
+  hipLaunchKernelGGL(k_seg_count, dim3((n + 255) / 256), dim3(256), 0, s, n, h->uoff.p,
+                     (long long)nunit, G, h->segcnt.p);
   DDX_HIP(hipGetLastError());
+  DDX_TRY(scan_exclusive(h->segcnt.p, h->segoff.p, n, h->scan_tmp, s));
+  h->nseg_cap = (int)G + n;   // segments: at most one per container plus one per worker
 
   DDX_TRY(h->tgeo.reserve(ntri + 1));
   DDX_TRY(h->ttw.reserve(ntri + 1));

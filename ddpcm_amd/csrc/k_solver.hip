@@ -27,7 +27,8 @@ constexpr int kMaxBlk = 1024;               // workgroups of k_jacobi (partials 
 // one or two systems (the paired direct and adjoint solves), selected by
 // blockIdx.y (blockIdx.x for the single-workgroup kernels).
 struct SysArgs {
-  const double *rhs, *y;
+  const double *rhs, *y;     // y: ny partial arrays of nylm nsph, summed here
+  int ny;
   double *x, *xs;            // xs = cf x, kept for the L X sweep (null for L*)
   double *xd, *ed, *xnew;    // DIIS history and the Jacobi step
   double *part, *part_e;     // Gram-row and increment-norm partial sums
@@ -46,6 +47,7 @@ __global__ __launch_bounds__(kJThreads) void k_jacobi(size_t nn, int nylm, int d
   if (st->done) return;
   const double *__restrict__ rhs = q0.rhs;
   const double *__restrict__ y = q0.y;
+  const int ny = q0.ny;
   const double *__restrict__ x = q0.x;
   double *__restrict__ xd = q0.xd;
   double *__restrict__ ed = q0.ed;
@@ -67,7 +69,9 @@ __global__ __launch_bounds__(kJThreads) void k_jacobi(size_t nn, int nylm, int d
   double self = 0.0;
   for (size_t k = (size_t)blockIdx.x * kJThreads + threadIdx.x; k < nn;
        k += (size_t)gridDim.x * kJThreads) {
-    double xn = facl[k % nylm] * (rhs[k] - y[k]);
+    double yk = y[k];
+    for (int j = 1; j < ny; ++j) yk += y[(size_t)j * nn + k];
+    double xn = facl[k % nylm] * (rhs[k] - yk);
     double e = xn - x[k];
     xo[k] = xn;
     if (eo) {
@@ -112,15 +116,6 @@ __global__ __launch_bounds__(kJThreads) void k_jacobi(size_t nn, int nylm, int d
 //                  unscrambling does not touch the solution and is skipped.
 constexpr int kB = kDiisMax;      // register columns per row (n1 <= 25)
 
-// a[c] for a wave-uniform c, written as an exact sum (one term is non-zero)
-// so that the compiler keeps a[] in registers instead of indexing memory
-__device__ __forceinline__ double sel(const double (&a)[kB], int c) {
-  double v = 0.0;
-#pragma unroll
-  for (int k = 0; k < kB; ++k) v += (c == k) ? a[k] : 0.0;
-  return v;
-}
-
 // value of v in lane `src`; src must be wave-uniform (it is moved to an SGPR)
 __device__ __forceinline__ double lane_value(double v, int src) {
   const int s = __builtin_amdgcn_readfirstlane(src);
@@ -128,6 +123,17 @@ __device__ __forceinline__ double lane_value(double v, int src) {
   const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffll), s);
   const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), s);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// one lane-pair stage of the pivot search: keep the larger value, the
+// smaller row-major index on ties
+template <int S>
+__device__ __forceinline__ void pivot_stage(double &big, int &best) {
+  const double pb = partner<S>(big);
+  const int pi = partner<S>(best);
+  const bool t = (pb > big) || (pb == big && pi < best);
+  big = t ? pb : big;
+  best = t ? pi : best;
 }
 
 constexpr int kSmallThreads = 256;       // 4 workgroup partials per thread
@@ -142,40 +148,24 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(int nblk, int diis
   const double *__restrict__ part = q0.part;
   const double *__restrict__ part_e = q0.part_e;
   int *__restrict__ newslot_out = q0.newslot;
-  __shared__ double red[kSmallThreads / kWave][kPartStride + 1];
+  __shared__ double red[kPartStride + 1];
   __shared__ int stop;
   const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
   int nmat = st->nmat;
   const int off = (nmat >= diis_max) ? 10 : 0;
   const int ndot = nmat - off - 1;
   const bool pending = st->pending_norm != 0;
-  // Gram-row dot products (each thread sums 4 workgroup partials) and the
-  // increment norm of the previous iteration, as straight-line butterflies
-  constexpr int kPer = kMaxBlk / kSmallThreads;
-  double v[kPartStride + 1];
-#pragma unroll
-  for (int q = 0; q < kPartStride; ++q) {
-    const bool need = (q < ndot || q == kDiisMax);
+  // Gram-row dot products and the increment norm of the previous iteration:
+  // wave w sums the workgroup partials of values w, w + 4, ... (26 = norm)
+  for (int q = wave; q <= kPartStride; q += kSmallThreads / kWave) {
+    const bool need = (q < ndot || q == kDiisMax) || (q == kPartStride && pending);
+    if (!need) continue;   // wave-uniform
+    const double *src = (q == kPartStride) ? part_e : part + (size_t)q * kMaxBlk;
+    const int nb = (q == kPartStride) ? nblk_e : nblk;
     double sum = 0.0;
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const int b = tid + kSmallThreads * k;
-      sum += (need && b < nblk) ? part[(size_t)q * kMaxBlk + b] : 0.0;
-    }
-    v[q] = sum;
-  }
-  {
-    double sum = 0.0;
-    for (int b = tid; b < nblk_e; b += kSmallThreads) sum += pending ? part_e[b] : 0.0;
-    v[kPartStride] = sum;
-  }
-#pragma unroll
-  for (int off2 = 32; off2 > 0; off2 >>= 1)
-#pragma unroll
-    for (int q = 0; q <= kPartStride; ++q) v[q] += __shfl_xor(v[q], off2, kWave);
-  if (lane == 0) {
-#pragma unroll
-    for (int q = 0; q <= kPartStride; ++q) red[wave][q] = v[q];
+    for (int b = lane; b < nb; b += kWave) sum += src[b];
+    sum = wave_sum(sum);
+    if (lane == 0) red[q] = sum;
   }
   __syncthreads();
   if (tid == 0) {
@@ -183,9 +173,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(int nblk, int diis
     if (pending) {
       // rms of the per-sphere norms and the convergence test of the previous
       // iteration (rmsvec, jacobi_diis.f90:405-438; jacobi_diis.f90:153-156)
-      double ss = 0.0;
-      for (int w = 0; w < kSmallThreads / kWave; ++w) ss += red[w][kPartStride];
-      const double norm = sqrt(ss / (double)nsph);
+      const double norm = sqrt(red[kPartStride] / (double)nsph);
       const int it = st->it;
       if (it <= kMaxIter) st->hist[it] = norm;
       st->it = it + 1;
@@ -200,18 +188,17 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(int nblk, int diis
   }
   __syncthreads();
   if (stop) return;
-  if (wave != 0) return;
+  __shared__ double bsh[kB][kB + 1];   // B after this iteration's update
+  __shared__ int nmat_sh;
+  if (wave == 0) {
   double a[kB];
 #pragma unroll
   for (int c = 0; c < kB; ++c) a[c] = (lane < kB) ? st->b[lane * kB + c] : 0.0;
   int myslot = (lane < diis_max) ? st->slot[lane] : 0;
   // lane i + 1 keeps dot i; every lane keeps the self dot
   const int qi = (lane >= 1 && lane <= kDiisMax) ? lane - 1 : 0;
-  double mydot = 0.0, self = 0.0;
-  for (int w = 0; w < kSmallThreads / kWave; ++w) {
-    mydot += red[w][qi];
-    self += red[w][kDiisMax];
-  }
+  double mydot = red[qi];
+  const double self = red[kDiisMax];
   if (lane == 0) st->pending_norm = 1;   // k_extrap leaves this iteration's norm
   if (!(lane >= 1 && lane - 1 < ndot)) mydot = 0.0;
   if (lane == 0) *newslot_out = st->slot[nmat - 1];
@@ -258,42 +245,75 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(int nblk, int diis
   // persist B and the slot map
   if (lane < kB) {
 #pragma unroll
-    for (int c = 0; c < kB; ++c) st->b[lane * kB + c] = a[c];
+    for (int c = 0; c < kB; ++c) {
+      st->b[lane * kB + c] = a[c];
+      bsh[lane][c] = a[c];
+    }
   }
   if (lane < diis_max) st->slot[lane] = myslot;
+  if (lane == 0) nmat_sh = nmat;
+  }   // wave 0
+  __syncthreads();
+  nmat = nmat_sh;
 
-  // Gauss-Jordan on a copy, n1 = nmat + 1.  Lane r holds the row that started
-  // as row r and `pos` is its current row index: the reference's row
-  // exchange becomes an exchange of two pos values, the arithmetic on every
-  // element stays the reference's.  Columns already pivoted hold the
-  // in-place inverse, which neither the later pivots nor the solution read,
-  // so only live columns are updated.
+  // Gauss-Jordan with full pivoting on B, n1 = nmat + 1, by all four waves:
+  // lane r holds row r (its current row index is `pos`: the reference's row
+  // exchange becomes an exchange of two pos values) and wave w the columns
+  // w, w + 4, ...; the right-hand side e_1 travels with wave 0.  Columns
+  // already pivoted hold the in-place inverse, which neither the later
+  // pivots nor the solution read, so only live columns are updated; the
+  // arithmetic on every element is the reference's.
+  __shared__ double cand_v[kSmallThreads / kWave];
+  __shared__ int cand_k[kSmallThreads / kWave];
+  __shared__ double dcol[kWave];
+  __shared__ double dpiv;
+  constexpr int kCW = (kB + 3) / 4;    // columns per wave
   const int n1 = nmat + 1;
-  double g[kB];
+  double g[kCW];
 #pragma unroll
-  for (int c = 0; c < kB; ++c) g[c] = (lane < n1 && c < n1) ? a[c] : 0.0;
+  for (int j = 0; j < kCW; ++j) {
+    const int c = wave + 4 * j;
+    g[j] = (lane < n1 && c < n1) ? bsh[lane][c] : 0.0;
+  }
   double bv = (lane == 0) ? 1.0 : 0.0;
   int pos = lane;
   unsigned int live = (n1 >= 32) ? 0xffffffffu : ((1u << n1) - 1u);   // columns not pivoted
   bool ok = true;
   for (int it = 0; it < n1; ++it) {
-    // pivot: largest |a(j, k)| over rows and columns not pivoted, first in
-    // row-major order of the current row indices on ties
-    double big = 0.0, bval = 0.0;
-    int best = 0x7fffffff;
+    // pivot: largest |a(j, k)| > 0 over rows and columns not pivoted, first
+    // in row-major order of the current row indices on ties
     const bool rowlive = lane < n1 && ((live >> pos) & 1u);
+    double big = -1.0;
+    int best = 0x7fffffff;
 #pragma unroll
-    for (int c = 0; c < kB; ++c) {
-      if ((live >> c) & 1u) {
-        const double v = fabs(g[c]);
-        if (rowlive && v > big) { big = v; bval = g[c]; best = pos * n1 + c; }
-      }
+    for (int j = 0; j < kCW; ++j) {
+      const int c = wave + 4 * j;
+      const double v = (rowlive && c < n1 && ((live >> c) & 1u)) ? fabs(g[j]) : -1.0;
+      const bool t = v > big;   // columns ascend: the first one wins ties
+      big = t ? v : big;
+      best = t ? pos * n1 + c : best;
     }
-    for (int o = 16; o > 0; o >>= 1) {   // candidates live in lanes 0..24
-      double ob = __shfl_xor(big, o, kWave);
-      double ov = __shfl_xor(bval, o, kWave);
-      int oi = __shfl_xor(best, o, kWave);
-      if (ob > big || (ob == big && oi < best)) { big = ob; bval = ov; best = oi; }
+    if (!(big > 0.0)) best = 0x7fffffff;
+    // rows live in lanes 0..31: five lane-pair stages
+    pivot_stage<1>(big, best);
+    pivot_stage<2>(big, best);
+    pivot_stage<3>(big, best);
+    pivot_stage<4>(big, best);
+    pivot_stage<5>(big, best);
+    if (lane == 0) {
+      cand_v[wave] = big;
+      cand_k[wave] = best;
+    }
+    __syncthreads();
+    big = cand_v[0];
+    best = cand_k[0];
+#pragma unroll
+    for (int w = 1; w < kSmallThreads / kWave; ++w) {
+      const double ov = cand_v[w];
+      const int ok2 = cand_k[w];
+      const bool t = (ov > big) || (ov == big && ok2 < best);
+      big = t ? ov : big;
+      best = t ? ok2 : best;
     }
     best = __builtin_amdgcn_readfirstlane(best);
     if (best == 0x7fffffff) { ok = false; break; }
@@ -302,38 +322,44 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(int nblk, int diis
     // the lane holding row irow becomes pivot row icol
     const int lr = __ffsll((unsigned long long)__ballot(lane < n1 && pos == irow)) - 1;
     if (lr < 0) { ok = false; break; }
+    // each row's multiplier a(j, icol), from the wave that holds the column,
+    // read before the pivot row is scaled
+    if (wave == (icol & 3)) {
+      const int jj = icol >> 2;
+      double v = 0.0;
+#pragma unroll
+      for (int j = 0; j < kCW; ++j) v = (j == jj) ? g[j] : v;
+      dcol[lane] = v;
+      if (lane == 0) dpiv = lane_value(v, lr);
+    }
+    __syncthreads();
+    const double dum = dcol[lane];
+    const double d = dpiv;
+    if (d == 0.0) { ok = false; break; }
+    const double pinv = 1.0 / d;
     if (irow != icol) {
       int np = pos;
       if (lane < n1 && pos == icol) np = irow;
       if (lane == lr) np = icol;
       pos = np;
     }
-    const double d = lane_value(bval, 0);
-    if (d == 0.0) { ok = false; break; }
-    // each row's multiplier a(j, icol), read before the pivot row is scaled
-    double dum = 0.0;
-#pragma unroll
-    for (int c = 0; c < kB; ++c)
-      if ((live >> c) & 1u) dum += (c == icol) ? g[c] : 0.0;
     live &= ~(1u << icol);
-    const double pinv = 1.0 / d;
-    if (lane == lr) {
+    // scale the pivot row (a(icol, :) * pivinv) and eliminate column icol
+    // from the other rows
 #pragma unroll
-      for (int c = 0; c < kB; ++c)
-        if ((live >> c) & 1u) g[c] = g[c] * pinv;
-      bv = bv * pinv;
-    }
-    // eliminate column icol from the other rows
-    const double bp = lane_value(bv, lr);
-#pragma unroll
-    for (int c = 0; c < kB; ++c) {
-      if ((live >> c) & 1u) {
-        const double pc = lane_value(g[c], lr);
-        if (lane != lr) g[c] = g[c] - pc * dum;
+    for (int j = 0; j < kCW; ++j) {
+      const int c = wave + 4 * j;
+      if (c < kB && ((live >> c) & 1u)) {
+        const double pc = lane_value(g[j], lr) * pinv;
+        g[j] = (lane == lr) ? pc : g[j] - pc * dum;
       }
     }
-    if (lane != lr) bv = bv - bp * dum;
+    if (wave == 0) {
+      const double bp = lane_value(bv, lr) * pinv;
+      bv = (lane == lr) ? bp : bv - bp * dum;
+    }
   }
+  if (wave != 0) return;
   if (ok) {
     if (lane < n1 && pos >= 1) st->coef[pos - 1] = bv;
   }
@@ -526,7 +552,7 @@ int build_rhs(ddx_handle *h, const double *d_phi, double *d_rhs) {
                      h->ui.p, h->cavidx.p, d_phi, ln.P.p);
   DDX_HIP(hipGetLastError());
   // rhs(lm, i) = sum_n Y(lm, n) w_n g(n, i)   (intrhs per sphere, cosmo.f90:114-116)
-  return project(h, h->stream, 1.0, ln.P.p, d_rhs, nullptr);
+  return project(h, h->stream, 1.0, ln.P.p, d_rhs, nullptr, 1);
 }
 
 int energy_dot(ddx_handle *h, const double *a, const double *b, double scale, double *d_out) {
@@ -546,6 +572,7 @@ SysArgs sys_args(const ddx_handle *h, Lane &ln) {
   SysArgs a;
   a.rhs = ln.rhs;
   a.y = ln.y.p;
+  a.ny = ln.star ? 1 : kProjSplit;
   a.x = ln.x;
   a.xs = ln.star ? nullptr : ln.xs.p;
   a.xd = ln.xd.p;
@@ -587,7 +614,7 @@ int solve_begin(ddx_handle *h, Lane &ln, int star, int diis_max, double tol, con
     return DDX_ERR_ARG;
   }
   size_t nn = (size_t)h->nylm * h->n;
-  DDX_TRY(ln.y.reserve(nn));
+  DDX_TRY(ln.y.reserve(nn * kProjSplit));
   DDX_TRY(ln.xnew.reserve(nn));
   DDX_TRY(ln.xs.reserve(nn + 1));
   if (diis_max > 0) {
@@ -636,7 +663,7 @@ int solve_enqueue(ddx_handle *h, Lane *const *lanes, int count) {
   }
   hipStream_t s = l0.stream;
   size_t nn = (size_t)h->nylm * h->n;
-  int nbj = grid_blocks(nn, kJThreads, kMaxBlk);
+  int nbj = grid_blocks(nn, kJThreads * 2, kMaxBlk);   // two elements per thread
   int nbe = grid_blocks(h->n, kJThreads / kWave, 1024);
   const double *invl1 = h->cf.p + h->nylm;   // 1/(l+1) table stored after cf
   const int diis_max = l0.diis_max;
@@ -646,11 +673,11 @@ int solve_enqueue(ddx_handle *h, Lane *const *lanes, int count) {
   for (int g = 0; g < l0.group && l0.enqueued < l0.rounds; ++g, ++l0.enqueued) {
     h->tag_round = l0.enqueued;
     if (pair) {
-      DDX_TRY(apply_pair(h, l0, l0.y.p, *lanes[1], lanes[1]->x, lanes[1]->y.p));
+      DDX_TRY(apply_pair(h, l0, l0.y.p, *lanes[1], lanes[1]->x, lanes[1]->y.p, kProjSplit));
     } else if (l0.star) {
       DDX_TRY(apply_lstarx(h, l0, l0.x, l0.y.p, l0.diis.p));
     } else {
-      DDX_TRY(apply_lx(h, l0, l0.x, l0.y.p, l0.diis.p, true));
+      DDX_TRY(apply_lx(h, l0, l0.x, l0.y.p, l0.diis.p, true, kProjSplit));
     }
     {
       TimedLaunch tl(h, DDX_K_JACOBI, s);

@@ -95,7 +95,70 @@ __device__ __forceinline__ double sweep_triple(double cthe, double sthe, double 
   return s;
 }
 
-constexpr int kRedChunk = 32;
+// List length per lane after s halvings of n values.
+__host__ __device__ constexpr int reduced_len(int n, int s) {
+  return s == 0 ? n : reduced_len((n + 1) / 2, s - 1);
+}
+
+// 64-bit value of the partner lane of stage S of wave_reduce_split (S >= 2):
+// mirror within 16 lanes, mirror within 8, then quad swaps by 2 and by 1
+template <int S>
+__device__ __forceinline__ double dpp_partner(double v) {
+  constexpr int ctrl = S == 2 ? 0x140 : S == 3 ? 0x141 : S == 4 ? 0x4E : 0xB1;
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), ctrl, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), ctrl, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Stage S of a reduce-scatter across the wave: each lane splits its list of
+// N values into halves x = v[0, H) and y = v[H, N), pairs with a lane that
+// differs in lane bit 5 - S (and holds the same columns), keeps one half
+// (bit clear: x, set: y) and adds the partner's copy of that half.  Stage 0
+// pairs lanes 32 apart and stage 1 lanes 16 apart by the gfx950 row swaps,
+// stages 2..5 use DPP.  After six stages each lane holds the wave sums of
+// columns base .. base + len - 1.
+template <int S, int N>
+__device__ __forceinline__ void wave_reduce_split(double *v, int &base, int &len) {
+  constexpr int H = (N + 1) / 2;
+  const bool side = (threadIdx.x >> (5 - S)) & 1;
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const double x = v[k];
+    const double y = (k + H < N) ? v[k + H] : 0.0;
+    if constexpr (S < 2) {
+      const long long bx = __double_as_longlong(x), by = __double_as_longlong(y);
+      const unsigned xl = (unsigned)(bx & 0xffffffffll), xh = (unsigned)(bx >> 32);
+      const unsigned yl = (unsigned)(by & 0xffffffffll), yh = (unsigned)(by >> 32);
+      unsigned nxl, nyl, nxh, nyh;
+      if constexpr (S == 0) {
+        auto rl = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
+        auto rh = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
+        nxl = rl[0]; nyl = rl[1]; nxh = rh[0]; nyh = rh[1];
+      } else {
+        auto rl = __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
+        auto rh = __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
+        nxl = rl[0]; nyl = rl[1]; nxh = rh[0]; nyh = rh[1];
+      }
+      // the swap leaves, in every lane, its own half and the partner's copy
+      // of it, one in each register
+      const double nx = __longlong_as_double(((long long)nxh << 32) | nxl);
+      const double ny = __longlong_as_double(((long long)nyh << 32) | nyl);
+      v[k] = nx + ny;
+    } else {
+      const double keep = side ? y : x;
+      const double send = side ? x : y;
+      v[k] = keep + dpp_partner<S>(send);
+    }
+  }
+  if (side) {
+    base += H;
+    len = len > H ? len - H : 0;
+  } else {
+    len = len < H ? len : H;
+  }
+  if constexpr (S < 5) wave_reduce_split<S + 1, H>(v, base, len);
+}
 
 // Triples beg..end-1 of container c: rw[tslot] = W (R . X_c) when DIR, and
 // when ADJ the chunk's share of (L*S)_c, -cf * sum_t w_n xi(n, o) W p~ trig,
@@ -105,75 +168,121 @@ __device__ __forceinline__ void sweep_chunk(
     int c, int beg, int end, const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
     const int *__restrict__ tslot, const int *__restrict__ tp, const double *__restrict__ Xs,
     double *__restrict__ rw, const double *__restrict__ xiw, const double *__restrict__ cf,
-    double *__restrict__ yp, double (*red)[kRedChunk + 1]) {
+    double *__restrict__ yp) {
   constexpr int NY = (LMAX + 1) * (LMAX + 1);
   const int lane = threadIdx.x;
   // X'_c in LDS: every lane reads the same word (broadcast)
   __shared__ double xsh[DIR ? NY : 1];
   if (DIR) {
+    __syncthreads();   // reads of the previous segment's X' are done
     for (int k = lane; k < NY; k += kWave) xsh[k] = Xs[(size_t)c * NY + k];
     __syncthreads();
   }
   double acc[ADJ ? NY : 1];
 #pragma unroll
   for (int k = 0; k < (ADJ ? NY : 1); ++k) acc[k] = 0.0;
-  for (int k = beg + lane; k < end; k += kWave) {
-    const double4 g = tgeo[k];
-    const double2 tw = ttw[k];
-    const double coef = ADJ ? xiw[tp[k]] * tw.y : 0.0;
+  // software pipeline: the geometry of the next triple and xi of its owner
+  // point load while the current one is evaluated (tp two triples ahead)
+  int k = beg + lane;
+  double4 g = make_double4(0.0, 0.0, 0.0, 0.0);
+  double2 tw = make_double2(0.0, 0.0);
+  double xv = 0.0;
+  int tpn = 0;
+  if (k < end) {
+    g = tgeo[k];
+    tw = ttw[k];
+    if (ADJ) xv = xiw[tp[k]];
+  }
+  if (ADJ && k + kWave < end) tpn = tp[k + kWave];
+  for (; k < end; k += kWave) {
+    const int kn = k + kWave;
+    double4 gn = g;
+    double2 twn = tw;
+    double xvn = 0.0;
+    int tpnn = 0;
+    if (kn < end) {
+      gn = tgeo[kn];
+      twn = ttw[kn];
+      if (ADJ) xvn = xiw[tpn];
+    }
+    if (ADJ && kn + kWave < end) tpnn = tp[kn + kWave];
+    const double coef = ADJ ? xv * tw.y : 0.0;
     const double v = sweep_triple<LMAX, DIR, ADJ>(g.x, g.y, g.z, g.w, tw.x, xsh, coef, acc);
     if (DIR) rw[tslot[k]] = tw.y * v;
+    g = gn;
+    tw = twn;
+    xv = xvn;
+    tpn = tpnn;
   }
   if (ADJ) {
-    double *yc = yp;
+    // sum of acc over the 64 lanes, column by column, spread over the lanes
+    int base = 0, len = NY;
+    wave_reduce_split<0, NY>(acc, base, len);
+    constexpr int NF = reduced_len(NY, 6);
 #pragma unroll
-    for (int c0 = 0; c0 < NY; c0 += kRedChunk) {
-#pragma unroll
-      for (int k = 0; k < kRedChunk; ++k)
-        if (c0 + k < NY) red[lane][k] = acc[c0 + k];
-      __syncthreads();
-      if (lane < kRedChunk && c0 + lane < NY) {
-        double r = 0.0;
-        for (int q = 0; q < kWave; ++q) r += red[q][lane];
-        yc[c0 + lane] = -(cf[c0 + lane] * r);
-      }
-      __syncthreads();
-    }
+    for (int k = 0; k < NF; ++k)
+      if (k < len) yp[base + k] = -(cf[base + k] * acc[k]);
   }
 }
 
-// One wave per chunk of triples (blockIdx.x).  DIR: L X terms (rw, slot
-// order); ADJ: the chunk's partial L* S (yp, per chunk).  A side whose solve
+// One wave per worker (blockIdx.x of G): units floor(w U / G) ..
+// floor((w+1) U / G) - 1, container by container.  DIR: L X terms (rw,
+// slot order); ADJ: one partial L* S per segment (yp).  A side whose solve
 // has ended (gate done) is skipped.
 template <int LMAX, bool DIR, bool ADJ>
 __global__ __launch_bounds__(kWave) void k_sweep(
-    const int2 *__restrict__ chunk, const int *__restrict__ inl, const int *__restrict__ pairoff,
+    long long U, long long G, const int *__restrict__ uoff, const int *__restrict__ segoff,
+    const int *__restrict__ inl, const int *__restrict__ pairoff, int n,
     const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
     const int *__restrict__ tslot, const int *__restrict__ tp, const double *__restrict__ Xs,
     double *__restrict__ rw, const double *__restrict__ xiw, const double *__restrict__ cf,
     double *__restrict__ yp, const DiisState *__restrict__ gdir,
     const DiisState *__restrict__ gadj) {
-  __shared__ double red[ADJ ? kWave : 1][kRedChunk + 1];
   const bool dir = DIR && !(gdir && gdir->done);
   const bool adj = ADJ && !(gadj && gadj->done);
-  const int2 ch = chunk[blockIdx.x];
-  const int c = ch.x, beg = ch.y;
-  const int end = min(beg + kTripleChunk, pairoff[inl[c + 1]]);
+  if (!dir && !adj) return;
   constexpr int NY = (LMAX + 1) * (LMAX + 1);
-  yp += (size_t)blockIdx.x * NY;
-  if constexpr (DIR && ADJ) {
-    if (dir && adj)
-      sweep_chunk<LMAX, true, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, yp, red);
-    else if (dir)
-      sweep_chunk<LMAX, true, false>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, yp, red);
-    else if (adj)
-      sweep_chunk<LMAX, false, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, yp, red);
-  } else if constexpr (DIR) {
-    if (dir)
-      sweep_chunk<LMAX, true, false>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, yp, red);
-  } else {
-    if (adj)
-      sweep_chunk<LMAX, false, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, yp, red);
+  const long long w = blockIdx.x;
+  const int u0 = (int)(w * U / G), u1 = (int)((w + 1) * U / G);
+  // container of unit u0: the last c with uoff[c] <= u0
+  int lo = 0, hi = n;   // uoff[lo] <= u0 < uoff[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (uoff[mid] <= u0) lo = mid;
+    else hi = mid;
+  }
+  int c = lo;
+  while (uoff[c + 1] <= u0) ++c;   // skip containers without triples
+  // global index of this worker's first segment
+  int seg = segoff[c];
+  if (u0 > uoff[c]) {
+    const long long wlo = (((long long)uoff[c] + 1) * G + U - 1) / U;
+    seg += 1 + (int)(w - wlo);
+  }
+  for (int u = u0; u < u1;) {
+    while (uoff[c + 1] <= u) {
+      ++c;
+      seg = segoff[c];
+    }
+    const int ue = min(u1, uoff[c + 1]);
+    const int cb = pairoff[inl[c]], ce = pairoff[inl[c + 1]];
+    const int beg = cb + (u - uoff[c]) * kWave;
+    const int end = min(ce, cb + (ue - uoff[c]) * kWave);
+    double *ys = yp + (size_t)seg * NY;
+    if constexpr (DIR && ADJ) {
+      if (dir && adj)
+        sweep_chunk<LMAX, true, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
+      else if (dir)
+        sweep_chunk<LMAX, true, false>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
+      else
+        sweep_chunk<LMAX, false, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
+    } else if constexpr (DIR) {
+      sweep_chunk<LMAX, true, false>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
+    } else {
+      sweep_chunk<LMAX, false, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
+    }
+    u = ue;
+    ++seg;
   }
 }
 
@@ -194,16 +303,16 @@ __global__ __launch_bounds__(256) void k_gather_points(size_t np, const int *__r
   P[p] = s;
 }
 
-// (L*S)_c = sum of its chunks' shares, in chunk order
-__global__ void k_chunk_sum(int n, int nylm, const int *__restrict__ choff,
-                            const double *__restrict__ yp, double *__restrict__ y,
-                            const DiisState *__restrict__ gate) {
+// (L*S)_c = sum of its segments' shares, in segment order
+__global__ void k_seg_sum(int n, int nylm, const int *__restrict__ segoff,
+                          const double *__restrict__ yp, double *__restrict__ y,
+                          const DiisState *__restrict__ gate) {
   if (gate && gate->done) return;
   size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (size_t)n * nylm) return;
   const int c = (int)(k / nylm), lm = (int)(k % nylm);
   double r = 0.0;
-  for (int j = choff[c]; j < choff[c + 1]; ++j) r += yp[(size_t)j * nylm + lm];
+  for (int j = segoff[c]; j < segoff[c + 1]; ++j) r += yp[(size_t)j * nylm + lm];
   y[k] = r;
 }
 
@@ -217,11 +326,12 @@ __global__ void k_cf_scale(size_t nn, int nylm, const double *__restrict__ cf,
 template <int LMAX, bool DIR, bool ADJ>
 int launch_sweep(ddx_handle *h, hipStream_t s, const double *Xs, double *rw, const double *xiw,
                  double *yp, const DiisState *gdir, const DiisState *gadj) {
-  if (h->nchunk == 0) return DDX_OK;
+  if (h->nworker == 0) return DDX_OK;
   TimedLaunch tl(h, (DIR && ADJ) ? DDX_K_SWEEP_PAIR : (DIR ? DDX_K_SWEEP_L : DDX_K_SWEEP_LSTAR), s);
-  hipLaunchKernelGGL((k_sweep<LMAX, DIR, ADJ>), dim3(h->nchunk), dim3(kWave), 0, s, h->chunk.p,
-                     h->inl.p, h->pairoff.p, h->tgeo.p, h->ttw.p, h->tslot.p, h->tp.p, Xs, rw,
-                     xiw, h->cf.p, yp, gdir, gadj);
+  hipLaunchKernelGGL((k_sweep<LMAX, DIR, ADJ>), dim3(h->nworker), dim3(kWave), 0, s,
+                     (long long)h->nunit, (long long)h->nworker, h->uoff.p, h->segoff.p, h->inl.p,
+                     h->pairoff.p, h->n, h->tgeo.p, h->ttw.p, h->tslot.p, h->tp.p, Xs, rw, xiw,
+                     h->cf.p, yp, gdir, gadj);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
@@ -234,7 +344,7 @@ int sweep_lmax(ddx_handle *h, hipStream_t s, int mode, const double *Xs, double 
   return launch_sweep<LMAX, false, true>(h, s, Xs, rw, xiw, y, gdir, gadj);
 }
 
-// mode: 1 = L X terms, 2 = L* S shares per chunk, 3 = both
+// mode: 1 = L X terms, 2 = L* S shares per segment, 3 = both
 int sweep(ddx_handle *h, hipStream_t s, int mode, const double *Xs, double *rw,
           const double *xiw, double *y, const DiisState *gdir, const DiisState *gadj) {
   if (h->n == 0) return DDX_OK;
@@ -261,13 +371,13 @@ int sweep(ddx_handle *h, hipStream_t s, int mode, const double *Xs, double *rw,
 }
 
 // P = gathered L X terms of each point; y = - Yw^T-projection of P
-int finish_lx(ddx_handle *h, Lane &ln, double *y, const DiisState *gate) {
+int finish_lx(ddx_handle *h, Lane &ln, double *y, const DiisState *gate, int ysplit) {
   hipStream_t s = ln.stream;
   hipLaunchKernelGGL(k_gather_points, dim3((h->np + 255) / 256), dim3(256), 0, s, h->np,
                      h->ptoff.p, ln.rw.p, h->ui.p, ln.P.p, gate);
   DDX_HIP(hipGetLastError());
   // y(lm, o) = - sum_n Yw(lm, n) P(n, o)
-  return project(h, s, -1.0, ln.P.p, y, gate);
+  return project(h, s, -1.0, ln.P.p, y, gate, ysplit);
 }
 
 }  // namespace
@@ -282,7 +392,7 @@ int cf_scale(ddx_handle *h, hipStream_t s, const double *x, double *xs) {
 }
 
 int apply_lx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate,
-             bool xs_ready) {
+             bool xs_ready, int ysplit) {
   hipStream_t s = ln.stream;
   size_t nn = (size_t)h->nylm * h->n;
   DDX_TRY(ln.rw.reserve(h->ntri + 1));
@@ -290,14 +400,14 @@ int apply_lx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisStat
   DDX_TRY(ln.xs.reserve(nn + 1));
   if (!xs_ready) DDX_TRY(cf_scale(h, s, x, ln.xs.p));
   DDX_TRY(sweep(h, s, 1, ln.xs.p, ln.rw.p, nullptr, nullptr, gate, nullptr));
-  return finish_lx(h, ln, y, gate);
+  return finish_lx(h, ln, y, gate, ysplit);
 }
 
-int chunk_sum(ddx_handle *h, hipStream_t s, const double *yp, double *y, const DiisState *gate) {
+int seg_sum(ddx_handle *h, hipStream_t s, const double *yp, double *y, const DiisState *gate) {
   size_t nn = (size_t)h->nylm * h->n;
   if (nn == 0) return DDX_OK;
-  hipLaunchKernelGGL(k_chunk_sum, dim3((nn + 255) / 256), dim3(256), 0, s, h->n, h->nylm,
-                     h->choff.p, yp, y, gate);
+  hipLaunchKernelGGL(k_seg_sum, dim3((nn + 255) / 256), dim3(256), 0, s, h->n, h->nylm,
+                     h->segoff.p, yp, y, gate);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
@@ -305,24 +415,25 @@ int chunk_sum(ddx_handle *h, hipStream_t s, const double *yp, double *y, const D
 int apply_lstarx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate) {
   hipStream_t s = ln.stream;
   DDX_TRY(ln.xiw.reserve(h->np));
-  DDX_TRY(ln.yp.reserve((size_t)h->nchunk * h->nylm + 1));
+  DDX_TRY(ln.yp.reserve((size_t)h->nseg_cap * h->nylm + 1));
   // w_n xi(n, o) = w_n sum_lm Y(lm, n) x(lm, o)
   DDX_TRY(expand(h, s, x, ln.xiw.p, true, gate));
   DDX_TRY(sweep(h, s, 2, nullptr, nullptr, ln.xiw.p, ln.yp.p, nullptr, gate));
-  return chunk_sum(h, s, ln.yp.p, y, gate);
+  return seg_sum(h, s, ln.yp.p, y, gate);
 }
 
-int apply_pair(ddx_handle *h, Lane &ld, double *yd, Lane &la, const double *xa, double *ya) {
+int apply_pair(ddx_handle *h, Lane &ld, double *yd, Lane &la, const double *xa, double *ya,
+               int ysplit) {
   hipStream_t s = ld.stream;
   DDX_TRY(ld.rw.reserve(h->ntri + 1));
   DDX_TRY(ld.P.reserve(h->np));
   DDX_TRY(la.xiw.reserve(h->np));
-  DDX_TRY(la.yp.reserve((size_t)h->nchunk * h->nylm + 1));
+  DDX_TRY(la.yp.reserve((size_t)h->nseg_cap * h->nylm + 1));
   const DiisState *gd = ld.diis.p, *ga = la.diis.p;
   DDX_TRY(expand(h, s, xa, la.xiw.p, true, ga));
   DDX_TRY(sweep(h, s, 3, ld.xs.p, ld.rw.p, la.xiw.p, la.yp.p, gd, ga));
-  DDX_TRY(chunk_sum(h, s, la.yp.p, ya, ga));
-  return finish_lx(h, ld, yd, gd);
+  DDX_TRY(seg_sum(h, s, la.yp.p, ya, ga));
+  return finish_lx(h, ld, yd, gd, ysplit);
 }
 
 }  // namespace ddx
