@@ -5,9 +5,9 @@
 // The potential kernel can return the field of the same sources as well
 // (the solute field at the cavity points that the forces contract with
 // zeta), which costs one all-pairs pass less than a separate field kernel.
-// 1/r comes from the hardware reciprocal square root refined by two Newton
-// steps (relative error a few ulp), which replaces an IEEE sqrt and divide
-// per pair.
+// 1/r comes from the hardware reciprocal square root refined by one
+// third-order step (relative error a few ulp), which replaces an IEEE sqrt
+// and divide per pair.
 // When there are few targets (the field of the cavity charges at the nuclei)
 // the sources are split over several workgroups and the partial fields are
 // summed afterwards in a fixed order, so results are deterministic.
@@ -19,13 +19,13 @@ namespace {
 
 constexpr int kCT = 256;
 
-// 1/sqrt(x) for x > 0: v_rsq_f64 and two Newton-Raphson steps
+// 1/sqrt(x) for x > 0: v_rsq_f64 (relative error below 2^-24 on gfx950)
+// and one third-order correction: with e = 1 - x y^2, y (1 + e/2 + 3e^2/8)
+// leaves a relative error of order e^3 plus rounding (about 1 ulp)
 __device__ __forceinline__ double rsqrt_nr(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  const double h = 0.5 * x;
-  y = y * (1.5 - h * y * y);
-  y = y * (1.5 - h * y * y);
-  return y;
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  return fma(y, e * fma(0.375, e, 0.5), y);
 }
 
 // phi(j) and, with FIELD, the field ef(:, j) of the same sources, from one

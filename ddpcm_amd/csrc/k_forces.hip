@@ -18,119 +18,101 @@
 namespace ddx {
 namespace {
 
-__host__ __device__ constexpr int pli(int l, int m) { return l * (l + 1) / 2 + m; }
 __host__ __device__ constexpr int lmi(int l, int m) { return l * l + l + m; }
 
+// Per triple, with X' = cf sigma_c (cf = facs 4pi/(2l+1)) and s the unit
+// vector from the container to the point, e_theta = (cos th cos ph,
+// cos th sin ph, -sin th), e_phi = (-sin ph, cos ph, 0) / sin th:
+//   beta = sum_l t^l A_l,   alp = sum_{l>=1} t^(l-1) (l A_l s + Bt_l e_theta + Bp_l e_phi)
+//   A_l  = sum_m P_l^m u_c(m),  u_c = cos(m ph) X'_l,m + sin(m ph) X'_l,-m
+//   Bt_l = P_l^1 X'_l,0 - sum_{m>=1} dP_lm u_c(m),
+//          dP_lm = (l+m)(l-m+1) P_l^(m-1) / 2 - P_l^(m+1) / 2
+//   Bp_l = sum_{m>=1} m P_l^m u_s(m),  u_s = cos(m ph) X'_l,-m - sin(m ph) X'_l,m
+// which regroups the l Y s + dY terms of dbasis (ddcosmo.f90:904-1043).  At
+// the poles (sin th = 0) e_phi = (0, 1, 0) and the e_phi weight of m is
+// -dP_lm int(cos th) (ddcosmo.f90:970-971).  The Legendre functions run
+// l-outer, three rows in registers, with polleg's recurrence.
 template <int LMAX>
 __global__ __launch_bounds__(256) void k_force_triple(
-    int ngrid, const int *__restrict__ inl, const int *__restrict__ pairoff,
-    const double4 *__restrict__ sph, const double4 *__restrict__ grid,
+    const int *__restrict__ inl, const int *__restrict__ pairoff,
     const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
-    const int *__restrict__ tp, const double *__restrict__ sigma,
-    const double *__restrict__ facs, const double *__restrict__ ifacl,
+    const double *__restrict__ sigma, const double *__restrict__ cf,
     double4 *__restrict__ fb) {
   constexpr int NY = (LMAX + 1) * (LMAX + 1);
+  __shared__ double xs[NY];
   const int c = blockIdx.x;
+  for (int k = threadIdx.x; k < NY; k += 256) xs[k] = cf[k] * sigma[(size_t)c * NY + k];
+  __syncthreads();
   const int beg = pairoff[inl[c]], end = pairoff[inl[c + 1]];
-  const double *sg = sigma + (size_t)c * NY;
-  const double4 sc = sph[c];
   for (int k = beg + threadIdx.x; k < end; k += 256) {
     const double4 g = tgeo[k];
     const double t = ttw[k].x;
-    const int p = tp[k];
-    const int o = p / ngrid, n = p - o * ngrid;
-    double v[3];
-    point_vec(sph[o], grid[n], sc, v);
-    const double vv = norm3(v);
-    const double s0 = v[0] / vv, s1 = v[1] / vv, s2 = v[2] / vv;
     const double cthe = g.x, sthe = g.y, cphi = g.z, sphi = g.w;
-    // Legendre table (polleg) and cos/sin(m phi) (trgev)
-    double P[(LMAX + 1) * (LMAX + 2) / 2];
-    {
-      double pmm = 1.0;
-#pragma unroll
-      for (int m = 0; m <= LMAX; ++m) {
-        P[pli(m, m)] = pmm;
-        if (m < LMAX) {
-          double p0 = pmm, p1 = cthe * (2.0 * m + 1.0) * pmm;
-          P[pli(m + 1, m)] = p1;
-#pragma unroll
-          for (int l = m + 2; l <= LMAX; ++l) {
-            double p2 = (cthe * (2.0 * l - 1.0) * p1 - (double)(l + m - 1) * p0) * (1.0 / (double)(l - m));
-            P[pli(l, m)] = p2;
-            p0 = p1;
-            p1 = p2;
-          }
-        }
-        pmm = -pmm * (2.0 * m + 1.0) * sthe;
-      }
-    }
     double cs[LMAX + 1], sn[LMAX + 1];
     cs[0] = 1.0;
     sn[0] = 0.0;
-    if (LMAX >= 1) { cs[1 < LMAX + 1 ? 1 : 0] = cphi; sn[1 < LMAX + 1 ? 1 : 0] = sphi; }
+    if constexpr (LMAX >= 1) {
+      cs[1] = cphi;
+      sn[1] = sphi;
 #pragma unroll
-    for (int m = 2; m <= LMAX; ++m) {
-      cs[m] = 2.0 * cphi * cs[m - 1] - cs[m - 2];
-      sn[m] = 2.0 * cphi * sn[m - 1] - sn[m - 2];
+      for (int m = 2; m <= LMAX; ++m) {
+        cs[m] = 2.0 * cphi * cs[m - 1] - cs[m - 2];
+        sn[m] = 2.0 * cphi * sn[m - 1] - sn[m - 2];
+      }
     }
     const bool pole = (sthe == 0.0);
-    const double et0 = cthe * cphi, et1 = cthe * sphi, et2 = -sthe;
-    double ep0 = 0.0, ep1 = 1.0;
-    if (!pole) { ep0 = -sphi / sthe; ep1 = cphi / sthe; }
-    const double vs = (double)(int)cthe;   // integer pole factor of dbasis (ddcosmo.f90:971)
-    double beta = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    double tl = 1.0, tlm1 = 1.0;   // t^l and t^(l-1)
+    const double vs = (double)(int)cthe;
+    double p2[LMAX + 1], p1[LMAX + 1], pc[LMAX + 1];   // rows l-2, l-1, l
+    double beta = 0.0, A = 0.0, Bt = 0.0, Bp = 0.0;
+    double tl = 1.0, tlm1 = 1.0;
 #pragma unroll
     for (int l = 0; l <= LMAX; ++l) {
-      const double fac = tl * ifacl[lmi(l, 0)];     // t^l 4pi/(2l+1), intmlp
-      const double faca = tlm1 * ifacl[lmi(l, 0)];  // t^(l-1) 4pi/(2l+1), d/dt of it / l
-      // m = 0
-      double fln = facs[lmi(l, 0)];
-      double y0 = fln * P[pli(l, 0)];
-      double sA = y0 * sg[lmi(l, 0)];
-      double d0 = 0.0, d1 = 0.0, d2 = 0.0;
-      if (l > 0) {
-        double gy = fln * P[pli(l, 1)] * sg[lmi(l, 0)];
-        d0 += gy * et0; d1 += gy * et1; d2 += gy * et2;
+#pragma unroll
+      for (int m = 0; m <= l; ++m) {
+        if (l == 0) pc[0] = 1.0;
+        else if (m == l) pc[m] = -(2.0 * l - 1.0) * sthe * p1[l - 1];
+        else if (m == l - 1) pc[m] = cthe * (2.0 * m + 1.0) * p1[m];
+        else pc[m] = (cthe * (2.0 * l - 1.0) * p1[m] - (double)(l + m - 1) * p2[m]) *
+                     (1.0 / (double)(l - m));
       }
+      double al = pc[0] * xs[lmi(l, 0)];
+      double bt = (l > 0) ? pc[1] * xs[lmi(l, 0)] : 0.0;
+      double bp = 0.0;
 #pragma unroll
       for (int m = 1; m <= l; ++m) {
-        fln = facs[lmi(l, m)];
-        const double plm = fln * P[pli(l, m)];
-        const double pp1 = (m < l) ? -0.5 * P[pli(l, m + 1)] : 0.0;
-        const double pm1 = 0.5 * ((double)(l + m) * (double)(l - m + 1) * P[pli(l, m - 1)]);
-        const double pp = pp1 + pm1;
-        const double sp = sg[lmi(l, m)], sm = sg[lmi(l, -m)];
-        sA += plm * cs[m] * sp + plm * sn[m] * sm;
-        // gradients of Y_l,m (cos) and Y_l,-m (sin)
-        double gp0, gp1, gp2, gm0, gm1, gm2;
-        const double fpc = -fln * pp * cs[m], fps = -fln * pp * sn[m];
-        if (!pole) {
-          const double mc = (double)m * plm * sn[m], ms = (double)m * plm * cs[m];
-          gp0 = fpc * et0 - mc * ep0; gp1 = fpc * et1 - mc * ep1; gp2 = fpc * et2;
-          gm0 = fps * et0 + ms * ep0; gm1 = fps * et1 + ms * ep1; gm2 = fps * et2;
-        } else {
-          // VC = 0 for m > 0 components, VS = int(cthe) for m < 0
-          gp0 = fpc * et0; gp1 = fpc * et1; gp2 = fpc * et2;
-          const double q = -fln * pp * vs;
-          gm0 = fps * et0 + q * ep0; gm1 = fps * et1 + q * ep1; gm2 = fps * et2;
-        }
-        d0 += sp * gp0 + sm * gm0;
-        d1 += sp * gp1 + sm * gm1;
-        d2 += sp * gp2 + sm * gm2;
+        const double xp = xs[lmi(l, m)], xm = xs[lmi(l, -m)];
+        const double uc = cs[m] * xp + sn[m] * xm;
+        const double us = cs[m] * xm - sn[m] * xp;
+        const double dp = 0.5 * ((double)(l + m) * (double)(l - m + 1) * pc[m - 1]) +
+                          ((m < l) ? -0.5 * pc[m < l ? m + 1 : m] : 0.0);
+        al += pc[m] * uc;
+        bt -= dp * uc;
+        bp += (pole ? -dp * vs : (double)m * pc[m]) * us;
       }
-      beta += fac * sA;
+      beta += tl * al;
       if (l > 0) {
-        const double fl = (double)l;
-        a0 += faca * (fl * sA * s0 + d0);
-        a1 += faca * (fl * sA * s1 + d1);
-        a2 += faca * (fl * sA * s2 + d2);
+        A += tlm1 * ((double)l * al);
+        Bt += tlm1 * bt;
+        Bp += tlm1 * bp;
         tlm1 = tl;
       }
       tl *= t;
+#pragma unroll
+      for (int m = 0; m <= l; ++m) {
+        p2[m] = p1[m];
+        p1[m] = pc[m];
+      }
     }
-    fb[k] = make_double4(a0, a1, a2, beta);
+    const double s0 = sthe * cphi, s1 = sthe * sphi, s2 = cthe;
+    const double et0 = cthe * cphi, et1 = cthe * sphi, et2 = -sthe;
+    double ep0 = 0.0, ep1 = 1.0;
+    if (!pole) {
+      const double is = 1.0 / sthe;
+      ep0 = -sphi * is;
+      ep1 = cphi * is;
+    }
+    fb[k] = make_double4(A * s0 + Bt * et0 + Bp * ep0, A * s1 + Bt * et1 + Bp * ep1,
+                         A * s2 + Bt * et2, beta);
   }
 }
 
@@ -336,9 +318,8 @@ __global__ void k_sph3(int n, const double4 *__restrict__ sph, double *__restric
 template <int LMAX>
 int launch_triple(ddx_handle *h, const double *sigma, double4 *fb) {
   TimedLaunch tl(h, DDX_K_FORCES, h->stream);
-  hipLaunchKernelGGL(k_force_triple<LMAX>, dim3(h->n), dim3(256), 0, h->stream, h->ngrid,
-                     h->inl.p, h->pairoff.p, h->sph.p, h->grid.p, h->tgeo.p, h->ttw.p, h->tp.p,
-                     sigma, h->facs.p, h->cf.p + 2 * h->nylm, fb);
+  hipLaunchKernelGGL(k_force_triple<LMAX>, dim3(h->n), dim3(256), 0, h->stream, h->inl.p,
+                     h->pairoff.p, h->tgeo.p, h->ttw.p, sigma, h->cf.p, fb);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
