@@ -263,9 +263,15 @@ int ddx_init(ddx_handle **out, int n, const double *x, const double *y, const do
     }
   for (Lane &ln : h->lane)
     if (hipHostMalloc((void **)&ln.h_diis, sizeof(DiisState), hipHostMallocDefault) !=
-        hipSuccess) {
+            hipSuccess ||
+        hipHostMalloc((void **)&ln.h_head, 8 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
       set_error("hipHostMalloc failed");
       return fail(DDX_ERR_ALLOC);
+    }
+  for (auto &e : h->gev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      set_error("hipEventCreate failed");
+      return fail(DDX_ERR_HIP);
     }
   // one-time tables (ddcosmo.f90:225-267)
   if (lebedev_grid(h->ngrid, h->h_grid, h->h_w) != 0) {
@@ -308,8 +314,12 @@ int ddx_init(ddx_handle **out, int n, const double *x, const double *y, const do
 int ddx_free(ddx_handle *h) {
   if (!h) return DDX_OK;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  for (Lane &ln : h->lane)
+  for (Lane &ln : h->lane) {
     if (ln.h_diis) (void)hipHostFree(ln.h_diis);
+    if (ln.h_head) (void)hipHostFree(ln.h_head);
+  }
+  for (auto &e : h->gev)
+    if (e) (void)hipEventDestroy(e);
   for (auto &t : h->ktimer) {
     for (auto &r : t.recs) {
       (void)hipEventDestroy(r.b);

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdint>
 #include <cstdio>
@@ -106,6 +107,7 @@ struct Lane {
   DevArray<DiisState> diis;
   DevArray<int> misc;
   DiisState *h_diis = nullptr;  // pinned host mirror of diis
+  int *h_head = nullptr;        // pinned: two copies of (nmat, it, done, ok), one per group
   // the solve in flight (solve_begin .. solve_end)
   int serial = -1, star = 0, diis_max = 0, max_iter = 0, rounds = 0, enqueued = 0, group = 3;
   double tol = 0.0;
@@ -169,6 +171,7 @@ struct ddx_handle {
   ddx::Lane lane[2];
   ddx::DevArray<double> x, y, rhs, r, part;
   int solve_serial = 0;
+  hipEvent_t gev[2] = {};           // completion of a group's state copy
   int tag_solve = -1, tag_round = 0;  // attached to timed launches
 
   // problem data / results
@@ -181,35 +184,31 @@ struct ddx_handle {
 };
 
 namespace ddx {
-// Brackets one launch with events when the kernel class is being timed.
+// Start and stop events of one launch when its kernel class is being timed
+// (null otherwise).  They go to hipExtLaunchKernelGGL, which records them as
+// the kernel starts and ends, so the measured time is the kernel's own.
 struct TimedLaunch {
-  ddx_handle *h;
-  int id;
-  hipStream_t s;
-  hipEvent_t end = nullptr;
-  TimedLaunch(ddx_handle *hh, int kid, hipStream_t stream) : h(hh), id(kid), s(stream) {
-    KernelTimer &t = h->ktimer[id];
+  hipEvent_t b = nullptr, e = nullptr;
+  TimedLaunch(ddx_handle *h, int kid, hipStream_t) {
+    KernelTimer &t = h->ktimer[kid];
     if (!t.on) return;
     hipEvent_t ev[2] = {nullptr, nullptr};
-    for (auto &e : ev) {
+    for (auto &x : ev) {
       if (!t.pool.empty()) {
-        e = t.pool.back();
+        x = t.pool.back();
         t.pool.pop_back();
-      } else if (hipEventCreate(&e) != hipSuccess) {
-        e = nullptr;
+      } else if (hipEventCreate(&x) != hipSuccess) {
+        x = nullptr;
       }
     }
     if (!ev[0] || !ev[1]) {
-      for (auto e : ev)
-        if (e) t.pool.push_back(e);
+      for (auto x : ev)
+        if (x) t.pool.push_back(x);
       return;
     }
-    (void)hipEventRecord(ev[0], s);
     t.recs.push_back({ev[0], ev[1], h->tag_solve, h->tag_round});
-    end = ev[1];
-  }
-  ~TimedLaunch() {
-    if (end) (void)hipEventRecord(end, s);
+    b = ev[0];
+    e = ev[1];
   }
 };
 }  // namespace ddx

@@ -145,11 +145,11 @@ int coulomb_potential(ddx_handle *h, int nsrc, const double *q, const double4 *s
   if (ntrg <= 0) return DDX_OK;
   TimedLaunch tl(h, DDX_K_POTENTIAL, h->stream);
   if (ef)
-    hipLaunchKernelGGL(k_potential<true>, dim3((ntrg + kCT - 1) / kCT), dim3(kCT), 0, h->stream,
-                       nsrc, q, src, ntrg, trg3, phi, ef);
+    hipExtLaunchKernelGGL(k_potential<true>, dim3((ntrg + kCT - 1) / kCT), dim3(kCT), 0,
+                          h->stream, tl.b, tl.e, 0, nsrc, q, src, ntrg, trg3, phi, ef);
   else
-    hipLaunchKernelGGL(k_potential<false>, dim3((ntrg + kCT - 1) / kCT), dim3(kCT), 0, h->stream,
-                       nsrc, q, src, ntrg, trg3, phi, nullptr);
+    hipExtLaunchKernelGGL(k_potential<false>, dim3((ntrg + kCT - 1) / kCT), dim3(kCT), 0,
+                          h->stream, tl.b, tl.e, 0, nsrc, q, src, ntrg, trg3, phi, nullptr);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
@@ -172,8 +172,8 @@ int coulomb_field(ddx_handle *h, int nsrc, const double *q, const double *src3, 
   DDX_TRY(h->fpart.reserve((size_t)nsplit * 3 * ntrg));
   {
     TimedLaunch tl(h, DDX_K_FIELD, h->stream);
-    hipLaunchKernelGGL(k_field, dim3(tb, nsplit), dim3(kCT), 0, h->stream, nsrc, q, src3, ntrg, trg3,
-                       chunk > 0 ? chunk : kCT, h->fpart.p);
+    hipExtLaunchKernelGGL(k_field, dim3(tb, nsplit), dim3(kCT), 0, h->stream, tl.b, tl.e, 0, nsrc,
+                          q, src3, ntrg, trg3, chunk > 0 ? chunk : kCT, h->fpart.p);
   }
   DDX_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_field_reduce, dim3((3 * ntrg + 255) / 256), dim3(256), 0, h->stream, nsplit,

@@ -318,8 +318,8 @@ __global__ void k_sph3(int n, const double4 *__restrict__ sph, double *__restric
 template <int LMAX>
 int launch_triple(ddx_handle *h, const double *sigma, double4 *fb) {
   TimedLaunch tl(h, DDX_K_FORCES, h->stream);
-  hipLaunchKernelGGL(k_force_triple<LMAX>, dim3(h->n), dim3(256), 0, h->stream, h->inl.p,
-                     h->pairoff.p, h->tgeo.p, h->ttw.p, sigma, h->cf.p, fb);
+  hipExtLaunchKernelGGL(k_force_triple<LMAX>, dim3(h->n), dim3(256), 0, h->stream, tl.b, tl.e, 0,
+                        h->inl.p, h->pairoff.p, h->tgeo.p, h->ttw.p, sigma, h->cf.p, fb);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }

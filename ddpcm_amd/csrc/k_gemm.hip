@@ -138,8 +138,8 @@ int project(ddx_handle *h, hipStream_t s, double alpha, const double *P, double 
             const DiisState *gate, int ysplit) {
   if (h->n <= 0) return DDX_OK;
   TimedLaunch tl(h, DDX_K_PROJECT, s);
-  hipLaunchKernelGGL(k_project, dim3((h->n + 15) / 16, ysplit), dim3(256), 0, s, h->nylm,
-                     h->ngrid, h->n, alpha, h->Yw.p, P, y, gate);
+  hipExtLaunchKernelGGL(k_project, dim3((h->n + 15) / 16, ysplit), dim3(256), 0, s, tl.b, tl.e, 0,
+                        h->nylm, h->ngrid, h->n, alpha, h->Yw.p, P, y, gate);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
@@ -149,8 +149,8 @@ int expand(ddx_handle *h, hipStream_t s, const double *S, double *xi, bool wscal
   if (h->n <= 0) return DDX_OK;
   dim3 g((h->ngrid + 63) / 64, (h->n + 15) / 16);
   TimedLaunch tl(h, DDX_K_EXPAND, s);
-  hipLaunchKernelGGL(k_expand, g, dim3(256), 0, s, h->nylm, h->ngrid, h->n, h->Y.p, S,
-                     h->grid.p, wscale ? 1 : 0, xi, gate);
+  hipExtLaunchKernelGGL(k_expand, g, dim3(256), 0, s, tl.b, tl.e, 0, h->nylm, h->ngrid, h->n,
+                        h->Y.p, S, h->grid.p, wscale ? 1 : 0, xi, gate);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }

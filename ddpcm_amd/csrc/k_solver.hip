@@ -653,7 +653,10 @@ int solve_begin(ddx_handle *h, Lane &ln, int star, int diis_max, double tol, con
   return DDX_OK;
 }
 
-int solve_enqueue(ddx_handle *h, Lane *const *lanes, int count) {
+// Enqueues the next group of rounds and the copy of each lane's state header
+// (nmat, it, done, ok) into slot `slot` of its pinned h_head, marked by
+// h->gev[slot].
+int solve_enqueue(ddx_handle *h, Lane *const *lanes, int count, int slot) {
   Lane &l0 = *lanes[0];
   const bool pair = count == 2;
   if (pair && (l0.star != 0 || lanes[1]->star != 1 || lanes[1]->stream != l0.stream ||
@@ -681,20 +684,20 @@ int solve_enqueue(ddx_handle *h, Lane *const *lanes, int count) {
     }
     {
       TimedLaunch tl(h, DDX_K_JACOBI, s);
-      hipLaunchKernelGGL(k_jacobi, dim3(nbj, count), dim3(kJThreads), 0, s, nn, h->nylm, diis_max,
-                         h->facl.p, sys);
+      hipExtLaunchKernelGGL(k_jacobi, dim3(nbj, count), dim3(kJThreads), 0, s, tl.b, tl.e, 0, nn,
+                            h->nylm, diis_max, h->facl.p, sys);
     }
     DDX_HIP(hipGetLastError());
     if (diis_max > 0) {
       TimedLaunch tl(h, DDX_K_DIIS, s);
-      hipLaunchKernelGGL(k_diis_small, dim3(count), dim3(kSmallThreads), 0, s, nbj, diis_max, nbe,
-                         h->n, l0.tol, l0.max_iter, sys);
+      hipExtLaunchKernelGGL(k_diis_small, dim3(count), dim3(kSmallThreads), 0, s, tl.b, tl.e, 0,
+                            nbj, diis_max, nbe, h->n, l0.tol, l0.max_iter, sys);
       DDX_HIP(hipGetLastError());
     }
     {
       TimedLaunch tl(h, DDX_K_EXTRAP, s);
-      hipLaunchKernelGGL(k_extrap, dim3(nbe, count), dim3(kJThreads), 0, s, h->n, h->nylm, nn,
-                         diis_max, invl1, h->cf.p, sys);
+      hipExtLaunchKernelGGL(k_extrap, dim3(nbe, count), dim3(kJThreads), 0, s, tl.b, tl.e, 0, h->n,
+                            h->nylm, nn, diis_max, invl1, h->cf.p, sys);
     }
     DDX_HIP(hipGetLastError());
     if (diis_max == 0) {
@@ -707,17 +710,9 @@ int solve_enqueue(ddx_handle *h, Lane *const *lanes, int count) {
   h->tag_round = 0;
   l0.group = 2;
   for (int k = 0; k < count; ++k)
-    DDX_HIP(hipMemcpyAsync(lanes[k]->h_diis, lanes[k]->diis.p, sizeof(DiisState),
+    DDX_HIP(hipMemcpyAsync(lanes[k]->h_head + 4 * slot, lanes[k]->diis.p, 4 * sizeof(int),
                            hipMemcpyDeviceToHost, s));
-  return DDX_OK;
-}
-
-int solve_poll(ddx_handle *h, Lane *const *lanes, int count, bool *finished) {
-  (void)h;
-  DDX_HIP(hipStreamSynchronize(lanes[0]->stream));
-  bool all = true;
-  for (int k = 0; k < count; ++k) all = all && lanes[k]->h_diis->done;
-  *finished = all || lanes[0]->enqueued >= lanes[0]->rounds;
+  DDX_HIP(hipEventRecord(h->gev[slot], s));
   return DDX_OK;
 }
 
@@ -734,11 +729,25 @@ int solve_end(ddx_handle *h, Lane &ln, int *n_iter, int *ok, std::vector<double>
 int solve_run(ddx_handle *h, Lane *const *lanes, int count) {
   const int serial = h->solve_serial++;
   for (int k = 0; k < count; ++k) lanes[k]->serial = serial;
-  bool finished = false;
-  while (!finished) {
-    DDX_TRY(solve_enqueue(h, lanes, count));
-    DDX_TRY(solve_poll(h, lanes, count, &finished));
+  // one group always runs ahead: while the host waits for the state of
+  // group k, group k + 1 is already queued behind it, so the device does not
+  // wait for the host between groups (a group queued after convergence
+  // returns at once)
+  int cur = 0;
+  DDX_TRY(solve_enqueue(h, lanes, count, cur));
+  while (true) {
+    const bool more = lanes[0]->enqueued < lanes[0]->rounds;
+    if (more) DDX_TRY(solve_enqueue(h, lanes, count, cur ^ 1));
+    DDX_HIP(hipEventSynchronize(h->gev[cur]));
+    bool all = true;
+    for (int k = 0; k < count; ++k) all = all && lanes[k]->h_head[4 * cur + 2] != 0;
+    if (all || !more) break;
+    cur ^= 1;
   }
+  for (int k = 0; k < count; ++k)
+    DDX_HIP(hipMemcpyAsync(lanes[k]->h_diis, lanes[k]->diis.p, sizeof(DiisState),
+                           hipMemcpyDeviceToHost, lanes[k]->stream));
+  DDX_HIP(hipStreamSynchronize(lanes[0]->stream));
   // launches of rounds after a solve's last iteration returned at once:
   // drop them from the timers
   int it[2] = {0, 0}, itmax = 0;

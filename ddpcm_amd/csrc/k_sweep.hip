@@ -328,8 +328,8 @@ int launch_sweep(ddx_handle *h, hipStream_t s, const double *Xs, double *rw, con
                  double *yp, const DiisState *gdir, const DiisState *gadj) {
   if (h->nworker == 0) return DDX_OK;
   TimedLaunch tl(h, (DIR && ADJ) ? DDX_K_SWEEP_PAIR : (DIR ? DDX_K_SWEEP_L : DDX_K_SWEEP_LSTAR), s);
-  hipLaunchKernelGGL((k_sweep<LMAX, DIR, ADJ>), dim3(h->nworker), dim3(kWave), 0, s,
-                     (long long)h->nunit, (long long)h->nworker, h->uoff.p, h->segoff.p, h->inl.p,
+  hipExtLaunchKernelGGL((k_sweep<LMAX, DIR, ADJ>), dim3(h->nworker), dim3(kWave), 0, s, tl.b,
+                        tl.e, 0, (long long)h->nunit, (long long)h->nworker, h->uoff.p, h->segoff.p, h->inl.p,
                      h->pairoff.p, h->n, h->tgeo.p, h->ttw.p, h->tslot.p, h->tp.p, Xs, rw, xiw,
                      h->cf.p, yp, gdir, gadj);
   DDX_HIP(hipGetLastError());
