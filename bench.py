@@ -135,6 +135,34 @@ def cpu_baseline(d, cores, timeout=300):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def copy_calibration(nbytes=1 << 30):
+    """One device-to-device copy of nbytes from a source that was evicted
+    from the caches (a second buffer of the same size is written after it)."""
+    import torch
+    n = nbytes // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    evict = torch.full((n,), 2.0, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "check": float(dst[-1].item() + evict[0].item())}))
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of `kernel` from profiles/<kernel>_*_traffic.md
+    (FETCH_SIZE + WRITE_SIZE counters scaled by the copy calibration), if
+    that summary exists for this workload."""
+    import glob
+    for path in glob.glob(os.path.join(ROOT, "profiles", "%s_*_traffic.md" % kernel)):
+        with open(path) as f:
+            for line in f:
+                m = re.match(r"\s*traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", line)
+                if m:
+                    return float(m.group(1))
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,7 +172,13 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64"])
     ap.add_argument("--cpu-cores", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one plain 1 GiB device copy (the reference point that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py measures one MI355X; this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -188,6 +222,7 @@ def main():
             ach = per_launch / avg_s / 1e9
             roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": ach / HBM_PEAK_GBS, "traffic": None}
+        roofline["traffic"] = traffic_from_profiles(dominant)
         roofline.update({"kernel": dominant, "launches": launches,
                          "avg_launch_us": avg_s * 1e6, "work_per_launch": per_launch,
                          "work_unit": unit})
