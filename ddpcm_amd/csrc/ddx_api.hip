@@ -487,7 +487,7 @@ static int op_apply(ddx_handle *h, int which, const double *x, double *y) {
   DDX_TRY(h->y.reserve(nn));
   DDX_TRY(upload(h->x, x, nn, s));
   if (which == 0) DDX_TRY(apply_lx(h, h->lane[0], h->x.p, h->y.p, nullptr, false, 1));
-  else if (which == 1) DDX_TRY(apply_lstarx(h, h->lane[0], h->x.p, h->y.p, nullptr));
+  else if (which == 1) DDX_TRY(apply_lstarx(h, h->lane[0], h->x.p, h->y.p, nullptr, true));
   else DDX_TRY(device_ldm1x(h, s, h->x.p, h->y.p));
   DDX_TRY(download(y, h->y.p, nn, s));
   return sync(h);

@@ -27,11 +27,12 @@ int expand(ddx_handle *h, hipStream_t s, const double *S, double *xi, bool wscal
 // unless xs_ready says that ln.xs already holds cf x.
 int apply_lx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate,
              bool xs_ready, int ysplit);
-int apply_lstarx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate);
-// yd = L xd (ld.xs holds cf xd) and ya = L* xa in one pass over the
-// triples, gated by the DIIS states of the two lanes
-int apply_pair(ddx_handle *h, Lane &ld, double *yd, Lane &la, const double *xa, double *ya,
-               int ysplit);
+// (want_y = false leaves L* S as per-segment partial sums in ln.yp)
+int apply_lstarx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate,
+                 bool want_y);
+// yd = L xd (ld.xs holds cf xd) and L* xa as per-segment partial sums in
+// la.yp, in one pass over the triples, gated by the DIIS states of the lanes
+int apply_pair(ddx_handle *h, Lane &ld, double *yd, Lane &la, const double *xa, int ysplit);
 // xs = cf x
 int cf_scale(ddx_handle *h, hipStream_t s, const double *x, double *xs);
 

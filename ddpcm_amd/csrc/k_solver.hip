@@ -27,8 +27,10 @@ constexpr int kMaxBlk = 1024;               // workgroups of k_jacobi (partials 
 // one or two systems (the paired direct and adjoint solves), selected by
 // blockIdx.y (blockIdx.x for the single-workgroup kernels).
 struct SysArgs {
-  const double *rhs, *y;     // y: ny partial arrays of nylm nsph, summed here
-  int ny;
+  const double *rhs, *y;     // y: ny partial arrays of nylm nsph, summed here,
+  int ny;                    // or with yp the per-segment sums of L* S:
+  const double *yp;          //   y(lm, c) = sum of yp(lm, s), s in segoff[c] ..
+  const int *segoff;
   double *x, *xs;            // xs = cf x, kept for the L X sweep (null for L*)
   double *xd, *ed, *xnew;    // DIIS history and the Jacobi step
   double *part, *part_e;     // Gram-row and increment-norm partial sums
@@ -48,6 +50,8 @@ __global__ __launch_bounds__(kJThreads) void k_jacobi(size_t nn, int nylm, int d
   const double *__restrict__ rhs = q0.rhs;
   const double *__restrict__ y = q0.y;
   const int ny = q0.ny;
+  const double *__restrict__ yp = q0.yp;
+  const int *__restrict__ segoff = q0.segoff;
   const double *__restrict__ x = q0.x;
   double *__restrict__ xd = q0.xd;
   double *__restrict__ ed = q0.ed;
@@ -69,8 +73,15 @@ __global__ __launch_bounds__(kJThreads) void k_jacobi(size_t nn, int nylm, int d
   double self = 0.0;
   for (size_t k = (size_t)blockIdx.x * kJThreads + threadIdx.x; k < nn;
        k += (size_t)gridDim.x * kJThreads) {
-    double yk = y[k];
-    for (int j = 1; j < ny; ++j) yk += y[(size_t)j * nn + k];
+    double yk;
+    if (yp) {
+      const size_t c = k / nylm, lm = k - c * nylm;
+      yk = 0.0;
+      for (int j = segoff[c]; j < segoff[c + 1]; ++j) yk += yp[(size_t)j * nylm + lm];
+    } else {
+      yk = y[k];
+      for (int j = 1; j < ny; ++j) yk += y[(size_t)j * nn + k];
+    }
     double xn = facl[k % nylm] * (rhs[k] - yk);
     double e = xn - x[k];
     xo[k] = xn;
@@ -573,6 +584,8 @@ SysArgs sys_args(const ddx_handle *h, Lane &ln) {
   a.rhs = ln.rhs;
   a.y = ln.y.p;
   a.ny = ln.star ? 1 : kProjSplit;
+  a.yp = ln.star ? ln.yp.p : nullptr;
+  a.segoff = h->segoff.p;
   a.x = ln.x;
   a.xs = ln.star ? nullptr : ln.xs.p;
   a.xd = ln.xd.p;
@@ -622,6 +635,9 @@ int solve_begin(ddx_handle *h, Lane &ln, int star, int diis_max, double tol, con
     DDX_TRY(ln.ed.reserve(nn * diis_max));
   }
   DDX_TRY(ln.part.reserve((size_t)kMaxBlk * kPartStride + 2048));
+  // the L* sweep leaves per-segment sums that k_jacobi reads: its buffer must
+  // not move once the solve has started
+  if (star) DDX_TRY(ln.yp.reserve((size_t)h->nseg_cap * h->nylm + 1));
   DDX_TRY(ln.diis.reserve(2));
   DDX_TRY(ln.misc.reserve(16));
   ln.star = star;
@@ -676,9 +692,9 @@ int solve_enqueue(ddx_handle *h, Lane *const *lanes, int count, int slot) {
   for (int g = 0; g < l0.group && l0.enqueued < l0.rounds; ++g, ++l0.enqueued) {
     h->tag_round = l0.enqueued;
     if (pair) {
-      DDX_TRY(apply_pair(h, l0, l0.y.p, *lanes[1], lanes[1]->x, lanes[1]->y.p, kProjSplit));
+      DDX_TRY(apply_pair(h, l0, l0.y.p, *lanes[1], lanes[1]->x, kProjSplit));
     } else if (l0.star) {
-      DDX_TRY(apply_lstarx(h, l0, l0.x, l0.y.p, l0.diis.p));
+      DDX_TRY(apply_lstarx(h, l0, l0.x, l0.y.p, l0.diis.p, false));
     } else {
       DDX_TRY(apply_lx(h, l0, l0.x, l0.y.p, l0.diis.p, true, kProjSplit));
     }

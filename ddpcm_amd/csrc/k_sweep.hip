@@ -286,11 +286,11 @@ __global__ __launch_bounds__(kWave) void k_sweep(
   }
 }
 
-// P(n,o) = sum of the point's weighted terms, in neighbour order; zero where
-// ui = 1 (calcv skips those points, ddcosmo.f90:1698).
+// P(n,o) = sum of the point's weighted terms, in neighbour order.  Points
+// with ui = 1, which calcv skips (ddcosmo.f90:1698), are exactly the points
+// without triples (ui < 1 iff some t < 1), so their P is zero here as well.
 __global__ __launch_bounds__(256) void k_gather_points(size_t np, const int *__restrict__ ptoff,
                                                        const double *__restrict__ rw,
-                                                       const double *__restrict__ ui,
                                                        double *__restrict__ P,
                                                        const DiisState *__restrict__ gate) {
   if (gate && gate->done) return;
@@ -298,8 +298,7 @@ __global__ __launch_bounds__(256) void k_gather_points(size_t np, const int *__r
   if (p >= np) return;
   double s = 0.0;
   const int b = ptoff[p], e = ptoff[p + 1];
-  if (b < e && ui[p] < 1.0)
-    for (int k = b; k < e; ++k) s += rw[k];
+  for (int k = b; k < e; ++k) s += rw[k];
   P[p] = s;
 }
 
@@ -374,7 +373,7 @@ int sweep(ddx_handle *h, hipStream_t s, int mode, const double *Xs, double *rw,
 int finish_lx(ddx_handle *h, Lane &ln, double *y, const DiisState *gate, int ysplit) {
   hipStream_t s = ln.stream;
   hipLaunchKernelGGL(k_gather_points, dim3((h->np + 255) / 256), dim3(256), 0, s, h->np,
-                     h->ptoff.p, ln.rw.p, h->ui.p, ln.P.p, gate);
+                     h->ptoff.p, ln.rw.p, ln.P.p, gate);
   DDX_HIP(hipGetLastError());
   // y(lm, o) = - sum_n Yw(lm, n) P(n, o)
   return project(h, s, -1.0, ln.P.p, y, gate, ysplit);
@@ -412,18 +411,18 @@ int seg_sum(ddx_handle *h, hipStream_t s, const double *yp, double *y, const Dii
   return DDX_OK;
 }
 
-int apply_lstarx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate) {
+int apply_lstarx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate,
+                 bool want_y) {
   hipStream_t s = ln.stream;
   DDX_TRY(ln.xiw.reserve(h->np));
   DDX_TRY(ln.yp.reserve((size_t)h->nseg_cap * h->nylm + 1));
   // w_n xi(n, o) = w_n sum_lm Y(lm, n) x(lm, o)
   DDX_TRY(expand(h, s, x, ln.xiw.p, true, gate));
   DDX_TRY(sweep(h, s, 2, nullptr, nullptr, ln.xiw.p, ln.yp.p, nullptr, gate));
-  return seg_sum(h, s, ln.yp.p, y, gate);
+  return want_y ? seg_sum(h, s, ln.yp.p, y, gate) : DDX_OK;
 }
 
-int apply_pair(ddx_handle *h, Lane &ld, double *yd, Lane &la, const double *xa, double *ya,
-               int ysplit) {
+int apply_pair(ddx_handle *h, Lane &ld, double *yd, Lane &la, const double *xa, int ysplit) {
   hipStream_t s = ld.stream;
   DDX_TRY(ld.rw.reserve(h->ntri + 1));
   DDX_TRY(ld.P.reserve(h->np));
@@ -432,7 +431,6 @@ int apply_pair(ddx_handle *h, Lane &ld, double *yd, Lane &la, const double *xa, 
   const DiisState *gd = ld.diis.p, *ga = la.diis.p;
   DDX_TRY(expand(h, s, xa, la.xiw.p, true, ga));
   DDX_TRY(sweep(h, s, 3, ld.xs.p, ld.rw.p, la.xiw.p, la.yp.p, gd, ga));
-  DDX_TRY(seg_sum(h, s, la.yp.p, ya, ga));
   return finish_lx(h, ld, yd, gd, ysplit);
 }
 
