@@ -27,6 +27,7 @@ namespace {
 constexpr int kMaxTiles = 8;   // row tiles of 16 per projection pass
 
 constexpr int kUnroll = 4;   // k-steps whose operands are loaded before their MFMAs
+constexpr int kExpandUnroll = 8;
 
 __global__ __launch_bounds__(256) void k_project(int nylm, int ngrid, int nsph, double alpha,
                                                  const double *__restrict__ Yw,
@@ -113,11 +114,18 @@ __global__ __launch_bounds__(256) void k_expand(int nylm, int ngrid, int nsph,
   const int na = n0 + row;          // A row: grid point
   const int sb = i0 + row;          // B column: sphere
   v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < nylm; k0 += 4) {
-    const int k = k0 + kq;
-    const double a = (na < ngrid && k < nylm) ? Y[k + (size_t)nylm * na] : 0.0;
-    const double b = (sb < nsph && k < nylm) ? S[k + (size_t)nylm * sb] : 0.0;
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  // the operands of kExpandUnroll k-steps are loaded before their MFMAs
+  for (int k0 = 0; k0 < nylm; k0 += 4 * kExpandUnroll) {
+    double a[kExpandUnroll], b[kExpandUnroll];
+#pragma unroll
+    for (int u = 0; u < kExpandUnroll; ++u) {
+      const int k = k0 + 4 * u + kq;
+      a[u] = (na < ngrid && k < nylm) ? Y[k + (size_t)nylm * na] : 0.0;
+      b[u] = (sb < nsph && k < nylm) ? S[k + (size_t)nylm * sb] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kExpandUnroll; ++u)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
   }
   const int sph = i0 + row;
   if (sph >= nsph) return;

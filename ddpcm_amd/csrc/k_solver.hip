@@ -166,17 +166,27 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(int nblk, int diis
   const int off = (nmat >= diis_max) ? 10 : 0;
   const int ndot = nmat - off - 1;
   const bool pending = st->pending_norm != 0;
-  // Gram-row dot products and the increment norm of the previous iteration:
-  // wave w sums the workgroup partials of values w, w + 4, ... (26 = norm)
-  for (int q = wave; q <= kPartStride; q += kSmallThreads / kWave) {
-    const bool need = (q < ndot || q == kDiisMax) || (q == kPartStride && pending);
-    if (!need) continue;   // wave-uniform
-    const double *src = (q == kPartStride) ? part_e : part + (size_t)q * kMaxBlk;
-    const int nb = (q == kPartStride) ? nblk_e : nblk;
+  // Gram-row dot products and the increment norm of the previous iteration
+  // (value 26): eight consecutive lanes share a value, each sums every eighth
+  // workgroup partial, so that all loads are in flight together, and the
+  // eight partial sums meet in three lane-pair stages within the group
+  {
+    constexpr int kTPQ = 8;
+    const int q = tid / kTPQ, r = tid % kTPQ;
     double sum = 0.0;
-    for (int b = lane; b < nb; b += kWave) sum += src[b];
-    sum = wave_sum(sum);
-    if (lane == 0) red[q] = sum;
+    if (q <= kPartStride) {
+      const bool need = (q < ndot || q == kDiisMax) || (q == kPartStride && pending);
+      if (need) {
+        const double *src = (q == kPartStride) ? part_e : part + (size_t)q * kMaxBlk;
+        const int nb = (q == kPartStride) ? nblk_e : nblk;
+#pragma unroll 8
+        for (int b = r; b < nb; b += kTPQ) sum += src[b];
+      }
+    }
+    sum += partner<3>(sum);
+    sum += partner<4>(sum);
+    sum += partner<5>(sum);
+    if (r == 0 && q <= kPartStride) red[q] = sum;
   }
   __syncthreads();
   if (tid == 0) {
@@ -274,10 +284,12 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(int nblk, int diis
   // already pivoted hold the in-place inverse, which neither the later
   // pivots nor the solution read, so only live columns are updated; the
   // arithmetic on every element is the reference's.
-  __shared__ double cand_v[kSmallThreads / kWave];
-  __shared__ int cand_k[kSmallThreads / kWave];
-  __shared__ double dcol[kWave];
-  __shared__ double dpiv;
+  // per step and wave: its best candidate and the whole column it lies in,
+  // double-buffered so that one barrier per step suffices
+  constexpr int kNW = kSmallThreads / kWave;
+  __shared__ double cand_v[2][kNW];
+  __shared__ int cand_k[2][kNW];
+  __shared__ double ccol[2][kNW][kWave];
   constexpr int kCW = (kB + 3) / 4;    // columns per wave
   const int n1 = nmat + 1;
   double g[kCW];
@@ -291,6 +303,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(int nblk, int diis
   unsigned int live = (n1 >= 32) ? 0xffffffffu : ((1u << n1) - 1u);   // columns not pivoted
   bool ok = true;
   for (int it = 0; it < n1; ++it) {
+    const int buf = it & 1;
     // pivot: largest |a(j, k)| > 0 over rows and columns not pivoted, first
     // in row-major order of the current row indices on ties
     const bool rowlive = lane < n1 && ((live >> pos) & 1u);
@@ -311,41 +324,44 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(int nblk, int diis
     pivot_stage<3>(big, best);
     pivot_stage<4>(big, best);
     pivot_stage<5>(big, best);
+    best = __builtin_amdgcn_readfirstlane(best);
+    {
+      // the column of this wave's candidate: the multipliers a(j, icol) if
+      // it wins
+      const int jj = (best == 0x7fffffff) ? 0 : ((best % n1) >> 2);
+      double v = 0.0;
+#pragma unroll
+      for (int j = 0; j < kCW; ++j) v = (j == jj) ? g[j] : v;
+      ccol[buf][wave][lane] = v;
+    }
     if (lane == 0) {
-      cand_v[wave] = big;
-      cand_k[wave] = best;
+      cand_v[buf][wave] = big;
+      cand_k[buf][wave] = best;
     }
     __syncthreads();
-    big = cand_v[0];
-    best = cand_k[0];
+    big = cand_v[buf][0];
+    best = cand_k[buf][0];
+    int win = 0;
 #pragma unroll
-    for (int w = 1; w < kSmallThreads / kWave; ++w) {
-      const double ov = cand_v[w];
-      const int ok2 = cand_k[w];
+    for (int w = 1; w < kNW; ++w) {
+      const double ov = cand_v[buf][w];
+      const int ok2 = cand_k[buf][w];
       const bool t = (ov > big) || (ov == big && ok2 < best);
       big = t ? ov : big;
       best = t ? ok2 : best;
+      win = t ? w : win;
     }
     best = __builtin_amdgcn_readfirstlane(best);
+    win = __builtin_amdgcn_readfirstlane(win);
     if (best == 0x7fffffff) { ok = false; break; }
     const int irow = best / n1, icol = best % n1;
     if (!((live >> icol) & 1u)) { ok = false; break; }
     // the lane holding row irow becomes pivot row icol
     const int lr = __ffsll((unsigned long long)__ballot(lane < n1 && pos == irow)) - 1;
     if (lr < 0) { ok = false; break; }
-    // each row's multiplier a(j, icol), from the wave that holds the column,
-    // read before the pivot row is scaled
-    if (wave == (icol & 3)) {
-      const int jj = icol >> 2;
-      double v = 0.0;
-#pragma unroll
-      for (int j = 0; j < kCW; ++j) v = (j == jj) ? g[j] : v;
-      dcol[lane] = v;
-      if (lane == 0) dpiv = lane_value(v, lr);
-    }
-    __syncthreads();
-    const double dum = dcol[lane];
-    const double d = dpiv;
+    // each row's multiplier a(j, icol), read before the pivot row is scaled
+    const double dum = ccol[buf][win][lane];
+    const double d = ccol[buf][win][lr];
     if (d == 0.0) { ok = false; break; }
     const double pinv = 1.0 / d;
     if (irow != icol) {
