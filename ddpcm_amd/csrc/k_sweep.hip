@@ -181,38 +181,47 @@ __device__ __forceinline__ void sweep_chunk(
   double acc[ADJ ? NY : 1];
 #pragma unroll
   for (int k = 0; k < (ADJ ? NY : 1); ++k) acc[k] = 0.0;
-  // software pipeline: the geometry of the next triple and xi of its owner
-  // point load while the current one is evaluated (tp two triples ahead)
+  // software pipeline, two triples deep: geometry, slot and owner point of
+  // triple k + 128 and xi of triple k + 64 load while triple k is evaluated
   int k = beg + lane;
-  double4 g = make_double4(0.0, 0.0, 0.0, 0.0);
-  double2 tw = make_double2(0.0, 0.0);
-  double xv = 0.0;
-  int tpn = 0;
+  const double4 z4 = make_double4(0.0, 0.0, 0.0, 0.0);
+  const double2 z2 = make_double2(0.0, 0.0);
+  double4 g0 = z4, g1 = z4;
+  double2 w0 = z2, w1 = z2;
+  int s0 = 0, s1 = 0, p1 = 0;
+  double x0 = 0.0;
   if (k < end) {
-    g = tgeo[k];
-    tw = ttw[k];
-    if (ADJ) xv = xiw[tp[k]];
+    g0 = tgeo[k];
+    w0 = ttw[k];
+    if (DIR) s0 = tslot[k];
+    if (ADJ) x0 = xiw[tp[k]];
   }
-  if (ADJ && k + kWave < end) tpn = tp[k + kWave];
+  if (k + kWave < end) {
+    g1 = tgeo[k + kWave];
+    w1 = ttw[k + kWave];
+    if (DIR) s1 = tslot[k + kWave];
+    if (ADJ) p1 = tp[k + kWave];
+  }
   for (; k < end; k += kWave) {
-    const int kn = k + kWave;
-    double4 gn = g;
-    double2 twn = tw;
-    double xvn = 0.0;
-    int tpnn = 0;
-    if (kn < end) {
-      gn = tgeo[kn];
-      twn = ttw[kn];
-      if (ADJ) xvn = xiw[tpn];
+    const int k2 = k + 2 * kWave;
+    double4 g2 = z4;
+    double2 w2 = z2;
+    int s2 = 0, p2 = 0;
+    if (k2 < end) {
+      g2 = tgeo[k2];
+      w2 = ttw[k2];
+      if (DIR) s2 = tslot[k2];
+      if (ADJ) p2 = tp[k2];
     }
-    if (ADJ && kn + kWave < end) tpnn = tp[kn + kWave];
-    const double coef = ADJ ? xv * tw.y : 0.0;
-    const double v = sweep_triple<LMAX, DIR, ADJ>(g.x, g.y, g.z, g.w, tw.x, xsh, coef, acc);
-    if (DIR) rw[tslot[k]] = tw.y * v;
-    g = gn;
-    tw = twn;
-    xv = xvn;
-    tpn = tpnn;
+    double x1 = 0.0;
+    if (ADJ && k + kWave < end) x1 = xiw[p1];
+    const double coef = ADJ ? x0 * w0.y : 0.0;
+    const double v = sweep_triple<LMAX, DIR, ADJ>(g0.x, g0.y, g0.z, g0.w, w0.x, xsh, coef, acc);
+    if (DIR) rw[s0] = w0.y * v;
+    g0 = g1; g1 = g2;
+    w0 = w1; w1 = w2;
+    s0 = s1; s1 = s2;
+    x0 = x1; p1 = p2;
   }
   if (ADJ) {
     // sum of acc over the 64 lanes, column by column, spread over the lanes
@@ -229,8 +238,10 @@ __device__ __forceinline__ void sweep_chunk(
 // floor((w+1) U / G) - 1, container by container.  DIR: L X terms (rw,
 // slot order); ADJ: one partial L* S per segment (yp).  A side whose solve
 // has ended (gate done) is skipped.
+// (two waves per SIMD: the register budget that keeps acc and the
+// two-deep load pipeline without spilling)
 template <int LMAX, bool DIR, bool ADJ>
-__global__ __launch_bounds__(kWave) void k_sweep(
+__global__ __launch_bounds__(kWave, 2) void k_sweep(
     long long U, long long G, const int *__restrict__ uoff, const int *__restrict__ segoff,
     const int *__restrict__ inl, const int *__restrict__ pairoff, int n,
     const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
