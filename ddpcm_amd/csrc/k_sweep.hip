@@ -32,65 +32,114 @@ namespace {
 
 __host__ __device__ constexpr int lmi(int l, int m) { return l * l + l + m; }
 
+// The adjoint accumulators of a lane live in registers.  Where all nylm of
+// them do not fit they are split into blocks of consecutive m, each built by
+// its own pass over the chunk's triples: block b holds the entries (l, +-m)
+// with mblock_begin(b) <= m < mblock_begin(b + 1).  The first pass of the
+// paired sweep also evaluates L X and gets a smaller budget.  The budgets
+// are the largest that the register allocator fits in 256 VGPRs without
+// spilling (its resource report, at two waves per SIMD).
+__host__ __device__ constexpr int acc_cap(int lmax) { return (lmax == 13 || lmax == 14) ? 64 : 72; }
+constexpr int kAccPairLess = 16;   // budget taken by L X in the paired pass
+constexpr int kAccOne = 81;        // up to lmax 8 all entries in one block
+
+// entries (l, +-m), l = m..lmax
+__host__ __device__ constexpr int mcount(int lmax, int m) {
+  return (m == 0 ? 1 : 2) * (lmax + 1 - m);
+}
+// accumulators of m0 <= m < m1
+__host__ __device__ constexpr int mspan(int lmax, int m0, int m1) {
+  int c = 0;
+  for (int m = m0; m < m1; ++m) c += mcount(lmax, m);
+  return c;
+}
+// first m of block b; lmax + 1 once the blocks are used up
+__host__ __device__ constexpr int mblock_begin(int lmax, bool dir, int b) {
+  if ((lmax + 1) * (lmax + 1) <= kAccOne) return b == 0 ? 0 : lmax + 1;
+  int m = 0;
+  for (int k = 0; k < b && m <= lmax; ++k) {
+    const int cap = acc_cap(lmax) - ((k == 0 && dir) ? kAccPairLess : 0);
+    int cnt = 0;
+    while (m <= lmax && cnt + mcount(lmax, m) <= cap) cnt += mcount(lmax, m++);
+  }
+  return m;
+}
+// position of entry (l, m) (neg: (l, -m)) among the accumulators of the
+// block m0 <= m < m1: lm order when the block holds all entries, else m by
+// m, and inside m: l, +m before -m
+__host__ __device__ constexpr int aidx(int lmax, int m0, int m1, int l, int m, bool neg) {
+  if (m0 == 0 && m1 == lmax + 1) return l * l + l + (neg ? -m : m);
+  return mspan(lmax, m0, m) + (m == 0 ? l : 2 * (l - m) + (neg ? 1 : 0));
+}
+
 // One triple: returns sum_lm p~_lm trig_m Xc'_lm when DIR and adds
-// coef p~_lm trig_m to acc_lm when ADJ (trig_m = cos(m phi) for m >= 0,
-// sin(|m| phi) for m < 0).
-template <int LMAX, bool DIR, bool ADJ>
+// coef p~_lm trig_m to the accumulators of AM0 <= |m| < AM1 (trig_m =
+// cos(m phi) for m >= 0, sin(|m| phi) for m < 0).  Without DIR the
+// recurrences in m only step up to AM0 before the l recurrence starts.
+template <int LMAX, bool DIR, int AM0, int AM1>
 __device__ __forceinline__ double sweep_triple(double cthe, double sthe, double cphi, double sphi,
                                                double t, const double *Xc,
                                                double coef, double *acc) {
+  constexpr bool ADJ = AM0 < AM1;
+  constexpr int MB = DIR ? 0 : AM0;          // first m with the l recurrence
+  constexpr int ME = DIR ? LMAX + 1 : AM1;   // one past the last m needed
   const double ct = cthe * t, st = sthe * t, t2 = t * t;
   double s = 0.0;
   double pmm = 1.0;   // t^m P_m^m
   double cm = 1.0, sm = 0.0, cprev = 0.0, sprev = 0.0;
 #pragma unroll
-  for (int m = 0; m <= LMAX; ++m) {
-    double qc = 0.0, qs = 0.0;
-    if (ADJ) {
-      qc = (m == 0) ? coef : coef * cm;
-      qs = (m == 0) ? 0.0 : coef * sm;
-    }
-    double a = 0.0, b = 0.0;
-    // X' of this m is read from LDS only after the previous m's sum: the
-    // compiler would otherwise keep all nylm values in registers
-    const double *Xm = Xc;
-    if (DIR) {
-      int off = 0;
-      asm volatile("" : "+v"(off) : "v"(s));
-      Xm = Xc + off;
-    }
-    auto visit = [&](int l, double p) {
+  for (int m = 0; m < ME; ++m) {
+    if (m >= MB) {
+      const bool acc_m = ADJ && m >= AM0 && m < AM1;
+      double qc = 0.0, qs = 0.0;
+      if (acc_m) {
+        qc = (m == 0) ? coef : coef * cm;
+        qs = (m == 0) ? 0.0 : coef * sm;
+      }
+      double a = 0.0, b = 0.0;
+      // X' of this m is read from LDS only after the previous m's sum: the
+      // compiler would otherwise keep all nylm values in registers
+      const double *Xm = Xc;
       if (DIR) {
-        a += p * Xm[lmi(l, m)];
-        if (m > 0) b += p * Xm[lmi(l, -m)];
+        int off = 0;
+        asm volatile("" : "+v"(off) : "v"(s));
+        Xm = Xc + off;
       }
-      if (ADJ) {
-        acc[lmi(l, m)] += p * qc;
-        if (m > 0) acc[lmi(l, -m)] += p * qs;
-      }
-    };
-    double p0 = pmm;
-    visit(m, p0);
-    if (m < LMAX) {
-      double p1 = ((double)(2 * m + 1) * ct) * p0;
-      visit(m + 1, p1);
+      auto visit = [&](int l, double p) {
+        if (DIR) {
+          a += p * Xm[lmi(l, m)];
+          if (m > 0) b += p * Xm[lmi(l, -m)];
+        }
+        if (acc_m) {
+          acc[aidx(LMAX, AM0, AM1, l, m, false)] += p * qc;
+          if (m > 0) acc[aidx(LMAX, AM0, AM1, l, m, true)] += p * qs;
+        }
+      };
+      double p0 = pmm;
+      visit(m, p0);
+      if (m < LMAX) {
+        double p1 = ((double)(2 * m + 1) * ct) * p0;
+        visit(m + 1, p1);
 #pragma unroll
-      for (int l = m + 2; l <= LMAX; ++l) {
-        constexpr double one = 1.0;
-        const double ca = (double)(2 * l - 1) * (one / (double)(l - m));
-        const double cb = (double)(l + m - 1) * (one / (double)(l - m));
-        const double p2 = ca * (ct * p1) - cb * (t2 * p0);
-        visit(l, p2);
-        p0 = p1;
-        p1 = p2;
+        for (int l = m + 2; l <= LMAX; ++l) {
+          constexpr double one = 1.0;
+          const double ca = (double)(2 * l - 1) * (one / (double)(l - m));
+          const double cb = (double)(l + m - 1) * (one / (double)(l - m));
+          const double p2 = ca * (ct * p1) - cb * (t2 * p0);
+          visit(l, p2);
+          p0 = p1;
+          p1 = p2;
+        }
       }
+      if (DIR) s += (m == 0) ? a : (cm * a + sm * b);
     }
-    if (DIR) s += (m == 0) ? a : (cm * a + sm * b);
-    pmm = (-(double)(2 * m + 1) * st) * pmm;
-    double cn, sn;
-    if (m == 0) { cn = cphi; sn = sphi; }
-    else { cn = 2.0 * cphi * cm - cprev; sn = 2.0 * cphi * sm - sprev; }
-    cprev = cm; sprev = sm; cm = cn; sm = sn;
+    if (m + 1 < ME) {
+      pmm = (-(double)(2 * m + 1) * st) * pmm;
+      double cn, sn;
+      if (m == 0) { cn = cphi; sn = sphi; }
+      else { cn = 2.0 * cphi * cm - cprev; sn = 2.0 * cphi * sm - sprev; }
+      cprev = cm; sprev = sm; cm = cn; sm = sn;
+    }
   }
   return s;
 }
@@ -160,30 +209,24 @@ __device__ __forceinline__ void wave_reduce_split(double *v, int &base, int &len
   if constexpr (S < 5) wave_reduce_split<S + 1, H>(v, base, len);
 }
 
-// Triples beg..end-1 of container c: rw[tslot] = W (R . X_c) when DIR, and
-// when ADJ the chunk's share of (L*S)_c, -cf * sum_t w_n xi(n, o) W p~ trig,
-// into yp.
-template <int LMAX, bool DIR, bool ADJ>
-__device__ __forceinline__ void sweep_chunk(
-    int c, int beg, int end, const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
-    const int *__restrict__ tslot, const int *__restrict__ tp, const double *__restrict__ Xs,
+// One pass over triples beg..end-1 of a container: rw[tslot] = W (R . X_c)
+// when DIR (X' in xsh), and when AM0 < AM1 the (l, +-m), AM0 <= m < AM1,
+// entries of the chunk's share of (L*S)_c, -cf * sum_t w_n xi(n, o) W p~
+// trig, into the segment row yp.
+template <int LMAX, bool DIR, int AM0, int AM1>
+__device__ __forceinline__ void sweep_pass(
+    int beg, int end, const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
+    const int *__restrict__ tslot, const int *__restrict__ tp, const double *xsh,
     double *__restrict__ rw, const double *__restrict__ xiw, const double *__restrict__ cf,
     double *__restrict__ yp) {
-  constexpr int NY = (LMAX + 1) * (LMAX + 1);
-  const int lane = threadIdx.x;
-  // X'_c in LDS: every lane reads the same word (broadcast)
-  __shared__ double xsh[DIR ? NY : 1];
-  if (DIR) {
-    __syncthreads();   // reads of the previous segment's X' are done
-    for (int k = lane; k < NY; k += kWave) xsh[k] = Xs[(size_t)c * NY + k];
-    __syncthreads();
-  }
-  double acc[ADJ ? NY : 1];
+  constexpr bool ADJ = AM0 < AM1;
+  constexpr int NA = ADJ ? mspan(LMAX, AM0, AM1) : 1;
+  double acc[NA];
 #pragma unroll
-  for (int k = 0; k < (ADJ ? NY : 1); ++k) acc[k] = 0.0;
+  for (int k = 0; k < NA; ++k) acc[k] = 0.0;
   // software pipeline, two triples deep: geometry, slot and owner point of
   // triple k + 128 and xi of triple k + 64 load while triple k is evaluated
-  int k = beg + lane;
+  int k = beg + (int)threadIdx.x;
   const double4 z4 = make_double4(0.0, 0.0, 0.0, 0.0);
   const double2 z2 = make_double2(0.0, 0.0);
   double4 g0 = z4, g1 = z4;
@@ -216,30 +259,81 @@ __device__ __forceinline__ void sweep_chunk(
     double x1 = 0.0;
     if (ADJ && k + kWave < end) x1 = xiw[p1];
     const double coef = ADJ ? x0 * w0.y : 0.0;
-    const double v = sweep_triple<LMAX, DIR, ADJ>(g0.x, g0.y, g0.z, g0.w, w0.x, xsh, coef, acc);
+    const double v = sweep_triple<LMAX, DIR, AM0, AM1>(g0.x, g0.y, g0.z, g0.w, w0.x, xsh, coef,
+                                                       acc);
     if (DIR) rw[s0] = w0.y * v;
     g0 = g1; g1 = g2;
     w0 = w1; w1 = w2;
     s0 = s1; s1 = s2;
     x0 = x1; p1 = p2;
   }
-  if (ADJ) {
-    // sum of acc over the 64 lanes, column by column, spread over the lanes
-    int base = 0, len = NY;
-    wave_reduce_split<0, NY>(acc, base, len);
-    constexpr int NF = reduced_len(NY, 6);
+  if constexpr (ADJ) {
+    // sum of acc over the 64 lanes, entry by entry, spread over the lanes
+    int base = 0, len = NA;
+    wave_reduce_split<0, NA>(acc, base, len);
+    constexpr int NF = reduced_len(NA, 6);
 #pragma unroll
-    for (int k = 0; k < NF; ++k)
-      if (k < len) yp[base + k] = -(cf[base + k] * acc[k]);
+    for (int j = 0; j < NF; ++j) {
+      if (j < len) {
+        // lm of block entry q
+        const int q = base + j;
+        int lm = q;
+#pragma unroll
+        for (int m = AM0; m < ((AM0 == 0 && AM1 == LMAX + 1) ? AM0 : AM1); ++m) {
+          const int o = mspan(LMAX, AM0, m);
+          if (q >= o && q < o + mcount(LMAX, m)) {
+            const int r = q - o;
+            const int l = m + (m == 0 ? r : (r >> 1));
+            lm = lmi(l, (m > 0 && (r & 1)) ? -m : m);
+          }
+        }
+        yp[lm] = -(cf[lm] * acc[j]);
+      }
+    }
   }
+}
+
+// The adjoint passes of blocks B.. (the first also evaluates L X when DIR)
+template <int LMAX, bool DIR, int B>
+__device__ __forceinline__ void sweep_blocks(
+    int beg, int end, const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
+    const int *__restrict__ tslot, const int *__restrict__ tp, const double *xsh,
+    double *__restrict__ rw, const double *__restrict__ xiw, const double *__restrict__ cf,
+    double *__restrict__ yp) {
+  constexpr int M0 = mblock_begin(LMAX, DIR, B), M1 = mblock_begin(LMAX, DIR, B + 1);
+  sweep_pass<LMAX, DIR && B == 0, M0, M1>(beg, end, tgeo, ttw, tslot, tp, xsh, rw, xiw, cf, yp);
+  if constexpr (M1 <= LMAX) sweep_blocks<LMAX, DIR, B + 1>(beg, end, tgeo, ttw, tslot, tp, xsh,
+                                                          rw, xiw, cf, yp);
+}
+
+// Triples beg..end-1 of container c: rw[tslot] = W (R . X_c) when DIR, and
+// when ADJ the chunk's share of (L*S)_c into the segment row yp.
+template <int LMAX, bool DIR, bool ADJ>
+__device__ __forceinline__ void sweep_chunk(
+    int c, int beg, int end, const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
+    const int *__restrict__ tslot, const int *__restrict__ tp, const double *__restrict__ Xs,
+    double *__restrict__ rw, const double *__restrict__ xiw, const double *__restrict__ cf,
+    double *__restrict__ yp) {
+  constexpr int NY = (LMAX + 1) * (LMAX + 1);
+  // X'_c in LDS: every lane reads the same word (broadcast)
+  __shared__ double xsh[DIR ? NY : 1];
+  if (DIR) {
+    __syncthreads();   // reads of the previous segment's X' are done
+    for (int k = threadIdx.x; k < NY; k += kWave) xsh[k] = Xs[(size_t)c * NY + k];
+    __syncthreads();
+  }
+  if constexpr (ADJ)
+    sweep_blocks<LMAX, DIR, 0>(beg, end, tgeo, ttw, tslot, tp, xsh, rw, xiw, cf, yp);
+  else
+    sweep_pass<LMAX, true, 0, 0>(beg, end, tgeo, ttw, tslot, tp, xsh, rw, xiw, cf, yp);
 }
 
 // One wave per worker (blockIdx.x of G): units floor(w U / G) ..
 // floor((w+1) U / G) - 1, container by container.  DIR: L X terms (rw,
 // slot order); ADJ: one partial L* S per segment (yp).  A side whose solve
 // has ended (gate done) is skipped.
-// (two waves per SIMD: the register budget that keeps acc and the
-// two-deep load pipeline without spilling)
+// (two waves per SIMD: the register budget that keeps a block of
+// accumulators and the two-deep load pipeline without spilling)
 template <int LMAX, bool DIR, bool ADJ>
 __global__ __launch_bounds__(kWave, 2) void k_sweep(
     long long U, long long G, const int *__restrict__ uoff, const int *__restrict__ segoff,

@@ -105,7 +105,10 @@ def main():
         sys.exit("build the reference first: make -C oracle ref")
     os.makedirs(os.path.join(HERE, "inputs"), exist_ok=True)
     os.makedirs(os.path.join(HERE, "logs"), exist_ok=True)
+    only = sys.argv[1:]   # case names to (re)write; all when empty
     for m in MOLECULES:
+        if only and m not in only:
+            continue
         deck = os.path.join(HERE, "inputs", m + ".txt")
         shutil.copy(os.path.join(REF, "tests", m + ".txt"), deck)
         shutil.copy(os.path.join(REF, "tests", m + ".log"), os.path.join(HERE, "logs", m + ".log"))
@@ -118,8 +121,16 @@ def main():
         "c2": (inputs.config_input("c2"), ["sigma", "s", "fx", "fxdd"]),
         "ion": (inputs.DDInput(0, 1, 6, 302, 10, 1, 78.39, 0.1, np.array([1.0]),
                                np.zeros((1, 3)), np.array([1.5])), FULL),
+        # the large workload's lmax and grid on a small system, and higher lmax
+        "c3_small": (inputs.config_input("c3", n=200, seed=20200), SOLUTION),
+        "l12": (inputs.config_input("c1", n=40, seed=4012, lmax=12, ngrid=590, igrad=1),
+                SOLUTION),
+        "l16": (inputs.config_input("c1", n=12, seed=4016, lmax=16, ngrid=1202, igrad=1),
+                SOLUTION),
     }
     for name, (d, keep) in synth.items():
+        if only and name not in only:
+            continue
         deck = os.path.join(HERE, "inputs", name + ".txt")
         inputs.write_input(deck, d)
         run_case(name, deck, keep)

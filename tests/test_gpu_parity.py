@@ -22,7 +22,10 @@ pytestmark = pytest.mark.gpu
 
 SMALL = ["formaldehyde", "benzene", "caffeine", "c1_forces", "ion"]
 MOLS = ["formaldehyde", "benzene", "caffeine", "taxol", "ala25", "crambin"]
-ALL = MOLS + ["c1", "c1_forces", "c2_small", "ion"]
+# c3_small: lmax 10 and 1202 points (the large workload's settings);
+# l12, l16: lmax 12 on 590 points and lmax 16 on 1202 points
+HIGH = ["c3_small", "l12", "l16"]
+ALL = MOLS + ["c1", "c1_forces", "c2_small", "ion"] + HIGH
 
 
 def make(name, **kw):

@@ -41,7 +41,8 @@ def test_setup_matches_reference_dump(oracle_mod, name):
     assert np.array_equal(o.get("inl") + 1, g["inl"]) and np.array_equal(o.get("nl") + 1, g["nl"])
 
 
-@pytest.mark.parametrize("name", MOLS + ["c1", "c1_forces", "c2_small", "ion"])
+@pytest.mark.parametrize("name", MOLS + ["c1", "c1_forces", "c2_small", "ion",
+                                          "c3_small", "l12", "l16"])
 def test_operators_and_solution_match_reference_dump(oracle_mod, name):
     d = deck(name)
     o = make(oracle_mod, name)
