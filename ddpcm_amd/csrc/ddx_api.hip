@@ -53,6 +53,7 @@ int set_geometry(ddx_handle *h, const double *x, const double *y, const double *
       return DDX_ERR_ARG;
     }
     sph[i] = make_double4(x[i], y[i], z[i], r[i]);
+    h->rmax = i == 0 || r[i] > h->rmax ? r[i] : h->rmax;
   }
   h->h_x.assign(x, x + h->n);
   h->h_y.assign(y, y + h->n);

@@ -146,8 +146,11 @@ struct ddx_handle {
   ddx::DevArray<double4> sph;     // (x, y, z, r)
   ddx::DevArray<double> charge;
 
-  // neighbours
+  // neighbours, and the spatial hash that finds them
   ddx::DevArray<int> inl, nl, rev, prow, degcnt;
+  ddx::DevArray<int> ckey, bcnt, boff, bsph, cand;
+  ddx::DevArray<int> inst;        // instance of each sphere (empty: one instance)
+  double rmax = 0.0;              // largest radius
 
   // per point
   ddx::DevArray<double> fi, ui, zi;

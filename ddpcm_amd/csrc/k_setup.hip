@@ -1,7 +1,8 @@
 // Device setup of one ddCOSMO system (the per-geometry part of ddinit,
 // ddcosmo.f90:308-468), plus the triple lists the sweeps run on.
 //
-//   neighbours  CSR, increasing j, |c_i - c_j| <= r_i + r_j (ddcosmo.f90:308-324)
+//   neighbours  CSR, increasing j, |c_i - c_j| <= r_i + r_j (ddcosmo.f90:308-324),
+//               found through a spatial hash in O(n) instead of the O(n^2) scan
 //   per point   fi = sum_j chi(t_n^ij), ui = max(0, 1 - fi) when fi <= 1,
 //               zi in the switch region (ddcosmo.f90:359-411), plus a 64-bit
 //               mask of the neighbours j with t_n^ij < 1
@@ -23,46 +24,109 @@ namespace {
 
 constexpr int kSetupThreads = 256;
 
-__global__ __launch_bounds__(kSetupThreads) void k_nbr_count(const double4 *__restrict__ sph,
-                                                             int n, int *__restrict__ deg) {
-  __shared__ int wcnt[kSetupThreads / kWave];
-  int i = blockIdx.x;
-  double4 si = sph[i];
-  int cnt = 0;
-  for (int j = threadIdx.x; j < n; j += kSetupThreads) {
-    cnt += (j != i && spheres_overlap(si, sph[j])) ? 1 : 0;
-  }
-  cnt = wave_sum_int(cnt);
-  if (lane_id() == 0) wcnt[threadIdx.x / kWave] = cnt;
-  __syncthreads();
-  if (threadIdx.x == 0) deg[i] = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+// Neighbour search on a spatial hash.  Cubes of edge h >= 2 max r: two
+// spheres that overlap lie in the same or in adjacent cubes.  Cube (ix, iy,
+// iz) of instance k goes to bucket hash(k, ix, iy, iz) mod T; a bucket may
+// hold several cubes, and the exact overlap test sorts them out.
+__device__ __forceinline__ unsigned cell_bucket(long long ix, long long iy, long long iz,
+                                                int inst, unsigned mask) {
+  unsigned long long v = (unsigned long long)ix * 0x9E3779B97F4A7C15ull;
+  v ^= (unsigned long long)iy * 0xC2B2AE3D27D4EB4Full;
+  v ^= (unsigned long long)iz * 0x165667B19E3779F9ull;
+  v ^= (unsigned long long)(unsigned)inst * 0xD6E8FEB86659FD93ull;
+  v ^= v >> 31;
+  return (unsigned)v & mask;
 }
 
-__global__ __launch_bounds__(kSetupThreads) void k_nbr_fill(const double4 *__restrict__ sph,
-                                                            int n, const int *__restrict__ inl,
-                                                            int *__restrict__ nl,
-                                                            int *__restrict__ prow) {
-  __shared__ int wcnt[kSetupThreads / kWave];
-  int i = blockIdx.x;
-  double4 si = sph[i];
-  int run = inl[i];
-  int wave = threadIdx.x / kWave;
-  for (int j0 = 0; j0 < n; j0 += kSetupThreads) {
-    int j = j0 + threadIdx.x;
-    bool hit = false;
-    if (j < n && j != i) hit = spheres_overlap(si, sph[j]);
-    unsigned long long b = __ballot(hit);
-    if (lane_id() == 0) wcnt[wave] = __popcll(b);
-    __syncthreads();
-    int before = 0;
-    for (int w = 0; w < wave; ++w) before += wcnt[w];
-    int pos = run + before + ballot_prefix(b);
-    if (hit && pos < inl[i + 1]) {
-      nl[pos] = j;
-      prow[pos] = i;
+__device__ __forceinline__ long long cube_of(double x, double inv_h) {
+  return (long long)floor(x * inv_h);
+}
+
+__device__ __forceinline__ unsigned sphere_bucket(const double4 &s, int inst, double inv_h,
+                                                  unsigned mask, int d) {
+  return cell_bucket(cube_of(s.x, inv_h) + d % 3 - 1, cube_of(s.y, inv_h) + (d / 3) % 3 - 1,
+                     cube_of(s.z, inv_h) + d / 9 - 1, inst, mask);
+}
+
+// bucket of each sphere's own cube (d = 13: offsets 0, 0, 0) and the bucket sizes
+__global__ void k_cell_count(const double4 *__restrict__ sph, const int *__restrict__ inst,
+                             int n, double inv_h, unsigned mask, int *__restrict__ key,
+                             int *__restrict__ bcnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned b = sphere_bucket(sph[i], inst ? inst[i] : 0, inv_h, mask, 13);
+  key[i] = (int)b;
+  atomicAdd(&bcnt[b], 1);
+}
+
+// spheres bucket by bucket (order inside a bucket is free; bcnt ends at 0)
+__global__ void k_cell_fill(int n, const int *__restrict__ key, const int *__restrict__ boff,
+                            int *__restrict__ bcnt, int *__restrict__ bsph) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = key[i];
+  bsph[boff[b] + atomicSub(&bcnt[b], 1) - 1] = i;
+}
+
+// One wave per sphere i: the spheres j != i of its instance that overlap it,
+// from the buckets of the 27 cubes around it, in increasing j, into
+// cand(kMaxDegree, i); deg[i] = their number (also when above kMaxDegree,
+// which the host turns into an error).
+__global__ __launch_bounds__(kSetupThreads) void k_nbr_find(
+    const double4 *__restrict__ sph, const int *__restrict__ inst, int n, double inv_h,
+    unsigned mask, const int *__restrict__ boff, const int *__restrict__ bsph,
+    int *__restrict__ deg, int *__restrict__ cand) {
+  constexpr int W = kSetupThreads / kWave;
+  __shared__ int list[W][kMaxDegree];
+  const int wv = threadIdx.x / kWave, lane = lane_id();
+  const int i = blockIdx.x * W + wv;
+  int cnt = 0;
+  if (i < n) {
+    const double4 si = sph[i];
+    const int ki = inst ? inst[i] : 0;
+    for (int d = 0; d < 27; ++d) {
+      const unsigned b = sphere_bucket(si, ki, inv_h, mask, d);
+      bool dup = false;   // a bucket that an earlier cube maps to is done
+      for (int e = 0; e < d; ++e) dup |= sphere_bucket(si, ki, inv_h, mask, e) == b;
+      if (dup) continue;
+      const int beg = boff[b], end = boff[b + 1];
+      for (int k0 = beg; k0 < end; k0 += kWave) {
+        const int k = k0 + lane;
+        bool hit = false;
+        int j = 0;
+        if (k < end) {
+          j = bsph[k];
+          hit = j != i && (!inst || inst[j] == ki) && spheres_overlap(si, sph[j]);
+        }
+        const unsigned long long bal = __ballot(hit);
+        const int pos = cnt + ballot_prefix(bal);
+        if (hit && pos < kMaxDegree) list[wv][pos] = j;
+        cnt += __popcll(bal);
+      }
     }
-    run += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
-    __syncthreads();
+  }
+  __syncthreads();
+  if (i < n) {
+    if (lane == 0) deg[i] = cnt;
+    if (cnt <= kMaxDegree && lane < cnt) {
+      const int j = list[wv][lane];
+      int rank = 0;
+      for (int m = 0; m < cnt; ++m) rank += list[wv][m] < j ? 1 : 0;
+      cand[(size_t)i * kMaxDegree + rank] = j;
+    }
+  }
+}
+
+// CSR from the sorted candidates
+__global__ void k_nbr_compact(int n, const int *__restrict__ inl, const int *__restrict__ cand,
+                              int *__restrict__ nl, int *__restrict__ prow) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (int)(t / kMaxDegree), k = (int)(t % kMaxDegree);
+  if (i >= n) return;
+  const int b = inl[i];
+  if (k < inl[i + 1] - b) {
+    nl[b + k] = cand[t];
+    prow[b + k] = i;
   }
 }
 
@@ -246,11 +310,32 @@ int run_setup(ddx_handle *h) {
   size_t np = (size_t)n * ng;
   h->np = np;
 
-  // neighbour list
+  // neighbour list on the spatial hash
+  unsigned T = 64;
+  while (T < 2u * (unsigned)n) T <<= 1;
+  const double inv_h = 1.0 / (2.0 * h->rmax * (1.0 + 1e-12));
+  const int *inst = h->inst.p;
   DDX_TRY(h->degcnt.reserve(n + 1));
   DDX_TRY(h->inl.reserve(n + 1));
-  hipLaunchKernelGGL(k_nbr_count, dim3(n), dim3(kSetupThreads), 0, s, h->sph.p, n, h->degcnt.p);
+  DDX_TRY(h->ckey.reserve(n));
+  DDX_TRY(h->bcnt.reserve(T + 1));
+  DDX_TRY(h->boff.reserve(T + 1));
+  DDX_TRY(h->bsph.reserve(n));
+  DDX_TRY(h->cand.reserve((size_t)n * kMaxDegree));
+  DDX_HIP(hipMemsetAsync(h->bcnt.p, 0, (T + 1) * sizeof(int), s));
+  hipLaunchKernelGGL(k_cell_count, dim3((n + 255) / 256), dim3(256), 0, s, h->sph.p, inst, n,
+                     inv_h, T - 1, h->ckey.p, h->bcnt.p);
   DDX_HIP(hipGetLastError());
+  DDX_TRY(scan_exclusive(h->bcnt.p, h->boff.p, T, h->scan_tmp, s));
+  hipLaunchKernelGGL(k_cell_fill, dim3((n + 255) / 256), dim3(256), 0, s, n, h->ckey.p,
+                     h->boff.p, h->bcnt.p, h->bsph.p);
+  DDX_HIP(hipGetLastError());
+  {
+    const int W = kSetupThreads / kWave;
+    hipLaunchKernelGGL(k_nbr_find, dim3((n + W - 1) / W), dim3(kSetupThreads), 0, s, h->sph.p,
+                       inst, n, inv_h, T - 1, h->boff.p, h->bsph.p, h->degcnt.p, h->cand.p);
+    DDX_HIP(hipGetLastError());
+  }
   DDX_TRY(scan_exclusive(h->degcnt.p, h->inl.p, n, h->scan_tmp, s));
   std::vector<int> deg(n);
   int nnl = 0;
@@ -271,8 +356,9 @@ int run_setup(ddx_handle *h) {
   DDX_TRY(h->pairoff.reserve(nnl + 1));
   DDX_TRY(h->prow.reserve(nnl + 1));
   if (nnl > 0) {
-    hipLaunchKernelGGL(k_nbr_fill, dim3(n), dim3(kSetupThreads), 0, s, h->sph.p, n, h->inl.p,
-                       h->nl.p, h->prow.p);
+    const size_t nt = (size_t)n * kMaxDegree;
+    hipLaunchKernelGGL(k_nbr_compact, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, n,
+                       h->inl.p, h->cand.p, h->nl.p, h->prow.p);
     DDX_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_nbr_rev, dim3((nnl + 255) / 256), dim3(256), 0, s, nnl, h->inl.p,
                        h->nl.p, h->prow.p, h->rev.p);

@@ -131,6 +131,59 @@ def test_c2_full_size():
     np.testing.assert_allclose(fx, g["fx"], rtol=0, atol=1e-7)
 
 
+def _brute_neighbours(x, y, z, r):
+    """CSR with increasing j of |c_i - c_j| <= r_i + r_j, j != i (ddcosmo.f90:308-324), 1-based."""
+    c = np.stack([x, y, z], 1)
+    inl, nl = [1], []
+    for i in range(len(r)):
+        d = c - c[i]
+        d2 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
+        hit = np.nonzero(d2 <= (r[i] + r) ** 2)[0]
+        nl.extend(int(j) + 1 for j in hit if j != i)
+        inl.append(len(nl) + 1)
+    return np.array(inl), np.array(nl)
+
+
+@pytest.mark.parametrize("case", ["spread", "touching", "far_clusters"])
+def test_neighbour_search_matches_brute_force(case):
+    """The spatial-hash search returns exactly the O(n^2) scan's lists."""
+    from ddpcm_amd.ddx import DDCosmo
+    rng = np.random.default_rng({"spread": 1, "touching": 2, "far_clusters": 3}[case])
+    if case == "spread":
+        n = 1500
+        c = rng.uniform(-40, 40, (n, 3))
+        r = rng.uniform(1.2, 4.0, n)
+    elif case == "touching":
+        # a chain of spheres at exactly r_i + r_j (exact binary distances),
+        # every fifth moved 0.5 bohr along the chain
+        n = 400
+        c = np.zeros((n, 3))
+        c[:, 0] = np.arange(n) * 4.0 + (np.arange(n) % 5 == 0) * 0.5
+        r = np.full(n, 2.0)
+    else:
+        # clusters thousands of bohr apart, with one large sphere setting the cube size
+        n = 900
+        centres = rng.uniform(-5e3, 5e3, (9, 3))
+        c = centres[np.arange(n) % 9] + rng.normal(0, 6.0, (n, 3))
+        r = rng.uniform(1.3, 2.6, n)
+        r[5] = 9.0
+    h = DDCosmo(c[:, 0], c[:, 1], c[:, 2], r, lmax=2, ngrid=26, igrad=0)
+    inl, nl = h.neighbours()
+    binl, bnl = _brute_neighbours(c[:, 0], c[:, 1], c[:, 2], r)
+    assert np.array_equal(inl, binl)
+    assert np.array_equal(nl, bnl)
+
+
+def test_neighbour_capacity_error():
+    """More than 64 overlapping spheres is a capacity error, not an overrun."""
+    from ddpcm_amd.ddx import DDCosmo, DDXError
+    rng = np.random.default_rng(5)
+    c = rng.uniform(-1, 1, (70, 3))
+    with pytest.raises(DDXError) as e:
+        DDCosmo(c[:, 0], c[:, 1], c[:, 2], np.full(70, 3.0), lmax=2, ngrid=26, igrad=0)
+    assert e.value.code == 4 and "overlaps" in str(e.value)
+
+
 def test_efld_against_oracle(oracle_mod):
     rng = np.random.default_rng(3)
     d, h = make("caffeine")
