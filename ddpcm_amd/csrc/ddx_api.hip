@@ -77,11 +77,27 @@ int begin_adjoint(ddx_handle *h, Lane &ln, const double *d_psi) {
   return solve_begin(h, ln, 1, kDiisMax, tol_of(h->iconv), d_psi, h->s.p, kMaxIter);
 }
 
-// End of the direct solve; the energy goes to energy[0] on the handle's stream.
+// per-instance iteration counts of the solve that just ended on a lane
+void inst_iterations(const ddx_handle *h, const Lane &ln, int nit, std::vector<int> &out) {
+  out.assign(h->ninst, nit);
+  if (h->ninst > 1)
+    for (int q = 0; q < h->ninst; ++q) out[q] = ln.h_res[2 * q];
+}
+
+// a call that only a plain handle takes
+int single(const ddx_handle *h, const char *what) {
+  if (h->ninst == 1) return DDX_OK;
+  set_error("%s takes a handle of one system; this one holds a batch of %d", what, h->ninst);
+  return DDX_ERR_STATE;
+}
+
+// End of the direct solve; the energies go to energy[0 .. ninst) on the
+// handle's stream.
 int end_direct(ddx_handle *h, Lane &ln, const double *d_psi, int *ok) {
   int nit = 0;
   DDX_TRY(solve_end(h, ln, &nit, ok, &h->hist_direct));
   h->stats.niter_direct = nit;
+  inst_iterations(h, ln, nit, h->iters_direct);
   DDX_TRY(energy_dot(h, h->sigma.p, d_psi, 0.5 * ((h->eps - 1.0) / h->eps), h->energy.p));
   h->have_sigma = true;
   return DDX_OK;
@@ -91,6 +107,7 @@ int end_adjoint(ddx_handle *h, Lane &ln, int *ok) {
   int nit = 0;
   DDX_TRY(solve_end(h, ln, &nit, ok, &h->hist_adjoint));
   h->stats.niter_adjoint = nit;
+  inst_iterations(h, ln, nit, h->iters_adjoint);
   h->have_s = true;
   return DDX_OK;
 }
@@ -127,7 +144,7 @@ int reserve_vectors(ddx_handle *h) {
   DDX_TRY(h->psi.reserve(nn));
   DDX_TRY(h->x.reserve(nn));
   DDX_TRY(h->fx.reserve(3 * (size_t)h->n + 3));
-  DDX_TRY(h->energy.reserve(4));
+  DDX_TRY(h->energy.reserve(h->ninst + 4));
   return DDX_OK;
 }
 
@@ -141,8 +158,8 @@ int pipeline(ddx_handle *h) {
   if (h->igrad) DDX_TRY(h->ef.reserve(field_buffer_size(h)));
   DDX_HIP(hipEventRecord(h->ev[1], s));
   // potential at the cavity points, with forces also the solute field there
-  DDX_TRY(coulomb_potential(h, h->n, h->charge.p, h->sph.p, h->ncav, h->ccav.p, h->phi.p,
-                            h->igrad ? h->ef.p : nullptr));
+  DDX_TRY(coulomb_potential(h, h->ctile.p, h->nctile, h->charge.p, h->sph.p, h->ccav.p,
+                            h->phi.p, h->igrad ? h->ef.p : nullptr));
   DDX_TRY(make_psi(h, h->charge.p, h->psi.p));
   DDX_TRY(build_rhs(h, h->phi.p, h->rhs.p));
   DDX_HIP(hipEventRecord(h->ev[2], s));
@@ -163,11 +180,13 @@ int pipeline(ddx_handle *h) {
     DDX_TRY(end_direct(h, *ln, h->psi.p, &ok_d));
   }
   DDX_HIP(hipEventRecord(h->ev[3], s));
-  if (!ok_d) {
+  // a plain handle stops at a solve that did not converge (the reference
+  // stops, cosmo.f90:154,192); a batch completes the others and reports it
+  if (h->ninst == 1 && !ok_d) {
     DDX_TRY(sync(h));
     return noconv(false);
   }
-  if (!ok_a) {
+  if (h->ninst == 1 && !ok_a) {
     DDX_TRY(sync(h));
     return noconv(true);
   }
@@ -189,6 +208,15 @@ int pipeline(ddx_handle *h) {
   if (h->igrad) {
     DDX_HIP(hipEventElapsedTime(&ms, h->ev[3], h->ev[5]));
     h->stats.ms_forces = ms;
+  }
+  if (!ok_d || !ok_a) {
+    int first = -1;
+    const std::vector<int> &it = !ok_d ? h->iters_direct : h->iters_adjoint;
+    for (int q = 0; q < h->ninst && first < 0; ++q)
+      if (it[q] > kMaxIter) first = q;
+    set_error("%s ddCOSMO did not converge for instance %d of the batch",
+              !ok_d ? "direct" : "adjoint", first);
+    return DDX_ERR_NOCONV;
   }
   return DDX_OK;
 }
@@ -213,10 +241,16 @@ extern "C" {
 
 const char *ddx_last_error(void) { return g_err; }
 
-int ddx_init(ddx_handle **out, int n, const double *x, const double *y, const double *z,
-             const double *rvdw, int iprint, int lmax, int ngrid, int iconv, int igrad,
-             double eps, double eta) {
-  if (!out || n < 1 || !x || !y || !z || !rvdw) {
+}  // extern "C"
+
+namespace ddx {
+namespace {
+// A handle of `count` instances with nsph[q] spheres each (count = 1, nsph
+// = null: a plain handle of n spheres).
+int create(ddx_handle **out, int n, int count, const int *nsph, const double *x,
+           const double *y, const double *z, const double *rvdw, int iprint, int lmax,
+           int ngrid, int iconv, int igrad, double eps, double eta) {
+  if (!out || n < 1 || count < 1 || !x || !y || !z || !rvdw) {
     set_error("ddx_init: invalid arguments");
     return DDX_ERR_ARG;
   }
@@ -230,6 +264,9 @@ int ddx_init(ddx_handle **out, int n, const double *x, const double *y, const do
   }
   ddx_handle *h = new ddx_handle();
   h->n = n;
+  h->ninst = count;
+  h->h_soff.assign(1, 0);
+  for (int q = 0; q < count; ++q) h->h_soff.push_back(h->h_soff.back() + (nsph ? nsph[q] : n));
   h->lmax = lmax;
   h->nylm = (lmax + 1) * (lmax + 1);
   h->ngrid = snap_ngrid(ngrid);
@@ -307,9 +344,50 @@ int ddx_init(ddx_handle **out, int n, const double *x, const double *y, const do
       (rc = upload(h->facs, h->h_facs.data(), nylm, s)) ||
       (rc = upload(h->cf, cf.data(), cf.size(), s)) || (rc = sync(h)))
     return fail(rc);
+  if (count > 1) {
+    std::vector<int> inst(n);
+    for (int q = 0; q < count; ++q)
+      for (int i = h->h_soff[q]; i < h->h_soff[q + 1]; ++i) inst[i] = q;
+    if ((rc = h->inst.reserve(n)) ||
+        (rc = hipMemcpy(h->inst.p, inst.data(), n * sizeof(int), hipMemcpyHostToDevice) ==
+                      hipSuccess ? DDX_OK : DDX_ERR_HIP))
+      return fail(rc);
+  }
   if ((rc = set_geometry(h, x, y, z, rvdw)) || (rc = timed_setup(h))) return fail(rc);
   *out = h;
   return DDX_OK;
+}
+}  // namespace
+}  // namespace ddx
+
+extern "C" {
+
+int ddx_init(ddx_handle **out, int n, const double *x, const double *y, const double *z,
+             const double *rvdw, int iprint, int lmax, int ngrid, int iconv, int igrad,
+             double eps, double eta) {
+  return create(out, n, 1, nullptr, x, y, z, rvdw, iprint, lmax, ngrid, iconv, igrad, eps, eta);
+}
+
+int ddx_batch_init(ddx_handle **out, int count, const int *nsph, const double *x,
+                   const double *y, const double *z, const double *rvdw, int lmax, int ngrid,
+                   int iconv, int igrad, double eps, double eta) {
+  if (!nsph || count < 1) {
+    set_error("ddx_batch_init: invalid arguments");
+    return DDX_ERR_ARG;
+  }
+  long long n = 0;
+  for (int q = 0; q < count; ++q) {
+    if (nsph[q] < 1) {
+      set_error("ddx_batch_init: instance %d has no spheres", q);
+      return DDX_ERR_ARG;
+    }
+    n += nsph[q];
+  }
+  if (n > 0x7fffffffll) {
+    set_error("ddx_batch_init: too many spheres");
+    return DDX_ERR_ARG;
+  }
+  return create(out, (int)n, count, nsph, x, y, z, rvdw, 0, lmax, ngrid, iconv, igrad, eps, eta);
 }
 
 int ddx_free(ddx_handle *h) {
@@ -401,8 +479,8 @@ int ddx_mkrhs(ddx_handle *h, const double *charge, double *phi, double *psi) {
   DDX_TRY(reserve_vectors(h));
   DDX_TRY(upload(h->charge, charge, h->n, h->stream));
   DDX_TRY(h->phi.reserve((size_t)h->ncav + 1));
-  DDX_TRY(coulomb_potential(h, h->n, h->charge.p, h->sph.p, h->ncav, h->ccav.p, h->phi.p,
-                            nullptr));
+  DDX_TRY(coulomb_potential(h, h->ctile.p, h->nctile, h->charge.p, h->sph.p, h->ccav.p,
+                            h->phi.p, nullptr));
   DDX_TRY(make_psi(h, h->charge.p, h->psi.p));
   DDX_TRY(download(phi, h->phi.p, h->ncav, h->stream));
   DDX_TRY(download(psi, h->psi.p, (size_t)h->nylm * h->n, h->stream));
@@ -413,6 +491,7 @@ int ddx_mkrhs(ddx_handle *h, const double *charge, double *phi, double *psi) {
 int ddx_cosmo(ddx_handle *h, int star, int cart, const double *phi, const double *glm,
               const double *psi, double *sigma, double *esolv) {
   if (!h || !psi || !sigma) return DDX_ERR_ARG;
+  DDX_TRY(single(h, "ddx_cosmo"));
   hipStream_t s = h->stream;
   size_t nn = (size_t)h->nylm * h->n;
   DDX_TRY(reserve_vectors(h));
@@ -475,7 +554,7 @@ int ddx_efld(ddx_handle *h, int nsrc, const double *src, const double *csrc, int
   DDX_TRY(upload(dsrc, csrc, 3 * (size_t)nsrc, s));
   DDX_TRY(upload(dtrg, ctrg, 3 * (size_t)ntrg, s));
   DDX_TRY(def.reserve(3 * (size_t)ntrg + 3));
-  DDX_TRY(coulomb_field(h, nsrc, dq.p, dsrc.p, ntrg, dtrg.p, def.p));
+  DDX_TRY(coulomb_field_points(h, nsrc, dq.p, dsrc.p, ntrg, dtrg.p, def.p));
   DDX_TRY(download(ef, def.p, 3 * (size_t)ntrg, s));
   return sync(h);
 }
@@ -509,6 +588,7 @@ int ddx_hnorm(ddx_handle *h, const double *x, double *norm) {
 int ddx_jacobi_diis(ddx_handle *h, int star, int diis_max, double tol, const double *rhs,
                     double *x, int *n_iter, int *ok) {
   if (!h || !rhs || !x || !n_iter || !ok) return DDX_ERR_ARG;
+  DDX_TRY(single(h, "ddx_jacobi_diis"));
   hipStream_t s = h->stream;
   size_t nn = (size_t)h->nylm * h->n;
   DDX_TRY(reserve_vectors(h));
@@ -552,18 +632,51 @@ int ddx_run(ddx_handle *h) {
 
 int ddx_result(ddx_handle *h, double *esolv, double *fx) {
   if (!h) return DDX_ERR_ARG;
-  if (esolv) DDX_TRY(download(esolv, h->energy.p, 1, h->stream));
+  DDX_TRY(single(h, "ddx_result"));
+  return ddx_batch_result(h, esolv, fx);
+}
+
+int ddx_batch_result(ddx_handle *h, double *esolv, double *fx) {
+  if (!h) return DDX_ERR_ARG;
+  if (esolv) DDX_TRY(download(esolv, h->energy.p, h->ninst, h->stream));
   if (fx && h->igrad) DDX_TRY(download(fx, h->fx.p, 3 * (size_t)h->n, h->stream));
   return sync(h);
 }
 
 int ddx_energy_forces(ddx_handle *h, const double *charge, double *esolv, double *fx) {
   if (!h || !charge || !esolv) return DDX_ERR_ARG;
+  DDX_TRY(single(h, "ddx_energy_forces"));
+  return ddx_batch_energy_forces(h, charge, esolv, fx);
+}
+
+int ddx_batch_energy_forces(ddx_handle *h, const double *charge, double *esolv, double *fx) {
+  if (!h || !charge || !esolv) return DDX_ERR_ARG;
   DDX_TRY(upload(h->charge, charge, h->n, h->stream));
   h->have_charge = true;
   if (!h->have_setup) DDX_TRY(timed_setup(h));
-  DDX_TRY(pipeline(h));
-  return ddx_result(h, esolv, fx);
+  const int rc = pipeline(h);
+  if (rc != DDX_OK && !(rc == DDX_ERR_NOCONV && h->ninst > 1)) return rc;
+  DDX_TRY(ddx_batch_result(h, esolv, fx));
+  return rc;
+}
+
+int ddx_batch_iterations(const ddx_handle *h, int *direct, int *adjoint) {
+  if (!h) return DDX_ERR_ARG;
+  for (int q = 0; q < h->ninst; ++q) {
+    if (direct) direct[q] = q < (int)h->iters_direct.size() ? h->iters_direct[q] : 0;
+    if (adjoint) adjoint[q] = q < (int)h->iters_adjoint.size() ? h->iters_adjoint[q] : 0;
+  }
+  return DDX_OK;
+}
+
+int ddx_batch_sizes(const ddx_handle *h, int *count, int *sph_off, int *cav_off) {
+  if (!h) return DDX_ERR_ARG;
+  if (count) *count = h->ninst;
+  for (int q = 0; q <= h->ninst; ++q) {
+    if (sph_off) sph_off[q] = h->h_soff[q];
+    if (cav_off) cav_off[q] = q < (int)h->h_cavoff.size() ? h->h_cavoff[q] : 0;
+  }
+  return DDX_OK;
 }
 
 int ddx_set_precision(ddx_handle *h, int prec) {

@@ -67,7 +67,10 @@ struct DevArray {
   operator T *() const { return p; }
 };
 
-// Device-side Jacobi/DIIS bookkeeping (one per solve in flight).
+// Device-side Jacobi/DIIS bookkeeping, one per instance of a solve in
+// flight.  A batch of several instances has one more state after theirs,
+// the summary: done once every instance is done (the gate of the sweeps and
+// of the host loop), it the largest iteration count, ok the conjunction.
 struct DiisState {
   int nmat;                 // reference's nmat (1-based count, jacobi_diis.f90:89)
   int it;                   // iterations done
@@ -77,6 +80,7 @@ struct DiisState {
   int pending_norm;         // the last k_extrap left increment-norm partials
   int slot[kDiisMax];       // logical history index -> physical slot
   int ncoef;                // coefficients valid in coef[] (0: keep Jacobi step)
+  int remaining;            // batch summary only: instances not yet done
   double coef[kDiisMax];    // extrapolation weights, by logical index
   double b[(kDiisMax + 1) * (kDiisMax + 1)];  // bordered Gram matrix, rows of 25, 0-based
   double norm;              // increment norm of the last iteration
@@ -104,8 +108,10 @@ struct KernelTimer {
 struct Lane {
   hipStream_t stream = nullptr;
   DevArray<double> y, xnew, xs, P, xiw, rw, yp, xd, ed, part;
-  DevArray<DiisState> diis;
-  DevArray<int> misc;
+  DevArray<DiisState> diis;       // ninst states (+ the summary of a batch)
+  DevArray<int> misc;             // per instance: slot of the newest vector
+  DevArray<int> res;              // per instance: iterations, ok (after the solve)
+  std::vector<int> h_res;         // host copy of res (batches)
   DiisState *h_diis = nullptr;  // pinned host mirror of diis
   int *h_head = nullptr;        // pinned: two copies of (nmat, it, done, ok), one per group
   // the solve in flight (solve_begin .. solve_end)
@@ -128,6 +134,11 @@ struct ddx_handle {
   // host copies of tables and geometry
   std::vector<double> h_grid, h_w, h_basis, h_facl, h_facs;
   std::vector<double> h_x, h_y, h_z, h_r;
+
+  // instances: a batch of independent systems, spheres soff[q] .. soff[q+1]-1
+  // of instance q (one instance for a plain handle)
+  int ninst = 1;
+  std::vector<int> h_soff, h_cavoff;   // ninst + 1 sphere / cavity offsets
 
   // sizes produced by setup
   int ncav = 0, nnl = 0, ntri = 0, maxdeg = 0;
@@ -170,6 +181,19 @@ struct ddx_handle {
   ddx::DevArray<int> pttri;       // per point: its triples in neighbour order
   ddx::DevArray<int> tslot;       // per triple: its slot in pttri
 
+  // block maps of the solver kernels: each workgroup of k_jacobi works on
+  // a range of one instance's elements (jblk: instance, begin, end), each of
+  // k_extrap on a range of its spheres (eblk); jboff / eboff: the blocks of
+  // each instance.  Coulomb tiles: per workgroup a range of <= 256 targets
+  // and of sources of one instance (ctile), and for the field at the
+  // nuclei the source chunks of each target tile (ftile, fgrp: tiles of
+  // each target group).
+  int nbj = 0, nbe = 0, nctile = 0, nftile = 0, nfgrp = 0;
+  ddx::DevArray<int4> jblk, eblk, ctile, ftile;
+  ddx::DevArray<int> jboff, eboff, fgrp, soff;
+  ddx::DevArray<int4> xtile;      // tiles of a field between arbitrary point sets
+  ddx::DevArray<int> xgrp;
+
   // solver workspace: lane 0 runs on `stream`, lane 1 on a stream of its own
   ddx::Lane lane[2];
   ddx::DevArray<double> x, y, rhs, r, part;
@@ -182,6 +206,7 @@ struct ddx_handle {
   ddx::DevArray<int> scan_tmp, misc;
 
   std::vector<double> hist_direct, hist_adjoint;
+  std::vector<int> iters_direct, iters_adjoint;   // per instance
   ddx_stats stats{};
   ddx::KernelTimer ktimer[DDX_K_COUNT];
 };

@@ -42,6 +42,12 @@ int cf_scale(ddx_handle *h, hipStream_t s, const double *x, double *xs);
 // lockstep on one stream, enqueueing groups of iterations and reading the
 // DIIS states back between groups; solve_end reports each.
 // solve_jacobi_diis runs the three for one solve.
+// the state that gates a lane's kernels: the instance's own for a plain
+// handle, the summary of a batch
+inline const DiisState *solve_gate(const ddx_handle *h, const Lane &ln) {
+  return ln.diis.p + (h->ninst > 1 ? h->ninst : 0);
+}
+int build_solver_maps(ddx_handle *h);
 int solve_begin(ddx_handle *h, Lane &ln, int star, int diis_max, double tol,
                 const double *d_rhs, double *d_x, int max_iter);
 int solve_run(ddx_handle *h, Lane *const *lanes, int count);
@@ -52,15 +58,25 @@ int solve_jacobi_diis(ddx_handle *h, Lane &ln, int star, int diis_max, double to
 int device_ldm1x(ddx_handle *h, hipStream_t s, const double *x, double *y);
 int device_hnorm(ddx_handle *h, const double *x, double *norm_host);
 int build_rhs(ddx_handle *h, const double *d_phi, double *d_rhs);
+// d_out[q] = scale <a, b> over the elements of instance q
 int energy_dot(ddx_handle *h, const double *a, const double *b, double scale,
                double *d_out);
 
-// k_coulomb.hip — all-pairs potential and field (mkrhs.f90, efld.f90)
-// phi(ntrg); with ef != NULL also the field ef(3, ntrg) of the same sources
-int coulomb_potential(ddx_handle *h, int nsrc, const double *q, const double4 *src,
-                      int ntrg, const double *trg3, double *phi, double *ef);
-int coulomb_field(ddx_handle *h, int nsrc, const double *q, const double *src3,
-                  int ntrg, const double *trg3, double *ef);
+// k_coulomb.hip — all-pairs potential and field (mkrhs.f90, efld.f90) on
+// tile tables (targets tb..te-1, sources sb..se-1 per workgroup)
+void coulomb_tiles(const std::vector<int> &toff, const std::vector<int> &soff, bool split,
+                   std::vector<int4> &tiles, std::vector<int> &grp);
+// h->ctile (cavity points from the spheres) and h->ftile / h->fgrp (the
+// nuclei from the cavity points), per instance
+int build_coulomb_tiles(ddx_handle *h);
+// phi; with ef != NULL also the field ef(3, .) of the same sources
+int coulomb_potential(ddx_handle *h, const int4 *tiles, int ntile, const double *q,
+                      const double4 *src, const double *trg3, double *phi, double *ef);
+// ef(3, .) from per-chunk partials summed per target group
+int coulomb_field(ddx_handle *h, const int4 *tiles, int ntile, const int *grp, int ngrp,
+                  const double *q, const double *src3, const double *trg3, double *ef);
+int coulomb_field_points(ddx_handle *h, int nsrc, const double *q, const double *src3, int ntrg,
+                         const double *trg3, double *ef);
 int make_psi(ddx_handle *h, const double *charge, double *psi);
 
 // k_forces.hip — forces_dd, ddmkzeta and the field contractions

@@ -2,6 +2,9 @@
 //   potential  phi(j) = sum_i q_i / |r_j - r_i|            (mkrhs.f90:102-114)
 //   field      E(j)   = sum_i q_i (r_j - r_i) / |r_j - r_i|^3  (efld.f90:98-114)
 // Each thread owns one target; sources stream through LDS in tiles of 256.
+// Work comes as a table of tiles (targets tb..te-1, at most 256, and sources
+// sb..se-1): for a batch of molecules every tile stays inside one
+// molecule, so each sums only its own molecule's charges.
 // The potential kernel can return the field of the same sources as well
 // (the solute field at the cavity points that the forces contract with
 // zeta), which costs one all-pairs pass less than a separate field kernel.
@@ -9,10 +12,13 @@
 // third-order step (relative error a few ulp), which replaces an IEEE sqrt
 // and divide per pair.
 // When there are few targets (the field of the cavity charges at the nuclei)
-// the sources are split over several workgroups and the partial fields are
-// summed afterwards in a fixed order, so results are deterministic.
+// the sources of a target tile are split into chunks, one tile each, and
+// the partial fields are summed afterwards in chunk order, so results are
+// deterministic.
 #include "ddx_device.hpp"
 #include "ddx_kernels.hpp"
+
+#include <algorithm>
 
 namespace ddx {
 namespace {
@@ -29,30 +35,32 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
 }
 
 // phi(j) and, with FIELD, the field ef(:, j) of the same sources, from one
-// evaluation of 1/r per pair
+// evaluation of 1/r per pair; tile (tb, te, sb, se) per workgroup
 template <bool FIELD>
-__global__ __launch_bounds__(kCT) void k_potential(int nsrc, const double *__restrict__ q,
-                                                   const double4 *__restrict__ src, int ntrg,
+__global__ __launch_bounds__(kCT) void k_potential(const int4 *__restrict__ tiles,
+                                                   const double *__restrict__ q,
+                                                   const double4 *__restrict__ src,
                                                    const double *__restrict__ trg,
                                                    double *__restrict__ phi,
                                                    double *__restrict__ ef) {
   __shared__ double4 tile[kCT];
-  int j = blockIdx.x * kCT + threadIdx.x;
+  const int4 tl = tiles[blockIdx.x];
+  const int j = tl.x + threadIdx.x;
   double tx = 0.0, ty = 0.0, tz = 0.0;
-  if (j < ntrg) {
+  if (j < tl.y) {
     tx = trg[3 * (size_t)j];
     ty = trg[3 * (size_t)j + 1];
     tz = trg[3 * (size_t)j + 2];
   }
   double v = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0;
-  for (int i0 = 0; i0 < nsrc; i0 += kCT) {
+  for (int i0 = tl.z; i0 < tl.w; i0 += kCT) {
     int i = i0 + threadIdx.x;
-    if (i < nsrc) {
+    if (i < tl.w) {
       double4 s = src[i];
       tile[threadIdx.x] = make_double4(s.x, s.y, s.z, q[i]);
     }
     __syncthreads();
-    int cnt = min(kCT, nsrc - i0);
+    int cnt = min(kCT, tl.w - i0);
     for (int k = 0; k < cnt; ++k) {
       double4 s = tile[k];
       double dx = tx - s.x, dy = ty - s.y, dz = tz - s.z;
@@ -69,7 +77,7 @@ __global__ __launch_bounds__(kCT) void k_potential(int nsrc, const double *__res
     }
     __syncthreads();
   }
-  if (j < ntrg) {
+  if (j < tl.y) {
     phi[j] = v;
     if (FIELD) {
       ef[3 * (size_t)j] = e0;
@@ -79,29 +87,29 @@ __global__ __launch_bounds__(kCT) void k_potential(int nsrc, const double *__res
   }
 }
 
-// partial fields of sources [split range) at targets; out[(split * ntrg + j) * 3]
-__global__ __launch_bounds__(kCT) void k_field(int nsrc, const double *__restrict__ q,
-                                               const double *__restrict__ src, int ntrg,
-                                               const double *__restrict__ trg, int chunk,
+// partial fields of tile (tb, te, sb, se) at its targets: out[(b kCT + t) 3]
+__global__ __launch_bounds__(kCT) void k_field(const int4 *__restrict__ tiles,
+                                               const double *__restrict__ q,
+                                               const double *__restrict__ src,
+                                               const double *__restrict__ trg,
                                                double *__restrict__ out) {
   __shared__ double4 tile[kCT];
-  int j = blockIdx.x * kCT + threadIdx.x;
-  int sbeg = blockIdx.y * chunk;
-  int send = min(nsrc, sbeg + chunk);
+  const int4 tl = tiles[blockIdx.x];
+  const int j = tl.x + threadIdx.x;
   double tx = 0.0, ty = 0.0, tz = 0.0;
-  if (j < ntrg) {
+  if (j < tl.y) {
     tx = trg[3 * (size_t)j];
     ty = trg[3 * (size_t)j + 1];
     tz = trg[3 * (size_t)j + 2];
   }
   double e0 = 0.0, e1 = 0.0, e2 = 0.0;
-  for (int i0 = sbeg; i0 < send; i0 += kCT) {
+  for (int i0 = tl.z; i0 < tl.w; i0 += kCT) {
     int i = i0 + threadIdx.x;
-    if (i < send)
+    if (i < tl.w)
       tile[threadIdx.x] = make_double4(src[3 * (size_t)i], src[3 * (size_t)i + 1],
                                        src[3 * (size_t)i + 2], q[i]);
     __syncthreads();
-    int cnt = min(kCT, send - i0);
+    int cnt = min(kCT, tl.w - i0);
     for (int k = 0; k < cnt; ++k) {
       double4 s = tile[k];
       double dx = tx - s.x, dy = ty - s.y, dz = tz - s.z;
@@ -114,21 +122,32 @@ __global__ __launch_bounds__(kCT) void k_field(int nsrc, const double *__restric
     }
     __syncthreads();
   }
-  if (j < ntrg) {
-    size_t o = ((size_t)blockIdx.y * ntrg + j) * 3;
-    out[o] = e0;
-    out[o + 1] = e1;
-    out[o + 2] = e2;
-  }
+  const size_t o = ((size_t)blockIdx.x * kCT + threadIdx.x) * 3;
+  out[o] = e0;
+  out[o + 1] = e1;
+  out[o + 2] = e2;
 }
 
-__global__ void k_field_reduce(int nsplit, int ntrg, const double *__restrict__ part,
-                               double *__restrict__ ef) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= 3 * ntrg) return;
-  double s = 0.0;
-  for (int b = 0; b < nsplit; ++b) s += part[(size_t)b * 3 * ntrg + k];
-  ef[k] = s;
+// ef of the targets of group g = sum of the partials of its tiles
+// grp[g] .. grp[g+1]-1 (chunks of one target tile), in chunk order
+__global__ __launch_bounds__(kCT) void k_field_reduce(const int *__restrict__ grp,
+                                                      const int4 *__restrict__ tiles,
+                                                      const double *__restrict__ part,
+                                                      double *__restrict__ ef) {
+  const int b0 = grp[blockIdx.x], b1 = grp[blockIdx.x + 1];
+  const int4 tl = tiles[b0];
+  const int j = tl.x + threadIdx.x;
+  if (j >= tl.y) return;
+  double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+  for (int b = b0; b < b1; ++b) {
+    const size_t o = ((size_t)b * kCT + threadIdx.x) * 3;
+    e0 += part[o];
+    e1 += part[o + 1];
+    e2 += part[o + 2];
+  }
+  ef[3 * (size_t)j] = e0;
+  ef[3 * (size_t)j + 1] = e1;
+  ef[3 * (size_t)j + 2] = e2;
 }
 
 __global__ void k_psi(int n, int nylm, double fac, const double *__restrict__ q,
@@ -140,46 +159,113 @@ __global__ void k_psi(int n, int nylm, double fac, const double *__restrict__ q,
 
 }  // namespace
 
-int coulomb_potential(ddx_handle *h, int nsrc, const double *q, const double4 *src, int ntrg,
-                      const double *trg3, double *phi, double *ef) {
-  if (ntrg <= 0) return DDX_OK;
+// Tiles of kCT targets, per instance q: targets toff[q] .. toff[q+1]-1,
+// sources soff[q] .. soff[q+1]-1.  With `split`, the sources of each target
+// tile are cut into chunks (multiples of kCT, at least 4 kCT) so that there
+// are about 1024 tiles in all; grp then holds the first tile of each target
+// tile's chunks.
+void coulomb_tiles(const std::vector<int> &toff, const std::vector<int> &soff, bool split,
+                   std::vector<int4> &tiles, std::vector<int> &grp) {
+  const size_t ninst = toff.size() - 1;
+  long long work = 0;   // target tiles x sources
+  for (size_t q = 0; q < ninst; ++q)
+    work += (long long)((toff[q + 1] - toff[q] + kCT - 1) / kCT) * (soff[q + 1] - soff[q]);
+  long long chunk = 0x7fffffff;
+  if (split) {
+    chunk = (work + 1023) / 1024;
+    chunk = ((chunk + kCT - 1) / kCT) * kCT;
+    if (chunk < 4 * kCT) chunk = 4 * kCT;
+  }
+  tiles.clear();
+  grp.clear();
+  for (size_t q = 0; q < ninst; ++q) {
+    const int sb = soff[q], se = soff[q + 1];
+    for (int tb = toff[q]; tb < toff[q + 1]; tb += kCT) {
+      const int te = std::min(tb + kCT, toff[q + 1]);
+      grp.push_back((int)tiles.size());
+      if (se <= sb) {
+        tiles.push_back(make_int4(tb, te, sb, sb));
+        continue;
+      }
+      for (long long c = sb; c < se; c += chunk)
+        tiles.push_back(make_int4(tb, te, (int)c, (int)std::min<long long>(c + chunk, se)));
+    }
+  }
+  grp.push_back((int)tiles.size());
+}
+
+namespace {
+int upload_tiles(ddx_handle *h, DevArray<int4> &dt, const std::vector<int4> &t) {
+  DDX_TRY(dt.reserve(t.size() + 1));
+  if (!t.empty())
+    DDX_HIP(hipMemcpyAsync(dt.p, t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice,
+                           h->stream));
+  return DDX_OK;
+}
+}  // namespace
+
+int build_coulomb_tiles(ddx_handle *h) {
+  std::vector<int4> t;
+  std::vector<int> g;
+  // potential (and solute field) at the cavity points from the spheres
+  coulomb_tiles(h->h_cavoff, h->h_soff, false, t, g);
+  h->nctile = (int)t.size();
+  DDX_TRY(upload_tiles(h, h->ctile, t));
+  // field of the cavity charges at the nuclei
+  coulomb_tiles(h->h_soff, h->h_cavoff, true, t, g);
+  h->nftile = (int)t.size();
+  h->nfgrp = (int)g.size() - 1;
+  DDX_TRY(upload_tiles(h, h->ftile, t));
+  DDX_TRY(h->fgrp.reserve(g.size()));
+  DDX_HIP(hipMemcpyAsync(h->fgrp.p, g.data(), g.size() * sizeof(int), hipMemcpyHostToDevice,
+                         h->stream));
+  DDX_HIP(hipStreamSynchronize(h->stream));
+  return DDX_OK;
+}
+
+int coulomb_potential(ddx_handle *h, const int4 *tiles, int ntile, const double *q,
+                      const double4 *src, const double *trg3, double *phi, double *ef) {
+  if (ntile <= 0) return DDX_OK;
   TimedLaunch tl(h, DDX_K_POTENTIAL, h->stream);
   if (ef)
-    hipExtLaunchKernelGGL(k_potential<true>, dim3((ntrg + kCT - 1) / kCT), dim3(kCT), 0,
-                          h->stream, tl.b, tl.e, 0, nsrc, q, src, ntrg, trg3, phi, ef);
+    hipExtLaunchKernelGGL(k_potential<true>, dim3(ntile), dim3(kCT), 0, h->stream, tl.b, tl.e, 0,
+                          tiles, q, src, trg3, phi, ef);
   else
-    hipExtLaunchKernelGGL(k_potential<false>, dim3((ntrg + kCT - 1) / kCT), dim3(kCT), 0,
-                          h->stream, tl.b, tl.e, 0, nsrc, q, src, ntrg, trg3, phi, nullptr);
+    hipExtLaunchKernelGGL(k_potential<false>, dim3(ntile), dim3(kCT), 0, h->stream, tl.b, tl.e,
+                          0, tiles, q, src, trg3, phi, nullptr);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
 
-int coulomb_field(ddx_handle *h, int nsrc, const double *q, const double *src3, int ntrg,
-                  const double *trg3, double *ef) {
-  if (ntrg <= 0) return DDX_OK;
-  int tb = (ntrg + kCT - 1) / kCT;
-  int nsplit = 1;
-  if (tb < 1024 && nsrc > 4 * kCT) {
-    nsplit = (1024 + tb - 1) / tb;
-    int maxsplit = (nsrc + 4 * kCT - 1) / (4 * kCT);
-    if (nsplit > maxsplit) nsplit = maxsplit;
-    if (nsplit < 1) nsplit = 1;
-  }
-  int chunk = (nsrc + nsplit - 1) / nsplit;
-  chunk = ((chunk + kCT - 1) / kCT) * kCT;
-  nsplit = chunk > 0 ? (nsrc + chunk - 1) / chunk : 1;
-  if (nsplit < 1) nsplit = 1;
-  DDX_TRY(h->fpart.reserve((size_t)nsplit * 3 * ntrg));
+int coulomb_field(ddx_handle *h, const int4 *tiles, int ntile, const int *grp, int ngrp,
+                  const double *q, const double *src3, const double *trg3, double *ef) {
+  if (ntile <= 0 || ngrp <= 0) return DDX_OK;
+  DDX_TRY(h->fpart.reserve((size_t)ntile * kCT * 3));
   {
     TimedLaunch tl(h, DDX_K_FIELD, h->stream);
-    hipExtLaunchKernelGGL(k_field, dim3(tb, nsplit), dim3(kCT), 0, h->stream, tl.b, tl.e, 0, nsrc,
-                          q, src3, ntrg, trg3, chunk > 0 ? chunk : kCT, h->fpart.p);
+    hipExtLaunchKernelGGL(k_field, dim3(ntile), dim3(kCT), 0, h->stream, tl.b, tl.e, 0, tiles, q,
+                          src3, trg3, h->fpart.p);
   }
   DDX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_field_reduce, dim3((3 * ntrg + 255) / 256), dim3(256), 0, h->stream, nsplit,
-                     ntrg, h->fpart.p, ef);
+  hipLaunchKernelGGL(k_field_reduce, dim3(ngrp), dim3(kCT), 0, h->stream, grp, tiles, h->fpart.p,
+                     ef);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
+}
+
+// the field of nsrc arbitrary charges at ntrg arbitrary points (efld.f90:87-116)
+int coulomb_field_points(ddx_handle *h, int nsrc, const double *q, const double *src3, int ntrg,
+                         const double *trg3, double *ef) {
+  if (ntrg <= 0) return DDX_OK;
+  std::vector<int4> t;
+  std::vector<int> g;
+  coulomb_tiles({0, ntrg}, {0, nsrc}, true, t, g);
+  DDX_TRY(upload_tiles(h, h->xtile, t));
+  DDX_TRY(h->xgrp.reserve(g.size()));
+  DDX_HIP(hipMemcpyAsync(h->xgrp.p, g.data(), g.size() * sizeof(int), hipMemcpyHostToDevice,
+                         h->stream));
+  return coulomb_field(h, h->xtile.p, (int)t.size(), h->xgrp.p, (int)g.size() - 1, q, src3, trg3,
+                       ef);
 }
 
 int make_psi(ddx_handle *h, const double *charge, double *psi) {

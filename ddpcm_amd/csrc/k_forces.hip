@@ -395,7 +395,7 @@ int make_zeta(ddx_handle *h, const double *d_s, double *d_zeta) {
 }
 
 size_t field_buffer_size(const ddx_handle *h) {
-  return 3 * ((size_t)h->ncav + h->n) + 6 + 3 * (size_t)h->n;
+  return 3 * ((size_t)h->ncav + h->n) + 6 + 3 * (size_t)h->n + (size_t)h->ncav + 1;
 }
 
 int field_forces(ddx_handle *h, const double *d_charge, const double *d_zeta, double *d_fx,
@@ -410,8 +410,12 @@ int field_forces(ddx_handle *h, const double *d_charge, const double *d_zeta, do
   DDX_HIP(hipGetLastError());
   // solute field at the cavity points (unless the potential kernel made
   // it), and the field of zeta at the nuclei
-  if (!solute_field) DDX_TRY(coulomb_field(h, n, d_charge, c3, h->ncav, h->ccav.p, ef));
-  DDX_TRY(coulomb_field(h, h->ncav, d_zeta, h->ccav.p, n, c3, ez));
+  if (!solute_field) {
+    double *phi_scratch = c3 + 3 * (size_t)n;
+    DDX_TRY(coulomb_potential(h, h->ctile.p, h->nctile, d_charge, h->sph.p, h->ccav.p,
+                              phi_scratch, ef));
+  }
+  DDX_TRY(coulomb_field(h, h->ftile.p, h->nftile, h->fgrp.p, h->nfgrp, d_zeta, h->ccav.p, c3, ez));
   hipLaunchKernelGGL(k_field_contract, dim3(n), dim3(256), 0, s, h->ngrid, h->cavidx.p, d_zeta, ef,
                      d_charge, ez, d_fx);
   DDX_HIP(hipGetLastError());

@@ -302,6 +302,13 @@ __global__ void k_seg_count(int n, const int *__restrict__ uoff, long long U, lo
   segcnt[c] = cnt;
 }
 
+// cavity offset of each instance: the cavity index of its first point
+__global__ void k_cav_offsets(int ninst, int ngrid, const int *__restrict__ soff,
+                              const int *__restrict__ cavidx, int *__restrict__ out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q <= ninst) out[q] = cavidx[(size_t)soff[q] * ngrid];
+}
+
 }  // namespace
 
 int run_setup(ddx_handle *h) {
@@ -437,6 +444,20 @@ int run_setup(ddx_handle *h) {
   hipLaunchKernelGGL(k_ccav, pg, dim3(kSetupThreads), 0, s, h->sph.p, h->grid.p, ng,
                      h->cavflag.p, h->cavidx.p, h->ccav.p);
   DDX_HIP(hipGetLastError());
+  // instance offsets of the cavity, the solver's block maps, Coulomb tiles
+  DDX_TRY(h->soff.reserve(h->ninst + 1));
+  DDX_TRY(h->cand.reserve((size_t)h->ninst + 1));
+  DDX_HIP(hipMemcpyAsync(h->soff.p, h->h_soff.data(), (h->ninst + 1) * sizeof(int),
+                         hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_cav_offsets, dim3((h->ninst + 256) / 256), dim3(256), 0, s, h->ninst, ng,
+                     h->soff.p, h->cavidx.p, h->cand.p);
+  DDX_HIP(hipGetLastError());
+  h->h_cavoff.resize(h->ninst + 1);
+  DDX_HIP(hipMemcpyAsync(h->h_cavoff.data(), h->cand.p, (h->ninst + 1) * sizeof(int),
+                         hipMemcpyDeviceToHost, s));
+  DDX_HIP(hipStreamSynchronize(s));
+  DDX_TRY(build_solver_maps(h));
+  DDX_TRY(build_coulomb_tiles(h));
   h->have_setup = true;
   h->stats.ntriples = ntri;
   h->stats.nnl = nnl;

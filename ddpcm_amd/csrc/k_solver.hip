@@ -13,6 +13,13 @@
 // k_norm.  Their bookkeeping lives in a device DiisState; once it says
 // done every later kernel returns at once, so the host can enqueue several
 // iterations before it looks at the state.
+//
+// A handle may hold a batch of independent instances (molecules): each has
+// its own DiisState and converges on its own, exactly as if solved alone.
+// Every workgroup of k_jacobi and k_extrap works inside one instance (the
+// block maps jblk, eblk), so that its partial sums belong to that instance;
+// k_diis_small runs one workgroup per instance.  A plain handle is a batch
+// of one.
 #include "ddx_device.hpp"
 #include "ddx_kernels.hpp"
 
@@ -21,7 +28,6 @@ namespace {
 
 constexpr int kJThreads = 256;
 constexpr int kPartStride = kDiisMax + 1;   // 25 dots + self dot per block
-constexpr int kMaxBlk = 1024;               // workgroups of k_jacobi (partials per dot)
 
 // Pointers of one system's Jacobi/DIIS iteration; the kernels below take
 // one or two systems (the paired direct and adjoint solves), selected by
@@ -33,19 +39,21 @@ struct SysArgs {
   const int *segoff;
   double *x, *xs;            // xs = cf x, kept for the L X sweep (null for L*)
   double *xd, *ed, *xnew;    // DIIS history and the Jacobi step
-  double *part, *part_e;     // Gram-row and increment-norm partial sums
-  int *newslot;
-  DiisState *st;
+  double *part, *part_e;     // Gram-row (26 rows of nbj) and increment-norm partial sums
+  int *newslot;              // per instance
+  DiisState *st;             // per instance, then the summary of a batch
 };
 struct Systems {
   SysArgs s[2];
 };
 
 __global__ __launch_bounds__(kJThreads) void k_jacobi(size_t nn, int nylm, int diis_max,
+                                                      const int4 *__restrict__ jblk, int nbj,
                                                       const double *__restrict__ facl,
                                                       Systems sys) {
   const SysArgs &q0 = sys.s[blockIdx.y];
-  const DiisState *__restrict__ st = q0.st;
+  const int4 blk = jblk[blockIdx.x];   // instance, elements blk.y .. blk.z - 1
+  const DiisState *__restrict__ st = q0.st + blk.x;
   if (st->done) return;
   const double *__restrict__ rhs = q0.rhs;
   const double *__restrict__ y = q0.y;
@@ -71,8 +79,7 @@ __global__ __launch_bounds__(kJThreads) void k_jacobi(size_t nn, int nylm, int d
 #pragma unroll
   for (int q = 0; q < kDiisMax; ++q) acc[q] = 0.0;
   double self = 0.0;
-  for (size_t k = (size_t)blockIdx.x * kJThreads + threadIdx.x; k < nn;
-       k += (size_t)gridDim.x * kJThreads) {
+  for (size_t k = (size_t)blk.y + threadIdx.x; k < (size_t)blk.z; k += kJThreads) {
     double yk;
     if (yp) {
       const size_t c = k / nylm, lm = k - c * nylm;
@@ -108,7 +115,7 @@ __global__ __launch_bounds__(kJThreads) void k_jacobi(size_t nn, int nylm, int d
   __syncthreads();
   const int q = threadIdx.x;
   if (q < ndot || q == kDiisMax)
-    part[(size_t)q * kMaxBlk + blockIdx.x] =
+    part[(size_t)q * nbj + blockIdx.x] =
         (wpart[0][q] + wpart[1][q]) + (wpart[2][q] + wpart[3][q]);
 }
 
@@ -149,16 +156,31 @@ __device__ __forceinline__ void pivot_stage(double &big, int &best) {
 
 constexpr int kSmallThreads = 256;       // 4 workgroup partials per thread
 
-__global__ __launch_bounds__(kSmallThreads) void k_diis_small(int nblk, int diis_max, int nblk_e,
-                                                              int nsph, double tol, int maxit,
-                                                              Systems sys) {
+// an instance of a batch has ended: fold it into the summary, which is done
+// when the last instance is
+__device__ __forceinline__ void instance_done(DiisState *sum, int it, int ok) {
+  if (!sum) return;
+  atomicMax(&sum->it, it);
+  atomicAnd(&sum->ok, ok);
+  if (atomicSub(&sum->remaining, 1) == 1) sum->done = 1;
+}
+
+__global__ __launch_bounds__(kSmallThreads) void k_diis_small(
+    int diis_max, int nbj, const int *__restrict__ jboff, const int *__restrict__ eboff,
+    const int *__restrict__ soff, int ninst, double tol, int maxit, Systems sys) {
 #pragma clang fp contract(off)
-  const SysArgs &q0 = sys.s[blockIdx.x];
-  DiisState *__restrict__ st = q0.st;
+  const SysArgs &q0 = sys.s[blockIdx.y];
+  const int inst = blockIdx.x;
+  DiisState *__restrict__ st = q0.st + inst;
   if (st->done) return;
-  const double *__restrict__ part = q0.part;
-  const double *__restrict__ part_e = q0.part_e;
-  int *__restrict__ newslot_out = q0.newslot;
+  DiisState *sum = ninst > 1 ? q0.st + ninst : nullptr;
+  // this instance's workgroup partials
+  const int jb = jboff[inst], eb = eboff[inst];
+  const int nblk = jboff[inst + 1] - jb, nblk_e = eboff[inst + 1] - eb;
+  const int nsph = soff[inst + 1] - soff[inst];
+  const double *__restrict__ part = q0.part + jb;
+  const double *__restrict__ part_e = q0.part_e + eb;
+  int *__restrict__ newslot_out = q0.newslot + inst;
   __shared__ double red[kPartStride + 1];
   __shared__ int stop;
   const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
@@ -177,7 +199,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(int nblk, int diis
     if (q <= kPartStride) {
       const bool need = (q < ndot || q == kDiisMax) || (q == kPartStride && pending);
       if (need) {
-        const double *src = (q == kPartStride) ? part_e : part + (size_t)q * kMaxBlk;
+        const double *src = (q == kPartStride) ? part_e : part + (size_t)q * nbj;
         const int nb = (q == kPartStride) ? nblk_e : nblk;
 #pragma unroll 8
         for (int b = r; b < nb; b += kTPQ) sum += src[b];
@@ -204,6 +226,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(int nblk, int diis
       if (norm < tol || it + 1 >= maxit) {
         st->done = 1;
         stop = 1;
+        instance_done(sum, it + 1, norm < tol);
       }
     }
   }
@@ -405,16 +428,18 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(int nblk, int diis
 // x_new = sum_i c_i x_i (or the plain Jacobi step), increment norm partials
 // per sphere (hsnorm, ddcosmo.f90:1233-1271), x <- x_new.  One wave per sphere.
 // ... and xs = cf x_new where the system keeps it.
-__global__ __launch_bounds__(kJThreads) void k_extrap(int nsph, int nylm, size_t nn, int diis_max,
+__global__ __launch_bounds__(kJThreads) void k_extrap(int nylm, size_t nn, int diis_max,
+                                                      const int4 *__restrict__ eblk,
                                                       const double *__restrict__ invl1,
                                                       const double *__restrict__ cf,
                                                       Systems sys) {
   const SysArgs &q0 = sys.s[blockIdx.y];
-  const DiisState *__restrict__ st = q0.st;
+  const int4 blk = eblk[blockIdx.x];   // instance, spheres blk.y .. blk.z - 1
+  const DiisState *__restrict__ st = q0.st + blk.x;
   if (st->done) return;
   const double *__restrict__ xd = q0.xd;
   const double *__restrict__ xnew = q0.xnew;
-  const int *__restrict__ newslot = q0.newslot;
+  const int *__restrict__ newslot = q0.newslot + blk.x;
   double *__restrict__ x = q0.x;
   double *__restrict__ xsc = q0.xs;
   double *__restrict__ part = q0.part_e;
@@ -431,7 +456,7 @@ __global__ __launch_bounds__(kJThreads) void k_extrap(int nsph, int nylm, size_t
   const int wave = threadIdx.x / kWave, lane = lane_id();
   const int wpb = kJThreads / kWave;
   double usq = 0.0;
-  for (int i = blockIdx.x * wpb + wave; i < nsph; i += gridDim.x * wpb) {
+  for (int i = blk.y + wave; i < blk.z; i += wpb) {
     double s = 0.0;
     for (int lm = lane; lm < nylm; lm += kWave) {
       size_t k = (size_t)i * nylm + lm;
@@ -461,23 +486,55 @@ __global__ __launch_bounds__(kJThreads) void k_extrap(int nsph, int nylm, size_t
 
 // rms of the per-sphere norms and the convergence test (rmsvec,
 // jacobi_diis.f90:405-438; jacobi_diis.f90:153-156)
-__global__ __launch_bounds__(kWave) void k_norm(int nblk, int nsph, double tol, int maxit,
-                                                Systems sys) {
-  DiisState *__restrict__ st = sys.s[blockIdx.x].st;
+__global__ __launch_bounds__(kWave) void k_norm(const int *__restrict__ eboff,
+                                                const int *__restrict__ soff, int ninst,
+                                                double tol, int maxit, Systems sys) {
+  const int inst = blockIdx.x;
+  DiisState *__restrict__ st = sys.s[blockIdx.y].st + inst;
   if (st->done) return;
-  const double *__restrict__ part = sys.s[blockIdx.x].part_e;
+  const double *__restrict__ part = sys.s[blockIdx.y].part_e;
   double s = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += kWave) s += part[b];
+  for (int b = eboff[inst] + threadIdx.x; b < eboff[inst + 1]; b += kWave) s += part[b];
   s = wave_sum(s);
   if (threadIdx.x == 0) {
-    double norm = sqrt(s / (double)nsph);
+    double norm = sqrt(s / (double)(soff[inst + 1] - soff[inst]));
     int it = st->it;
     if (it <= kMaxIter) st->hist[it] = norm;
     st->it = it + 1;
     st->norm = norm;
     st->ok = norm < tol;
     st->done = (norm < tol) || (it + 1 >= maxit);
+    if (st->done) instance_done(ninst > 1 ? sys.s[blockIdx.y].st + ninst : nullptr, it + 1,
+                                norm < tol);
   }
+}
+
+// fresh states for the ninst instances and the summary of a batch
+__global__ void k_diis_init(int ninst, DiisState *__restrict__ st) {
+  const int q = blockIdx.x;
+  DiisState *d = st + q;
+  const int t = threadIdx.x;
+  if (t < kDiisMax) d->slot[t] = t;
+  if (t == 0) {
+    d->nmat = 1;
+    d->it = 0;
+    d->done = 0;
+    d->ok = q < ninst ? 0 : 1;
+    d->singular = 0;
+    d->pending_norm = 0;
+    d->ncoef = 0;
+    d->remaining = q < ninst ? 0 : ninst;
+    d->norm = 0.0;
+  }
+}
+
+// per instance: iterations as the reference reports them and the ok flag
+__global__ void k_diis_results(int ninst, int maxit, const DiisState *__restrict__ st,
+                               int *__restrict__ res) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ninst) return;
+  res[2 * q] = st[q].ok ? st[q].it : maxit + 1;
+  res[2 * q + 1] = st[q].ok;
 }
 
 __global__ void k_ldm1x(size_t nn, int nylm, const double *__restrict__ facl,
@@ -526,16 +583,29 @@ __global__ void k_wghpot(size_t np, const double *__restrict__ ui, const int *__
   g[p] = (u != 0.0) ? -u * phi[cavidx[p]] : 0.0;
 }
 
-__global__ __launch_bounds__(kJThreads) void k_dot_part(size_t nn, const double *__restrict__ a,
+// partial dot products over the elements of each k_jacobi block
+__global__ __launch_bounds__(kJThreads) void k_dot_part(const int4 *__restrict__ jblk,
+                                                        const double *__restrict__ a,
                                                         const double *__restrict__ b,
                                                         double *__restrict__ part) {
   __shared__ double lds[kJThreads / kWave];
+  const int4 blk = jblk[blockIdx.x];
   double s = 0.0;
-  for (size_t k = (size_t)blockIdx.x * kJThreads + threadIdx.x; k < nn;
-       k += (size_t)gridDim.x * kJThreads)
+  for (size_t k = (size_t)blk.y + threadIdx.x; k < (size_t)blk.z; k += kJThreads)
     s += a[k] * b[k];
   double tot = block_sum_256(s, lds);
   if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// out[q] = scale * sum of the partials of instance q's blocks, one wave each
+__global__ __launch_bounds__(kWave) void k_inst_sum(const int *__restrict__ boff, double scale,
+                                                    const double *__restrict__ part,
+                                                    double *__restrict__ out) {
+  const int q = blockIdx.x;
+  double s = 0.0;
+  for (int b = boff[q] + threadIdx.x; b < boff[q + 1]; b += kWave) s += part[b];
+  s = wave_sum(s);
+  if (threadIdx.x == 0) out[q] = scale * s;
 }
 
 int grid_blocks(size_t work, int per_block, int cap) {
@@ -557,16 +627,16 @@ int device_ldm1x(ddx_handle *h, hipStream_t s, const double *x, double *y) {
 
 int device_hnorm(ddx_handle *h, const double *x, double *norm_host) {
   int nb = grid_blocks(h->n, kJThreads / kWave, 1024);
-  DDX_TRY(h->part.reserve((size_t)1024 * kPartStride + 2048));
-  double *part = h->part.p + (size_t)1024 * kPartStride;
+  DDX_TRY(h->part.reserve(1024 + 1));
   hipLaunchKernelGGL(k_hnorm_part, dim3(nb), dim3(kJThreads), 0, h->stream, h->n, h->nylm,
-                     h->cf.p + h->nylm, x, part);
+                     h->cf.p + h->nylm, x, h->part.p);
   DDX_HIP(hipGetLastError());
-  DDX_TRY(h->energy.reserve(4));
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kWave), 0, h->stream, nb, 1.0, part, h->energy.p + 1);
+  DDX_TRY(h->energy.reserve(h->ninst + 4));
+  double *out = h->energy.p + h->ninst + 1;
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kWave), 0, h->stream, nb, 1.0, h->part.p, out);
   DDX_HIP(hipGetLastError());
   double s = 0.0;
-  DDX_HIP(hipMemcpyAsync(&s, h->energy.p + 1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  DDX_HIP(hipMemcpyAsync(&s, out, sizeof(double), hipMemcpyDeviceToHost, h->stream));
   DDX_HIP(hipStreamSynchronize(h->stream));
   *norm_host = sqrt(s / (double)h->n);
   return DDX_OK;
@@ -583,14 +653,54 @@ int build_rhs(ddx_handle *h, const double *d_phi, double *d_rhs) {
 }
 
 int energy_dot(ddx_handle *h, const double *a, const double *b, double scale, double *d_out) {
-  size_t nn = (size_t)h->nylm * h->n;
-  int nb = grid_blocks(nn, kJThreads * 4, 1024);
-  DDX_TRY(h->part.reserve((size_t)1024 * kPartStride + 2048));
-  double *part = h->part.p + (size_t)1024 * kPartStride;
-  hipLaunchKernelGGL(k_dot_part, dim3(nb), dim3(kJThreads), 0, h->stream, nn, a, b, part);
+  DDX_TRY(h->part.reserve((size_t)h->nbj + 1));
+  hipLaunchKernelGGL(k_dot_part, dim3(h->nbj), dim3(kJThreads), 0, h->stream, h->jblk.p, a, b,
+                     h->part.p);
   DDX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kWave), 0, h->stream, nb, scale, part, d_out);
+  hipLaunchKernelGGL(k_inst_sum, dim3(h->ninst), dim3(kWave), 0, h->stream, h->jboff.p, scale,
+                     h->part.p, d_out);
   DDX_HIP(hipGetLastError());
+  return DDX_OK;
+}
+
+// Block maps of the solver kernels, from the instances' sphere ranges: each
+// instance's elements (spheres) are split evenly over
+// ceil(elements / 512) (ceil(spheres / 4)) workgroups, at most 1024 each.
+int build_solver_maps(ddx_handle *h) {
+  std::vector<int4> jb, eb;
+  std::vector<int> jo(h->ninst + 1), eo(h->ninst + 1);
+  const long long ny = h->nylm;
+  for (int q = 0; q < h->ninst; ++q) {
+    const int s0 = h->h_soff[q], s1 = h->h_soff[q + 1];
+    const long long e0 = s0 * ny, e1 = s1 * ny;
+    jo[q] = (int)jb.size();
+    const int nj = grid_blocks((size_t)(e1 - e0), kJThreads * 2, 1024);
+    for (int j = 0; j < nj; ++j)
+      jb.push_back(make_int4(q, (int)(e0 + (e1 - e0) * j / nj), (int)(e0 + (e1 - e0) * (j + 1) / nj), 0));
+    eo[q] = (int)eb.size();
+    const int ne = grid_blocks((size_t)(s1 - s0), kJThreads / kWave, 1024);
+    for (int j = 0; j < ne; ++j)
+      eb.push_back(make_int4(q, s0 + (int)((long long)(s1 - s0) * j / ne),
+                             s0 + (int)((long long)(s1 - s0) * (j + 1) / ne), 0));
+  }
+  jo[h->ninst] = (int)jb.size();
+  eo[h->ninst] = (int)eb.size();
+  if ((long long)h->n * ny > 0x7fffffffll) {
+    set_error("nylm * nsph = %lld exceeds the 32-bit element index", (long long)h->n * ny);
+    return DDX_ERR_CAPACITY;
+  }
+  h->nbj = (int)jb.size();
+  h->nbe = (int)eb.size();
+  DDX_TRY(h->jblk.reserve(jb.size()));
+  DDX_TRY(h->eblk.reserve(eb.size()));
+  DDX_TRY(h->jboff.reserve(jo.size()));
+  DDX_TRY(h->eboff.reserve(eo.size()));
+  hipStream_t s = h->stream;
+  DDX_HIP(hipMemcpyAsync(h->jblk.p, jb.data(), jb.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+  DDX_HIP(hipMemcpyAsync(h->eblk.p, eb.data(), eb.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+  DDX_HIP(hipMemcpyAsync(h->jboff.p, jo.data(), jo.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  DDX_HIP(hipMemcpyAsync(h->eboff.p, eo.data(), eo.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  DDX_HIP(hipStreamSynchronize(s));
   return DDX_OK;
 }
 
@@ -608,10 +718,9 @@ SysArgs sys_args(const ddx_handle *h, Lane &ln) {
   a.ed = ln.ed.p;
   a.xnew = ln.xnew.p;
   a.part = ln.part.p;
-  a.part_e = ln.part.p + (size_t)1024 * kPartStride;
+  a.part_e = ln.part.p + (size_t)h->nbj * kPartStride;
   a.newslot = ln.misc.p;
   a.st = ln.diis.p;
-  (void)h;
   return a;
 }
 
@@ -650,12 +759,13 @@ int solve_begin(ddx_handle *h, Lane &ln, int star, int diis_max, double tol, con
     DDX_TRY(ln.xd.reserve(nn * diis_max));
     DDX_TRY(ln.ed.reserve(nn * diis_max));
   }
-  DDX_TRY(ln.part.reserve((size_t)kMaxBlk * kPartStride + 2048));
+  DDX_TRY(ln.part.reserve((size_t)h->nbj * kPartStride + h->nbe + 1));
   // the L* sweep leaves per-segment sums that k_jacobi reads: its buffer must
   // not move once the solve has started
   if (star) DDX_TRY(ln.yp.reserve((size_t)h->nseg_cap * h->nylm + 1));
-  DDX_TRY(ln.diis.reserve(2));
-  DDX_TRY(ln.misc.reserve(16));
+  DDX_TRY(ln.diis.reserve(h->ninst + 1));
+  DDX_TRY(ln.misc.reserve(h->ninst + 1));
+  DDX_TRY(ln.res.reserve(2 * (size_t)h->ninst + 2));
   ln.star = star;
   ln.diis_max = diis_max;
   ln.max_iter = max_iter;
@@ -669,17 +779,9 @@ int solve_begin(ddx_handle *h, Lane &ln, int star, int diis_max, double tol, con
   ln.group = 3;
   ln.active = true;
 
-  DiisState *hs = ln.h_diis;
-  hs->nmat = 1;
-  hs->it = 0;
-  hs->done = 0;
-  hs->ok = 0;
-  hs->singular = 0;
-  hs->ncoef = 0;
-  hs->pending_norm = 0;
-  for (int q = 0; q < kDiisMax; ++q) hs->slot[q] = q;
-  hs->norm = 0.0;
-  DDX_HIP(hipMemcpyAsync(ln.diis.p, hs, sizeof(DiisState), hipMemcpyHostToDevice, ln.stream));
+  const int nst = h->ninst > 1 ? h->ninst + 1 : 1;
+  hipLaunchKernelGGL(k_diis_init, dim3(nst), dim3(kWave), 0, ln.stream, h->ninst, ln.diis.p);
+  DDX_HIP(hipGetLastError());
   // the L X sweep reads cf x: from here on k_extrap keeps it
   if (!star) DDX_TRY(cf_scale(h, ln.stream, d_x, ln.xs.p));
   return DDX_OK;
@@ -698,8 +800,7 @@ int solve_enqueue(ddx_handle *h, Lane *const *lanes, int count, int slot) {
   }
   hipStream_t s = l0.stream;
   size_t nn = (size_t)h->nylm * h->n;
-  int nbj = grid_blocks(nn, kJThreads * 2, kMaxBlk);   // two elements per thread
-  int nbe = grid_blocks(h->n, kJThreads / kWave, 1024);
+  const int nbj = h->nbj, nbe = h->nbe, ninst = h->ninst;
   const double *invl1 = h->cf.p + h->nylm;   // 1/(l+1) table stored after cf
   const int diis_max = l0.diis_max;
   Systems sys{};
@@ -710,31 +811,32 @@ int solve_enqueue(ddx_handle *h, Lane *const *lanes, int count, int slot) {
     if (pair) {
       DDX_TRY(apply_pair(h, l0, l0.y.p, *lanes[1], lanes[1]->x, kProjSplit));
     } else if (l0.star) {
-      DDX_TRY(apply_lstarx(h, l0, l0.x, l0.y.p, l0.diis.p, false));
+      DDX_TRY(apply_lstarx(h, l0, l0.x, l0.y.p, solve_gate(h, l0), false));
     } else {
-      DDX_TRY(apply_lx(h, l0, l0.x, l0.y.p, l0.diis.p, true, kProjSplit));
+      DDX_TRY(apply_lx(h, l0, l0.x, l0.y.p, solve_gate(h, l0), true, kProjSplit));
     }
     {
       TimedLaunch tl(h, DDX_K_JACOBI, s);
       hipExtLaunchKernelGGL(k_jacobi, dim3(nbj, count), dim3(kJThreads), 0, s, tl.b, tl.e, 0, nn,
-                            h->nylm, diis_max, h->facl.p, sys);
+                            h->nylm, diis_max, h->jblk.p, nbj, h->facl.p, sys);
     }
     DDX_HIP(hipGetLastError());
     if (diis_max > 0) {
       TimedLaunch tl(h, DDX_K_DIIS, s);
-      hipExtLaunchKernelGGL(k_diis_small, dim3(count), dim3(kSmallThreads), 0, s, tl.b, tl.e, 0,
-                            nbj, diis_max, nbe, h->n, l0.tol, l0.max_iter, sys);
+      hipExtLaunchKernelGGL(k_diis_small, dim3(ninst, count), dim3(kSmallThreads), 0, s, tl.b,
+                            tl.e, 0, diis_max, nbj, h->jboff.p, h->eboff.p, h->soff.p, ninst,
+                            l0.tol, l0.max_iter, sys);
       DDX_HIP(hipGetLastError());
     }
     {
       TimedLaunch tl(h, DDX_K_EXTRAP, s);
-      hipExtLaunchKernelGGL(k_extrap, dim3(nbe, count), dim3(kJThreads), 0, s, tl.b, tl.e, 0, h->n,
-                            h->nylm, nn, diis_max, invl1, h->cf.p, sys);
+      hipExtLaunchKernelGGL(k_extrap, dim3(nbe, count), dim3(kJThreads), 0, s, tl.b, tl.e, 0,
+                            h->nylm, nn, diis_max, h->eblk.p, invl1, h->cf.p, sys);
     }
     DDX_HIP(hipGetLastError());
     if (diis_max == 0) {
-      hipLaunchKernelGGL(k_norm, dim3(count), dim3(kWave), 0, s, nbe, h->n, l0.tol, l0.max_iter,
-                         sys);
+      hipLaunchKernelGGL(k_norm, dim3(ninst, count), dim3(kWave), 0, s, h->eboff.p, h->soff.p,
+                         ninst, l0.tol, l0.max_iter, sys);
       DDX_HIP(hipGetLastError());
     }
   }
@@ -742,19 +844,22 @@ int solve_enqueue(ddx_handle *h, Lane *const *lanes, int count, int slot) {
   h->tag_round = 0;
   l0.group = 2;
   for (int k = 0; k < count; ++k)
-    DDX_HIP(hipMemcpyAsync(lanes[k]->h_head + 4 * slot, lanes[k]->diis.p, 4 * sizeof(int),
+    DDX_HIP(hipMemcpyAsync(lanes[k]->h_head + 4 * slot, solve_gate(h, *lanes[k]), 4 * sizeof(int),
                            hipMemcpyDeviceToHost, s));
   DDX_HIP(hipEventRecord(h->gev[slot], s));
   return DDX_OK;
 }
 
 int solve_end(ddx_handle *h, Lane &ln, int *n_iter, int *ok, std::vector<double> *hist) {
-  (void)h;
   const DiisState *hs = ln.h_diis;
   ln.active = false;
+  // a batch: the summary's ok and largest count; per instance in h_res
   *n_iter = hs->ok ? hs->it : ln.max_iter + 1;
   *ok = hs->ok;
-  if (hist) hist->assign(hs->hist, hs->hist + (hs->it < kMaxIter ? hs->it : kMaxIter));
+  if (hist) {
+    if (h->ninst == 1) hist->assign(hs->hist, hs->hist + (hs->it < kMaxIter ? hs->it : kMaxIter));
+    else hist->clear();
+  }
   return DDX_OK;
 }
 
@@ -776,9 +881,19 @@ int solve_run(ddx_handle *h, Lane *const *lanes, int count) {
     if (all || !more) break;
     cur ^= 1;
   }
-  for (int k = 0; k < count; ++k)
-    DDX_HIP(hipMemcpyAsync(lanes[k]->h_diis, lanes[k]->diis.p, sizeof(DiisState),
-                           hipMemcpyDeviceToHost, lanes[k]->stream));
+  for (int k = 0; k < count; ++k) {
+    Lane &ln = *lanes[k];
+    DDX_HIP(hipMemcpyAsync(ln.h_diis, solve_gate(h, ln), sizeof(DiisState),
+                           hipMemcpyDeviceToHost, ln.stream));
+    if (h->ninst > 1) {
+      hipLaunchKernelGGL(k_diis_results, dim3((h->ninst + 255) / 256), dim3(256), 0, ln.stream,
+                         h->ninst, ln.max_iter, ln.diis.p, ln.res.p);
+      DDX_HIP(hipGetLastError());
+      ln.h_res.resize(2 * (size_t)h->ninst);
+      DDX_HIP(hipMemcpyAsync(ln.h_res.data(), ln.res.p, 2 * sizeof(int) * h->ninst,
+                             hipMemcpyDeviceToHost, ln.stream));
+    }
+  }
   DDX_HIP(hipStreamSynchronize(lanes[0]->stream));
   // launches of rounds after a solve's last iteration returned at once:
   // drop them from the timers

@@ -533,7 +533,7 @@ int apply_pair(ddx_handle *h, Lane &ld, double *yd, Lane &la, const double *xa, 
   DDX_TRY(ld.P.reserve(h->np));
   DDX_TRY(la.xiw.reserve(h->np));
   DDX_TRY(la.yp.reserve((size_t)h->nseg_cap * h->nylm + 1));
-  const DiisState *gd = ld.diis.p, *ga = la.diis.p;
+  const DiisState *gd = solve_gate(h, ld), *ga = solve_gate(h, la);
   DDX_TRY(expand(h, s, xa, la.xiw.p, true, ga));
   DDX_TRY(sweep(h, s, 3, ld.xs.p, ld.rw.p, la.xiw.p, la.yp.p, gd, ga));
   return finish_lx(h, ld, yd, gd, ysplit);

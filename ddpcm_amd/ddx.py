@@ -27,6 +27,8 @@ EXPORTS = (
     "ddx_efld", "ddx_lx", "ddx_lstarx", "ddx_ldm1x", "ddx_hnorm", "ddx_jacobi_diis",
     "ddx_get_history", "ddx_energy_forces", "ddx_upload", "ddx_run", "ddx_result",
     "ddx_set_precision", "ddx_get_stats", "ddx_kernel_timer", "ddx_kernel_timer_read",
+    "ddx_batch_init", "ddx_batch_energy_forces", "ddx_batch_result", "ddx_batch_iterations",
+    "ddx_batch_sizes",
 )
 KERNELS = ("sweep_l", "sweep_lstar", "project", "expand", "jacobi", "diis", "extrap", "field",
            "potential", "forces", "sweep_pair")
@@ -89,6 +91,12 @@ def load_library(path: str = LIB_PATH):
     L.ddx_get_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
     L.ddx_kernel_timer.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.ddx_kernel_timer_read.argtypes = [C.c_void_p, C.c_int, _ip, _dp]
+    L.ddx_batch_init.argtypes = [C.POINTER(C.c_void_p), C.c_int, _ip, _dp, _dp, _dp, _dp,
+                                 C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
+    L.ddx_batch_energy_forces.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    L.ddx_batch_result.argtypes = [C.c_void_p, _dp, _dp]
+    L.ddx_batch_iterations.argtypes = [C.c_void_p, _ip, _ip]
+    L.ddx_batch_sizes.argtypes = [C.c_void_p, _ip, _ip, _ip]
     _LIB = L
     return L
 
@@ -294,3 +302,69 @@ class DDCosmo:
 
     def set_precision(self, prec):
         self._check(self.L.ddx_set_precision(self.h, prec))
+
+
+class DDCosmoBatch(DDCosmo):
+    """A batch of independent ddCOSMO systems on one handle (no counterpart in
+    the reference, which solves one system per process).  Each instance
+    gets what main.f90:164-265 gives it alone; all run in the same kernels.
+
+    xs, ys, zs, rs: per-instance coordinate and radius arrays in bohr.
+    """
+
+    def __init__(self, xs, ys, zs, rs, lmax, ngrid, iconv=8, igrad=1, eps=78.39, eta=0.1):
+        self.L = load_library()
+        sizes = np.array([len(np.ravel(x)) for x in xs], dtype=np.int32)
+        x, y, z, r = (np.ascontiguousarray(np.concatenate([np.ravel(a) for a in v]),
+                                           dtype=np.float64) for v in (xs, ys, zs, rs))
+        self.h = C.c_void_p()
+        self._check(self.L.ddx_batch_init(C.byref(self.h), sizes.size,
+                                          sizes.ctypes.data_as(_ip), _p(x), _p(y), _p(z), _p(r),
+                                          lmax, ngrid, iconv, igrad, eps, eta))
+        s = [C.c_int() for _ in range(5)]
+        self._check(self.L.ddx_get_sizes(self.h, *[C.byref(v) for v in s]))
+        self.nsph, self.ncav, self.nylm, self.ngrid, self.lmax = (v.value for v in s)
+        self.iconv, self.igrad, self.eps, self.eta, self.iprint = iconv, igrad, eps, eta, 0
+        self.count = int(sizes.size)
+        so = np.zeros(self.count + 1, dtype=np.int32)
+        co = np.zeros(self.count + 1, dtype=np.int32)
+        self._check(self.L.ddx_batch_sizes(self.h, None, so.ctypes.data_as(_ip),
+                                           co.ctypes.data_as(_ip)))
+        self.sph_off, self.cav_off = so, co
+
+    @classmethod
+    def from_inputs(cls, decks, **over):
+        b = [d.bohr() for d in decks]
+        d0 = decks[0]
+        kw = dict(lmax=d0.lmax, ngrid=d0.ngrid, iconv=d0.iconv, igrad=d0.igrad, eps=d0.eps,
+                  eta=d0.eta)
+        kw.update(over)
+        return cls([v[0] for v in b], [v[1] for v in b], [v[2] for v in b], [v[3] for v in b],
+                   **kw)
+
+    def energy_forces(self, charges):
+        """(esolv[count] in Eh, fx (nsph, 3) in Eh/bohr, instance after instance)."""
+        q = np.ascontiguousarray(np.concatenate([np.ravel(c) for c in charges]),
+                                 dtype=np.float64)
+        es = np.zeros(self.count)
+        fx = np.zeros((self.nsph, 3))
+        self._check(self.L.ddx_batch_energy_forces(self.h, _p(q), _p(es), _p(fx)))
+        return es, fx
+
+    def result(self):
+        es = np.zeros(self.count)
+        fx = np.zeros((self.nsph, 3))
+        self._check(self.L.ddx_batch_result(self.h, _p(es), _p(fx)))
+        return es, fx
+
+    def iterations(self):
+        """(direct, adjoint) iteration counts per instance."""
+        d = np.zeros(self.count, dtype=np.int32)
+        a = np.zeros(self.count, dtype=np.int32)
+        self._check(self.L.ddx_batch_iterations(self.h, d.ctypes.data_as(_ip),
+                                                a.ctypes.data_as(_ip)))
+        return d, a
+
+    def split(self, fx):
+        """Per-instance views of a (nsph, 3) array."""
+        return [fx[self.sph_off[q]:self.sph_off[q + 1]] for q in range(self.count)]

@@ -146,6 +146,14 @@ def batch_sizes(count: int = 512, lo: int = 20, hi: int = 800, seed: int = 4000)
     return np.rint(np.exp(rng.uniform(math.log(lo), math.log(hi), size=count))).astype(int)
 
 
+def batch_inputs(count: int = 512, lo: int = 20, hi: int = 800, seed: int = 4000,
+                 lmax: int = 6, ngrid: int = 302, igrad: int = 1) -> list:
+    """The batched workload: `count` molecules of batch_sizes() atoms, molecule
+    q drawn with seed + 1 + q."""
+    return [synthetic_input(int(n), seed + 1 + q, lmax, ngrid, igrad=igrad)
+            for q, n in enumerate(batch_sizes(count, lo, hi, seed))]
+
+
 def trajectory(n: int = 1000, frames: int = 2000, seed: int = 1000,
                sigma: float = 0.05):
     """Yields (charge, xyz, rvdw) per frame: a synthetic molecule whose atoms

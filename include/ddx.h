@@ -126,6 +126,33 @@ int ddx_upload(ddx_handle *h, const double *x, const double *y,
 int ddx_run(ddx_handle *h);
 int ddx_result(ddx_handle *h, double *esolv, double *fx);
 
+/* Batches (no counterpart in the reference, which holds one system per
+ * process in module state; each instance follows main.f90:164-265 as if
+ * it ran alone, with its own neighbour list, potential, Jacobi/DIIS state,
+ * convergence test and forces).  `count` molecules with nsph[q] spheres
+ * each; coordinates, radii (and later charges) are concatenated in
+ * instance order.  One handle, one pass of every kernel for all instances:
+ * the sweeps, projections and Jacobi steps cover the whole batch, and each
+ * instance stops updating once it has converged.
+ *   ddx_batch_energy_forces  esolv[count]; fx(3, sum nsph).  If an instance
+ *                            does not converge the others are still
+ *                            complete and the call returns DDX_ERR_NOCONV.
+ *   ddx_batch_result         after ddx_upload + ddx_run on a batch handle
+ *   ddx_batch_iterations     direct / adjoint iterations per instance
+ *                            (n_iter + 1 where it did not converge)
+ *   ddx_batch_sizes          count, sphere and cavity offsets (count + 1)
+ * The single-system calls ddx_cosmo, ddx_jacobi_diis, ddx_energy_forces and
+ * ddx_result refuse batch handles (DDX_ERR_STATE); the operator calls apply
+ * to all instances at once. */
+int ddx_batch_init(ddx_handle **h, int count, const int *nsph, const double *x,
+                   const double *y, const double *z, const double *rvdw, int lmax,
+                   int ngrid, int iconv, int igrad, double eps, double eta);
+int ddx_batch_energy_forces(ddx_handle *h, const double *charge, double *esolv,
+                            double *fx);
+int ddx_batch_result(ddx_handle *h, double *esolv, double *fx);
+int ddx_batch_iterations(const ddx_handle *h, int *direct, int *adjoint);
+int ddx_batch_sizes(const ddx_handle *h, int *count, int *sph_off, int *cav_off);
+
 int ddx_set_precision(ddx_handle *h, int prec);
 int ddx_get_stats(const ddx_handle *h, ddx_stats *st);
 
