@@ -454,7 +454,17 @@ int ddx_get_array(const ddx_handle *hc, const char *name, double *dst) {
     if (!h->igrad) { set_error("zi exists only with igrad = 1"); return DDX_ERR_STATE; }
     src = h->zi.p; cnt = 3 * np;
   } else if (!strcmp(name, "ccav")) { src = h->ccav.p; cnt = 3 * (size_t)h->ncav; }
-  else {
+  else if (!strcmp(name, "rhs") || !strcmp(name, "xi")) {
+    // the last RHS a direct solve used (cosmo.f90:102-128), the last
+    // xi = basis^T S of the forces (forces_dd.f90:108-114)
+    const bool rhs = name[0] == 'r';
+    src = rhs ? h->rhs.p : h->xi.p;
+    cnt = rhs ? (size_t)h->nylm * h->n : np;
+    if (!src) {
+      set_error("'%s' is not available yet", name);
+      return DDX_ERR_STATE;
+    }
+  } else {
     set_error("unknown array '%s'", name);
     return DDX_ERR_ARG;
   }

@@ -88,7 +88,35 @@ contains
     call ddx_check(ddx_get_neighbours(handle, inl, nl), 'ddinit')
     nnl = inl(nsph+1) - 1
     nl = nl(1:max(1,nnl))
+    if (iprint >= 4) call dump_setup
   end subroutine ddinit
+
+  ! the tables and the geometry that ddinit prints at iprint >= 4
+  ! (ddcosmo.f90:274-280, 329-338, 413-416, 472-478), in its order and
+  ! formats; the table labelled facl holds facs there, and so here
+  subroutine dump_setup
+    integer :: isph, ii
+    call prtsph('facs', 1, 0, facs)
+    call prtsph('facl', 1, 0, facs)
+    call prtsph('basis', ngrid, 0, basis)
+    call ptcart('grid', 3, 0, grid)
+    call ptcart('weights', 1, 0, w)
+    write(iout,*) '   inl:'
+    write(iout,'(10i6)') inl(1:nsph+1)
+    write(iout,*)
+    do isph = 1, nsph
+      write(iout,'(t3,"neighbours of sphere ",i6)') isph
+      write(iout,'(t5,12i6)') nl(inl(isph):inl(isph+1)-1)
+    end do
+    write(iout,*)
+    call ptcart('fi', nsph, 0, fi)
+    call ptcart('ui', nsph, 0, ui)
+    write(iout,*) '   external cavity points:'
+    do ii = 1, ncav
+      write(iout,'(t3,i8,3f14.6)') ii, ccav(:,ii)
+    end do
+    write(iout,*)
+  end subroutine dump_setup
 
   ! memfree (ddcosmo.f90:491)
   subroutine memfree
@@ -124,12 +152,24 @@ contains
     integer(c_int) :: rc, istar, icart
     integer :: c1, c2, cr
     real*8 :: es
+    real*8, allocatable :: rhs(:,:)
     call system_clock(count_rate=cr)
     call system_clock(count=c1)
     istar = merge(1, 0, star)
     icart = merge(1, 0, cart)
     es = zero
     rc = ddx_cosmo(handle, istar, icart, phi, glm, psi, sigma, es)
+    ! the right-hand side, before the iterations (cosmo.f90:131,169)
+    if (iprint >= 4 .and. (rc == 0 .or. rc == DDX_ERR_NOCONV)) then
+      if (star) then
+        call prtsph('rhs of the ddCOSMO adjoint equation', nsph, 0, psi)
+      else
+        allocate(rhs(nylm,nsph))
+        call ddx_check(ddx_get_array(handle, 'rhs'//c_null_char, rhs), 'cosmo')
+        call prtsph('rhs of the ddCOSMO equation', nsph, 0, rhs)
+        deallocate(rhs)
+      end if
+    end if
     call print_history(istar)
     if (rc == DDX_ERR_NOCONV) then
       if (star) then
@@ -175,11 +215,26 @@ contains
     real*8, dimension(ncav), intent(in) :: phi
     real*8, dimension(nylm,nsph), intent(in) :: sigma, s
     real*8, dimension(3,n), intent(inout) :: fx
-    integer :: c1, c2, cr
+    integer :: c1, c2, cr, isph
+    real*8 :: fep
+    real*8, allocatable :: xi(:,:)
     call system_clock(count_rate=cr)
     call system_clock(count=c1)
     call ddx_check(ddx_forces_dd(handle, phi, sigma, s, fx), 'forces_dd')
     call system_clock(count=c2)
+    if (iprint >= 4) then
+      ! xi = basis^T s, and the contributions before the factor 1/2 f(eps)
+      ! (forces_dd.f90:116,141-147)
+      allocate(xi(ngrid,nsph))
+      call ddx_check(ddx_get_array(handle, 'xi'//c_null_char, xi), 'forces_dd')
+      call ptcart('xi', nsph, 0, xi)
+      deallocate(xi)
+      fep = pt5*(eps - one)/eps
+      write(iout, '(1x,"ddCOSMO-only contributions to the forces (atomic units):",/,1x," atom",15x,"x",15x,"y",15x,"z")')
+      do isph = 1, nsph
+        write(iout, '(1x,i5,3f16.8)') isph, fx(:,isph)/fep
+      end do
+    end if
     if (iprint > 0) write(iout, '(a,f8.3,a)') &
       ' the computation of the ddCOSMO part of the forces took ', dble(c2-c1)/dble(cr), ' seconds.'
   end subroutine forces_dd
@@ -286,8 +341,10 @@ contains
         end do
       end do
     else
+      ! blocks of five columns, then the block of the remaining ncol mod 5,
+      ! which the reference writes also when it is empty
       write(iout, '(3x,a)') label
-      do c0 = 1, ncol, 5
+      do c0 = 1, ncol - mod(ncol, 5) + 1, 5
         c1 = min(ncol, c0 + 4)
         write(iout, '(8x,5i14)') (j, j = c0, c1)
         do l = 0, lmax
@@ -313,7 +370,7 @@ contains
       end do
     else
       write(iout, '(3x,a)') label
-      do c0 = 1, ncol, 5
+      do c0 = 1, ncol - mod(ncol, 5) + 1, 5
         c1 = min(ncol, c0 + 4)
         write(iout, '(6x,5i14)') (j, j = c0, c1)
         do ig = 1, ngrid

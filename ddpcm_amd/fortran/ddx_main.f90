@@ -12,6 +12,7 @@ program ddx_main
   real*8, parameter :: toang = 0.52917721092d0, tokcal = 627.509469d0
   real*8, allocatable :: x(:), y(:), z(:), rvdw(:), charge(:)
   real*8, allocatable :: phi(:), psi(:,:), sigma(:,:), s(:,:), fx(:,:), zeta(:), ef(:,:)
+  real*8, allocatable :: zpad(:)
   character(len=512) :: fname
 
   fname = 'Input.txt'
@@ -55,6 +56,15 @@ program ddx_main
     if (iprint >= 3) call prtsph('solution to the ddCOSMO adjoint equation', nsph, 0, s)
     call forces_dd(n, phi, sigma, s, fx)
     call ddmkzeta(s, zeta)
+    ! (main.f90:214 prints zeta(ncav) as an (ngrid, nsph) table, reading
+    ! past its end; here the positions beyond ncav print as zero)
+    if (iprint >= 4) then
+      allocate(zpad(ngrid*nsph))
+      zpad = zero
+      zpad(1:ncav) = zeta
+      call ptcart('zeta', nsph, 0, zpad)
+      deallocate(zpad)
+    end if
     ! the solute's field at the cavity points contracted with zeta
     call efld(n, charge, csph, ncav, ccav, ef)
     ii = 0

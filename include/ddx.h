@@ -64,7 +64,10 @@ int ddx_get_sizes(const ddx_handle *h, int *nsph, int *ncav, int *nylm,
 
 /* module arrays (ddcosmo.f90:119-123): "ccav" (3,ncav), "csph" (3,nsph),
  * "rsph", "w", "grid" (3,ngrid), "basis" (nylm,ngrid), "facl", "facs",
- * "fi", "ui" (ngrid,nsph), "zi" (3,ngrid,nsph). */
+ * "fi", "ui" (ngrid,nsph), "zi" (3,ngrid,nsph); and the intermediates the
+ * reference prints at iprint >= 4: "rhs" (nylm,nsph) of the last direct
+ * solve (cosmo.f90:131), "xi" (ngrid,nsph) of the last forces
+ * (forces_dd.f90:116). */
 int ddx_get_array(const ddx_handle *h, const char *name, double *dst);
 /* inl (nsph+1, 1-based as in the reference) and nl (inl[nsph]-1 entries) */
 int ddx_get_neighbours(const ddx_handle *h, int *inl, int *nl);

@@ -89,6 +89,31 @@ def test_mkrhs_and_direct_solve(name):
     assert h.stats().niter_direct == g["niter_direct"]
 
 
+@pytest.mark.parametrize("name", ["formaldehyde", "caffeine", "crambin", "c2_small"])
+def test_qm_path_rhs_from_multipoles(name, oracle_mod):
+    """cosmo(star=.false., cart=.false., glm) takes the right-hand side as
+    given (cosmo.f90:125-128), the path of QM codes: with the reference's
+    own RHS it must give the reference's solution, and with any other
+    multipoles the oracle's."""
+    d, h = make(name)
+    g = golden(name)
+    _, psi = h.mkrhs(d.charge)
+    sigma, es = h.cosmo(False, False, glm=g["rhs"], psi=psi)
+    assert abs(es - g["esolv"]) <= 1e-10 * abs(g["esolv"])
+    assert hsnorm_rms(sigma - g["sigma"], h.lmax) < 1e-8
+    assert h.stats().niter_direct == g["niter_direct"]
+    rng = np.random.default_rng(11)
+    glm = rng.standard_normal(g["rhs"].shape) * 1e-2
+    sigma2, es2 = h.cosmo(False, False, glm=glm, psi=psi)
+    o = oracle_mod.Oracle.from_input(d)
+    so, eo, ho, rc = o.cosmo(False, psi=psi, glm=glm)
+    assert rc == 0 and h.stats().niter_direct == ho.n_iter
+    # random multipoles: the energy sum cancels, so relative to its terms
+    scale = 0.5 * (d.eps - 1.0) / d.eps * float(np.sum(np.abs(so * psi)))
+    assert abs(es2 - eo) <= 1e-10 * scale
+    assert hsnorm_rms(sigma2 - so, h.lmax) < 1e-8
+
+
 @pytest.mark.parametrize("name", [n for n in ALL if n != "c1"])
 def test_adjoint_solve_and_forces(name):
     d, h = make(name)
