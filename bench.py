@@ -1,13 +1,15 @@
 """Energy + forces ddCOSMO solves per second on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload c2] [--precision fp64]
+    python bench.py [--steps K] [--warmup W] [--workload c2|c3|c4] [--precision fp64]
 
 One step is one complete solve from coordinates already resident in HBM:
 device setup (neighbours, fi/ui/zi, cavity, triple lists), RHS (solute
 potential at the cavity points, projection), direct Jacobi/DIIS solve and
 energy, adjoint solve, analytic forces with zeta and both field terms.  The
 workload is the 2000-atom synthetic system (lmax 8, 590 Lebedev points,
-eps 78.39, eta 0.1, convergence 1e-8).
+eps 78.39, eta 0.1, convergence 1e-8); c3 is the 20000-atom one (lmax 10,
+1202 points), c4 a batch of 512 molecules of 20..800 atoms (lmax 6, 302
+points) solved on one handle, one step being the whole batch.
 
 Beside the GPU figure the reference's own OpenMP Fortran program (compiled
 from its sources into oracle/_ref/main.exe) is timed on the host cores on
@@ -49,23 +51,31 @@ def diis_depths(niter, ndiis=25):
     return out
 
 
-def algorithmic_work(h, st, d):
+def algorithmic_work(h, st, iters=None):
     """Per kernel class: (unit, work per solve, launches per solve) for an
     energy + forces solve.  Counting convention: sweeps T (7 nylm + 40) flops
     per operator apply; MFMA products 2 nylm ngrid N flops; all-pairs
     20 ncav N flops per pass (the potential pass also returns the field, the
     field kernel is the zeta field at the nuclei); Jacobi update (4 + m) n 8
     bytes (rhs, y, x, m - 1 history residuals read; x_new, e written) and
-    extrapolation (m + 2) n 8 bytes at DIIS depth m."""
-    n = h.nylm * h.nsph
+    extrapolation (m + 2) n 8 bytes at DIIS depth m.  A batch counts each
+    molecule's own vector length, iterations and all-pairs sums (iters:
+    per-molecule (sizes, ncav, direct, adjoint)); its sweeps and products run
+    over the whole batch until the last molecule converges."""
     T = st.ntriples
     sweep = T * (7.0 * h.nylm + 40.0)
     mm = 2.0 * h.nylm * h.ngrid * h.nsph
-    nd, na = st.niter_direct, st.niter_adjoint
-    depth = diis_depths(nd) + diis_depths(na)
-    jac = sum((4 + m) * n * 8.0 for m in depth)
-    ext = sum((m + 2) * n * 8.0 for m in depth)
-    pairs = 20.0 * h.ncav * h.nsph
+    if iters is None:
+        iters = [(h.nsph, h.ncav, st.niter_direct, st.niter_adjoint)]
+    jac = ext = pairs = 0.0
+    for nq, cq, ndq, naq in iters:
+        n = h.nylm * nq
+        depth = diis_depths(ndq) + diis_depths(naq)
+        jac += sum((4 + m) * n * 8.0 for m in depth)
+        ext += sum((m + 2) * n * 8.0 for m in depth)
+        pairs += 20.0 * cq * nq
+    nd = max(v[2] for v in iters)
+    na = max(v[3] for v in iters)
     # with forces the two solves iterate in lockstep: one sweep, Jacobi, DIIS
     # and extrapolation launch per round serves both
     rounds = max(nd, na)
@@ -80,13 +90,13 @@ def algorithmic_work(h, st, d):
     }
 
 
-def kernel_table(h, st, d):
+def kernel_table(h, iters_fn=None):
     """One solve with every kernel class timed: per-class launches, time and
     achieved rate against its roofline."""
     for k in KERNELS:
         h.kernel_timer(k, True)
     h.run()
-    work = algorithmic_work(h, h.stats(), d)
+    work = algorithmic_work(h, h.stats(), iters_fn() if iters_fn else None)
     table = {}
     for k in KERNELS:
         n, ms = h.kernel_timer_read(k)
@@ -104,11 +114,11 @@ def kernel_table(h, st, d):
     return table
 
 
-def cpu_baseline(d, cores, timeout=300):
-    """One solve of the workload by the reference program on `cores` threads."""
+def _reference_wall(d, cores, timeout):
+    """Process wall time of one solve of deck d by the reference program."""
     exe = os.path.join(ROOT, "oracle", "_ref", "main.exe")
     if not os.path.exists(exe):
-        return None
+        return None, None
     from ddpcm_amd import inputs
     tmp = tempfile.mkdtemp(prefix="ddpcm_ref_")
     try:
@@ -121,18 +131,44 @@ def cpu_baseline(d, cores, timeout=300):
                            env=env)
         wall = time.perf_counter() - t0
         if r.returncode != 0:
-            return None
+            return None, None
         m = re.search(r"solvation energy \(kcal/mol\):\s*([-0-9.Ee+]+)", r.stdout)
-        e_kcal = float(m.group(1)) if m else None
-        return {"value": 1.0 / wall, "unit": "solves/s", "cores": cores, "kind": "reference",
-                "sample": "1 E+F solve of the %d-atom workload by the reference program "
-                          "(oracle/_ref/main.exe, flang -O3 -fopenmp, nproc=%d), process wall "
-                          "time %.2f s" % (d.n, cores, wall),
-                "energy_kcal": e_kcal}
+        return wall, (float(m.group(1)) if m else None)
     except subprocess.TimeoutExpired:
-        return None
+        return None, None
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
+
+
+def cpu_baseline(d, cores, timeout=300):
+    """One solve of the workload by the reference program on `cores` threads."""
+    wall, e_kcal = _reference_wall(d, cores, timeout)
+    if wall is None:
+        return None
+    return {"value": 1.0 / wall, "unit": "solves/s", "cores": cores, "kind": "reference",
+            "sample": "1 E+F solve of the %d-atom workload by the reference program "
+                      "(oracle/_ref/main.exe, flang -O3 -fopenmp, nproc=%d), process wall "
+                      "time %.2f s" % (d.n, cores, wall),
+            "energy_kcal": e_kcal}
+
+
+def cpu_baseline_batch(decks, cores, budget_s=20.0):
+    """The reference program on the batch's molecules one after another (it
+    holds one system per process), for about budget_s seconds."""
+    tot, done = 0.0, 0
+    for d in decks:
+        wall, _ = _reference_wall(d, cores, 120)
+        if wall is None:
+            return None
+        tot += wall
+        done += 1
+        if tot >= budget_s:
+            break
+    return {"value": done / tot, "unit": "solves/s", "cores": cores, "kind": "reference",
+            "sample": "the first %d molecules of the batch (%d atoms in all), one reference "
+                      "process each (oracle/_ref/main.exe, flang -O3 -fopenmp, nproc=%d), "
+                      "%.2f s of process wall time" % (done, sum(x.n for x in decks[:done]),
+                                                       cores, tot)}
 
 
 def copy_calibration(nbytes=1 << 30):
@@ -149,12 +185,12 @@ def copy_calibration(nbytes=1 << 30):
     print(json.dumps({"copy_bytes": nbytes, "check": float(dst[-1].item() + evict[0].item())}))
 
 
-def traffic_from_profiles(kernel):
-    """HBM bytes per launch of `kernel` from profiles/<kernel>_*_traffic.md
+def traffic_from_profiles(kernel, tag):
+    """HBM bytes per launch of `kernel` from profiles/<kernel>_<tag>_traffic.md
     (FETCH_SIZE + WRITE_SIZE counters scaled by the copy calibration), if
     that summary exists for this workload."""
     import glob
-    for path in glob.glob(os.path.join(ROOT, "profiles", "%s_*_traffic.md" % kernel)):
+    for path in glob.glob(os.path.join(ROOT, "profiles", "%s_%s_traffic.md" % (kernel, tag))):
         with open(path) as f:
             for line in f:
                 m = re.match(r"\s*traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", line)
@@ -185,16 +221,35 @@ def main():
         sys.exit(2)
 
     from ddpcm_amd import inputs
-    from ddpcm_amd.ddx import DDCosmo
+    from ddpcm_amd.ddx import DDCosmo, DDCosmoBatch
 
-    d = inputs.config_input(args.workload)
-    x, y, z, r = d.bohr()
-    h = DDCosmo.from_input(d)
-    h.upload(x, y, z, r, d.charge)
+    batch = args.workload == "c4"
+    iters_fn = None
+    if batch:
+        # 512 molecules of 20..800 atoms (log-uniform), lmax 6, 302 points
+        decks = inputs.batch_inputs()
+        d = decks[0]
+        h = DDCosmoBatch.from_inputs(decks)
+        b = [dk.bohr() for dk in decks]
+        h.upload(*[np.concatenate([v[k] for v in b]) for k in range(4)],
+                 np.concatenate([dk.charge for dk in decks]))
+
+        def iters_fn():
+            nd, na = h.iterations()
+            sz = np.diff(h.sph_off)
+            cz = np.diff(h.cav_off)
+            return list(zip(sz.tolist(), cz.tolist(), nd.tolist(), na.tolist()))
+        tag = "c4batch"
+    else:
+        d = inputs.config_input(args.workload)
+        x, y, z, r = d.bohr()
+        h = DDCosmo.from_input(d)
+        h.upload(x, y, z, r, d.charge)
+        tag = {"c2": "2000atoms", "c3": "20000atoms"}.get(args.workload, args.workload)
     for _ in range(args.warmup):
         h.run()
     # which kernel class dominates one solve
-    table = kernel_table(h, h.stats(), d)
+    table = kernel_table(h, iters_fn)
     dominant = max(table, key=lambda k: table[k]["ms"])
     # timed region; the dominant class is bracketed by HIP events on the
     # library's stream, launch by launch
@@ -208,7 +263,7 @@ def main():
     es, fx = h.result()
     st = h.stats()
     sec = (t1 - t0) / args.steps
-    work = algorithmic_work(h, st, d)
+    work = algorithmic_work(h, st, iters_fn() if iters_fn else None)
     roofline = None
     if dominant in work and launches > 0:
         unit, w, per_solve = work[dominant]
@@ -222,7 +277,7 @@ def main():
             ach = per_launch / avg_s / 1e9
             roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": ach / HBM_PEAK_GBS, "traffic": None}
-        roofline["traffic"] = traffic_from_profiles(dominant)
+        roofline["traffic"] = traffic_from_profiles(dominant, tag)
         roofline.update({"kernel": dominant, "launches": launches,
                          "avg_launch_us": avg_s * 1e6, "work_per_launch": per_launch,
                          "work_unit": unit})
@@ -236,26 +291,44 @@ def main():
                       "iters_match": bool(st.niter_direct == int(g["niter_direct"]) and
                                           st.niter_adjoint == int(g["niter_adjoint"]))}
 
+    if batch:
+        nd, na = h.iterations()
+        config = {"workload": "c4: %d synthetic molecules of %d..%d atoms (%d in all), lmax %d, "
+                              "%d Lebedev points, eps %.2f, eta %.2f, conv 1e-%d, energy + forces, "
+                              "one batched handle" % (h.count, int(np.diff(h.sph_off).min()),
+                                                      int(np.diff(h.sph_off).max()), h.nsph,
+                                                      d.lmax, h.ngrid, d.eps, d.eta, d.iconv),
+                  "precision": args.precision, "nsph": h.nsph, "ncav": h.ncav,
+                  "ntriples": st.ntriples, "molecules": h.count}
+        iterations = {"direct": [int(nd.min()), int(nd.max())],
+                      "adjoint": [int(na.min()), int(na.max())]}
+        value = h.count / sec
+    else:
+        config = {"workload": "%s: %d synthetic atoms, lmax %d, %d Lebedev points, eps %.2f, "
+                              "eta %.2f, conv 1e-%d, energy + forces" % (
+                                  args.workload, d.n, d.lmax, h.ngrid, d.eps, d.eta, d.iconv),
+                  "precision": args.precision, "nsph": d.n, "ncav": h.ncav,
+                  "ntriples": st.ntriples}
+        iterations = {"direct": st.niter_direct, "adjoint": st.niter_adjoint}
+        value = 1.0 / sec
     out = {
-        "metric": METRIC, "value": 1.0 / sec, "unit": "solves/s", "n_gpus": 1,
+        "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "%s: %d synthetic atoms, lmax %d, %d Lebedev points, eps %.2f, "
-                               "eta %.2f, conv 1e-%d, energy + forces" % (
-                                   args.workload, d.n, d.lmax, h.ngrid, d.eps, d.eta, d.iconv),
-                   "precision": args.precision, "nsph": d.n, "ncav": h.ncav,
-                   "ntriples": st.ntriples},
+        "config": config,
         "stages_ms": {"setup": st.ms_setup, "rhs": st.ms_rhs,
                       "solves": st.ms_direct + st.ms_adjoint, "forces": st.ms_forces},
-        "iterations": {"direct": st.niter_direct, "adjoint": st.niter_adjoint},
-        "energy_hartree": es, "parity_vs_reference": parity,
+        "iterations": iterations,
+        "energy_hartree": es if not batch else float(np.sum(es)),
+        "parity_vs_reference": parity,
         "roofline": roofline,
         "kernels": table,
         "hbm_gbs": {k: table[k].get("gbs") for k in ("jacobi", "extrap")},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(d, args.cpu_cores)
+        out["cpu_baseline"] = (cpu_baseline_batch(decks, args.cpu_cores) if batch
+                               else cpu_baseline(d, args.cpu_cores))
     print(json.dumps(out))
 
 
