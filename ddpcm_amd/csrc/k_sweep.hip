@@ -331,7 +331,9 @@ __device__ __forceinline__ void sweep_chunk(
 // One wave per worker (blockIdx.x of G): units floor(w U / G) ..
 // floor((w+1) U / G) - 1, container by container.  DIR: L X terms (rw,
 // slot order); ADJ: one partial L* S per segment (yp).  A side whose solve
-// has ended (gate done) is skipped.
+// has ended (gate done) is skipped; in a batch (inst: the instance of each
+// sphere, gates: the summaries after ninst instance states) so is every
+// container of an instance that has converged.
 // (two waves per SIMD: the register budget that keeps a block of
 // accumulators and the two-deep load pipeline without spilling)
 template <int LMAX, bool DIR, bool ADJ>
@@ -342,7 +344,7 @@ __global__ __launch_bounds__(kWave, 2) void k_sweep(
     const int *__restrict__ tslot, const int *__restrict__ tp, const double *__restrict__ Xs,
     double *__restrict__ rw, const double *__restrict__ xiw, const double *__restrict__ cf,
     double *__restrict__ yp, const DiisState *__restrict__ gdir,
-    const DiisState *__restrict__ gadj) {
+    const DiisState *__restrict__ gadj, const int *__restrict__ inst, int ninst) {
   const bool dir = DIR && !(gdir && gdir->done);
   const bool adj = ADJ && !(gadj && gadj->done);
   if (!dir && !adj) return;
@@ -374,17 +376,25 @@ __global__ __launch_bounds__(kWave, 2) void k_sweep(
     const int beg = cb + (u - uoff[c]) * kWave;
     const int end = min(ce, cb + (ue - uoff[c]) * kWave);
     double *ys = yp + (size_t)seg * NY;
+    bool dc = dir, ac = adj;
+    if (inst) {
+      const int q = inst[c] - ninst;   // the instance's state, before the summary
+      dc = dc && !(gdir && gdir[q].done);
+      ac = ac && !(gadj && gadj[q].done);
+    }
     if constexpr (DIR && ADJ) {
-      if (dir && adj)
+      if (dc && ac)
         sweep_chunk<LMAX, true, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
-      else if (dir)
+      else if (dc)
         sweep_chunk<LMAX, true, false>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
-      else
+      else if (ac)
         sweep_chunk<LMAX, false, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
     } else if constexpr (DIR) {
-      sweep_chunk<LMAX, true, false>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
+      if (dc)
+        sweep_chunk<LMAX, true, false>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
     } else {
-      sweep_chunk<LMAX, false, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
+      if (ac)
+        sweep_chunk<LMAX, false, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
     }
     u = ue;
     ++seg;
@@ -435,7 +445,7 @@ int launch_sweep(ddx_handle *h, hipStream_t s, const double *Xs, double *rw, con
   hipExtLaunchKernelGGL((k_sweep<LMAX, DIR, ADJ>), dim3(h->nworker), dim3(kWave), 0, s, tl.b,
                         tl.e, 0, (long long)h->nunit, (long long)h->nworker, h->uoff.p, h->segoff.p, h->inl.p,
                      h->pairoff.p, h->n, h->tgeo.p, h->ttw.p, h->tslot.p, h->tp.p, Xs, rw, xiw,
-                     h->cf.p, yp, gdir, gadj);
+                     h->cf.p, yp, gdir, gadj, h->ninst > 1 ? h->inst.p : nullptr, h->ninst);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
