@@ -28,7 +28,7 @@ EXPORTS = (
     "ddx_get_history", "ddx_energy_forces", "ddx_upload", "ddx_run", "ddx_result",
     "ddx_set_precision", "ddx_get_stats", "ddx_kernel_timer", "ddx_kernel_timer_read",
     "ddx_batch_init", "ddx_batch_energy_forces", "ddx_batch_result", "ddx_batch_iterations",
-    "ddx_batch_sizes",
+    "ddx_batch_sizes", "ddx_gemm_bf16",
 )
 KERNELS = ("sweep_l", "sweep_lstar", "project", "expand", "jacobi", "diis", "extrap", "field",
            "potential", "forces", "sweep_pair")
@@ -97,6 +97,8 @@ def load_library(path: str = LIB_PATH):
     L.ddx_batch_result.argtypes = [C.c_void_p, _dp, _dp]
     L.ddx_batch_iterations.argtypes = [C.c_void_p, _ip, _ip]
     L.ddx_batch_sizes.argtypes = [C.c_void_p, _ip, _ip, _ip]
+    _u16 = C.POINTER(C.c_uint16)
+    L.ddx_gemm_bf16.argtypes = [C.c_int, C.c_int, C.c_int, _u16, _u16, C.POINTER(C.c_float)]
     _LIB = L
     return L
 
@@ -368,3 +370,32 @@ class DDCosmoBatch(DDCosmo):
     def split(self, fx):
         """Per-instance views of a (nsph, 3) array."""
         return [fx[self.sph_off[q]:self.sph_off[q + 1]] for q in range(self.count)]
+
+
+def bf16_bits(x):
+    """float array -> BF16 bit patterns (round to nearest even) as uint16."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def bf16_value(b):
+    """BF16 bit patterns -> float32 values."""
+    return (np.ascontiguousarray(b, dtype=np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def gemm_bf16(a_bits, b_bits):
+    """C = A B on the device's BF16 product kernel (FP32 result)."""
+    L = load_library()
+    a = np.ascontiguousarray(a_bits, dtype=np.uint16)
+    b = np.ascontiguousarray(b_bits, dtype=np.uint16)
+    M, K = a.shape
+    K2, N = b.shape
+    assert K == K2
+    c = np.zeros((M, N), dtype=np.float32)
+    rc = L.ddx_gemm_bf16(M, N, K, a.ctypes.data_as(C.POINTER(C.c_uint16)),
+                         b.ctypes.data_as(C.POINTER(C.c_uint16)),
+                         c.ctypes.data_as(C.POINTER(C.c_float)))
+    if rc:
+        raise DDXError(rc, L.ddx_last_error().decode())
+    return c

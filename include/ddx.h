@@ -157,6 +157,13 @@ int ddx_batch_iterations(const ddx_handle *h, int *direct, int *adjoint);
 int ddx_batch_sizes(const ddx_handle *h, int *count, int *sph_off, int *cav_off);
 
 int ddx_set_precision(ddx_handle *h, int prec);
+
+/* The BF16 product kernel of the learned guess on its own (no counterpart
+ * in the reference; for its tests): C (M x N, FP32, row-major) = A (M x K)
+ * B (K x N), A and B row-major BF16 given as raw 16-bit patterns, FP32
+ * accumulation. */
+int ddx_gemm_bf16(int M, int N, int K, const unsigned short *a, const unsigned short *b,
+                  float *c);
 int ddx_get_stats(const ddx_handle *h, ddx_stats *st);
 
 /* Kernel timing (instrumentation; no counterpart in the reference).  While
