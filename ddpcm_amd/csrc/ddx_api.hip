@@ -65,15 +65,29 @@ int set_geometry(ddx_handle *h, const double *x, const double *y, const double *
 
 // Start of the direct solve L X = g (rhs in h->rhs, X in h->sigma) on a lane,
 // from the guess D^-1 rhs (cosmo.f90:136-138).
+// The guess of solve `star` for right-hand side v into x: D^-1 v as the
+// reference (cosmo.f90:136-138,174-176), or the handle's previous solution,
+// or the learned guess (ddx_set_guess).
+int initial_guess(ddx_handle *h, int star, const double *v, double *x) {
+  const bool prev = star ? h->have_s : h->have_sigma;
+  if (h->guess_mode == DDX_GUESS_PREVIOUS && prev) return DDX_OK;
+  if (h->guess_mode == DDX_GUESS_LEARNED) return learned_guess(h, star, v, x, false);
+  if (h->guess_mode == DDX_GUESS_LEARNED_PREVIOUS) {
+    if (!prev) DDX_TRY(device_ldm1x(h, h->stream, v, x));
+    return learned_guess(h, star, v, x, prev);
+  }
+  return device_ldm1x(h, h->stream, v, x);
+}
+
 int begin_direct(ddx_handle *h, Lane &ln) {
-  DDX_TRY(device_ldm1x(h, ln.stream, h->rhs.p, h->sigma.p));
+  DDX_TRY(initial_guess(h, 0, h->rhs.p, h->sigma.p));
   return solve_begin(h, ln, 0, kDiisMax, tol_of(h->iconv), h->rhs.p, h->sigma.p, kMaxIter);
 }
 
 // Start of the adjoint solve L* S = psi (S in h->s) from D^-1 psi
 // (cosmo.f90:174-176).
 int begin_adjoint(ddx_handle *h, Lane &ln, const double *d_psi) {
-  DDX_TRY(device_ldm1x(h, ln.stream, d_psi, h->s.p));
+  DDX_TRY(initial_guess(h, 1, d_psi, h->s.p));
   return solve_begin(h, ln, 1, kDiisMax, tol_of(h->iconv), d_psi, h->s.p, kMaxIter);
 }
 

@@ -87,6 +87,25 @@ struct DiisState {
   double hist[kMaxIter + 1];
 };
 
+// The network of the learned guess (k_guess.hip): in -> 128 -> 128 -> out,
+// widths padded to multiples of 16.  FP32 master weights with Adam moments,
+// BF16 copies for the products.
+constexpr int kGuessHidden = 128;
+constexpr int kGuessGeo = 16;      // neighbourhood features per sphere
+struct GuessNet {
+  int in = 0, inp = 0, out = 0, outp = 0;
+  DevArray<float> w[3], b[3], mw[3], vw[3], mb[3], vb[3], gw[3], gb[3];
+  DevArray<unsigned short> wb[3];
+  int t = 0;                 // Adam steps taken
+  bool trained = false;
+};
+// Workspace of the network for n rows (spheres)
+struct GuessWork {
+  DevArray<unsigned short> F, H1, H2, dYb, dZb;
+  DevArray<float> Z1, Z2, Y, T, dY, dH, dZ;
+  DevArray<double> scale, part;
+};
+
 // Event pairs around the launches of one kernel class (ddx_kernel_timer).
 // A launch made for a Jacobi/DIIS solve carries the solve's serial number
 // and round, so that rounds enqueued after convergence can be dropped.
@@ -204,6 +223,15 @@ struct ddx_handle {
   // problem data / results
   ddx::DevArray<double> phi, psi, sigma, s, xi, fx, zeta, ef, energy, fpart;
   ddx::DevArray<int> scan_tmp, misc;
+
+  // initial guess of the solves (DDX_GUESS_*) and the learned guess's
+  // networks for the direct (0) and adjoint (1) solve
+  int guess_mode = 0;
+  ddx::GuessNet gnet[2];
+  ddx::GuessWork gwork;
+  unsigned guess_seed = 1;
+  ddx::DevArray<double> gprev[2];     // the learned map of the last right-hand side
+  bool gprev_ok[2] = {false, false};
 
   std::vector<double> hist_direct, hist_adjoint;
   std::vector<int> iters_direct, iters_adjoint;   // per instance

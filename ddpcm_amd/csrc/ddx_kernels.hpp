@@ -90,4 +90,12 @@ int field_forces(ddx_handle *h, const double *d_charge, const double *d_zeta,
 // size of h->ef that field_forces uses
 size_t field_buffer_size(const ddx_handle *h);
 
+// k_guess.hip — the learned initial guess
+int gemm_bf16(hipStream_t s, int M, int N, int K, const unsigned short *a, long ai, long ak,
+              const unsigned short *b, long bk, long bj, float *c, long ldc, bool accumulate);
+int convert_bf16(hipStream_t s, long n, const float *x, unsigned short *y);
+int learned_guess(ddx_handle *h, int star, const double *v, double *x, bool delta);
+int train_guess(ddx_handle *h, int star, const double *v, const double *x, int steps, double lr,
+                double *loss);
+
 }  // namespace ddx

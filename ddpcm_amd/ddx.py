@@ -28,8 +28,10 @@ EXPORTS = (
     "ddx_get_history", "ddx_energy_forces", "ddx_upload", "ddx_run", "ddx_result",
     "ddx_set_precision", "ddx_get_stats", "ddx_kernel_timer", "ddx_kernel_timer_read",
     "ddx_batch_init", "ddx_batch_energy_forces", "ddx_batch_result", "ddx_batch_iterations",
-    "ddx_batch_sizes", "ddx_gemm_bf16",
+    "ddx_batch_sizes", "ddx_gemm_bf16", "ddx_set_guess", "ddx_guess_train", "ddx_guess_reset",
+    "ddx_mlp_grad",
 )
+DDX_GUESS_REFERENCE, DDX_GUESS_PREVIOUS, DDX_GUESS_LEARNED, DDX_GUESS_LEARNED_PREVIOUS = 0, 1, 2, 3
 KERNELS = ("sweep_l", "sweep_lstar", "project", "expand", "jacobi", "diis", "extrap", "field",
            "potential", "forces", "sweep_pair")
 
@@ -99,6 +101,12 @@ def load_library(path: str = LIB_PATH):
     L.ddx_batch_sizes.argtypes = [C.c_void_p, _ip, _ip, _ip]
     _u16 = C.POINTER(C.c_uint16)
     L.ddx_gemm_bf16.argtypes = [C.c_int, C.c_int, C.c_int, _u16, _u16, C.POINTER(C.c_float)]
+    L.ddx_set_guess.argtypes = [C.c_void_p, C.c_int]
+    L.ddx_guess_train.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp]
+    L.ddx_guess_reset.argtypes = [C.c_void_p, C.c_uint]
+    _fp = C.POINTER(C.c_float)
+    L.ddx_mlp_grad.argtypes = [C.c_int, C.c_int, C.c_int, _u16, _fp, C.POINTER(_fp), _fp,
+                               C.POINTER(_fp), _dp]
     _LIB = L
     return L
 
@@ -305,6 +313,21 @@ class DDCosmo:
     def set_precision(self, prec):
         self._check(self.L.ddx_set_precision(self.h, prec))
 
+    # ---- initial guess (no counterpart in the reference) ----------------------
+    def set_guess(self, mode):
+        """DDX_GUESS_REFERENCE (D^-1 rhs), _PREVIOUS or _LEARNED."""
+        self._check(self.L.ddx_set_guess(self.h, int(mode)))
+
+    def guess_train(self, steps=4, lr=1e-3):
+        """Adam steps of the guess networks towards the last solve's solutions;
+        returns their losses (direct, adjoint)."""
+        loss = np.zeros(2)
+        self._check(self.L.ddx_guess_train(self.h, int(steps), float(lr), _p(loss)))
+        return float(loss[0]), float(loss[1])
+
+    def guess_reset(self, seed=1):
+        self._check(self.L.ddx_guess_reset(self.h, int(seed)))
+
 
 class DDCosmoBatch(DDCosmo):
     """A batch of independent ddCOSMO systems on one handle (no counterpart in
@@ -399,3 +422,26 @@ def gemm_bf16(a_bits, b_bits):
     if rc:
         raise DDXError(rc, L.ddx_last_error().decode())
     return c
+
+
+def mlp_grad(F_bits, T, weights):
+    """One forward/backward pass of the guess network on the device:
+    F_bits (n, in) BF16 patterns, T (n, out), weights [W1, b1, W2, b2, W3, b3]
+    (FP32, W row-major (in, out)); returns (Y, grads like weights, loss)."""
+    L = load_library()
+    F = np.ascontiguousarray(F_bits, dtype=np.uint16)
+    T = np.ascontiguousarray(T, dtype=np.float32)
+    n, nin = F.shape
+    nout = T.shape[1]
+    ws = [np.ascontiguousarray(w, dtype=np.float32) for w in weights]
+    gs = [np.zeros_like(w) for w in ws]
+    fp = C.POINTER(C.c_float)
+    warr = (fp * 6)(*[w.ctypes.data_as(fp) for w in ws])
+    garr = (fp * 6)(*[g.ctypes.data_as(fp) for g in gs])
+    Y = np.zeros((n, nout), dtype=np.float32)
+    loss = C.c_double()
+    rc = L.ddx_mlp_grad(n, nin, nout, F.ctypes.data_as(C.POINTER(C.c_uint16)),
+                        T.ctypes.data_as(fp), warr, Y.ctypes.data_as(fp), garr, C.byref(loss))
+    if rc:
+        raise DDXError(rc, L.ddx_last_error().decode())
+    return Y, gs, loss.value

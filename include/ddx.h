@@ -158,6 +158,44 @@ int ddx_batch_sizes(const ddx_handle *h, int *count, int *sph_off, int *cav_off)
 
 int ddx_set_precision(ddx_handle *h, int prec);
 
+/* Initial guess of the solves (no counterpart in the reference beyond
+ * DDX_GUESS_REFERENCE, which it always uses: cosmo.f90:136-138,174-176).
+ *   DDX_GUESS_REFERENCE  D^-1 rhs
+ *   DDX_GUESS_PREVIOUS   the handle's last solution of the same solve (the
+ *                        previous frame of a trajectory); D^-1 rhs before
+ *                        the first solve
+ *   DDX_GUESS_LEARNED    D^-1 rhs + the correction of a per-sphere network
+ *                        (inputs: D^-1 rhs of the sphere and 16 features of
+ *                        its neighbourhood; two hidden layers of 128; BF16
+ *                        weights, FP32 accumulation on MFMA), trained on the
+ *                        handle's own converged solutions by
+ *                        ddx_guess_train; D^-1 rhs until it is trained
+ *   DDX_GUESS_LEARNED_PREVIOUS  the previous solution plus the change of the
+ *                        network's map between the previous and the current
+ *                        right-hand side (the previous solution alone until
+ *                        the network is trained)
+ * The Jacobi/DIIS convergence test is unchanged: the guess changes the
+ * iteration count, not the tolerance.
+ * ddx_guess_train: `steps` Adam steps (learning rate lr) of the direct and,
+ * with forces, the adjoint network towards the last solve's solutions;
+ * loss[2] = their mean squared errors (in units of the rms of D^-1 rhs).
+ * ddx_guess_reset: fresh networks from `seed`. */
+#define DDX_GUESS_REFERENCE 0
+#define DDX_GUESS_PREVIOUS 1
+#define DDX_GUESS_LEARNED 2
+#define DDX_GUESS_LEARNED_PREVIOUS 3
+int ddx_set_guess(ddx_handle *h, int mode);
+int ddx_guess_train(ddx_handle *h, int steps, double lr, double *loss);
+int ddx_guess_reset(ddx_handle *h, unsigned seed);
+
+/* One forward and backward pass of the guess network with weights given in
+ * host memory (no counterpart in the reference; for the tests of the
+ * network's kernels against a CPU restatement).  F (n x in) BF16 bits; T (n x
+ * out); wts = {W1 (in x 128), b1, W2 (128 x 128), b2, W3 (128 x out), b3}
+ * FP32 row-major; returns Y (n x out), grads in the layouts of wts, loss. */
+int ddx_mlp_grad(int n, int in, int out, const unsigned short *F, const float *T,
+                 const float *const *wts, float *Y, float *const *grads, double *loss);
+
 /* The BF16 product kernel of the learned guess on its own (no counterpart
  * in the reference; for its tests): C (M x N, FP32, row-major) = A (M x K)
  * B (K x N), A and B row-major BF16 given as raw 16-bit patterns, FP32
