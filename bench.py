@@ -1,6 +1,6 @@
 """Energy + forces ddCOSMO solves per second on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload c2|c3|c4] [--precision fp64]
+    python bench.py [--steps K] [--warmup W] [--workload c2|c3|c4|c5] [--precision fp64]
 
 One step is one complete solve from coordinates already resident in HBM:
 device setup (neighbours, fi/ui/zi, cavity, triple lists), RHS (solute
@@ -9,7 +9,9 @@ energy, adjoint solve, analytic forces with zeta and both field terms.  The
 workload is the 2000-atom synthetic system (lmax 8, 590 Lebedev points,
 eps 78.39, eta 0.1, convergence 1e-8); c3 is the 20000-atom one (lmax 10,
 1202 points), c4 a batch of 512 molecules of 20..800 atoms (lmax 6, 302
-points) solved on one handle, one step being the whole batch.
+points) solved on one handle, one step being the whole batch; c5 a 1000-atom
+trajectory timed under each initial-guess mode (the learned guess trained on
+the first frames).
 
 Beside the GPU figure the reference's own OpenMP Fortran program (compiled
 from its sources into oracle/_ref/main.exe) is timed on the host cores on
@@ -199,12 +201,77 @@ def traffic_from_profiles(kernel, tag):
     return None
 
 
+def trajectory_bench(args):
+    """c5: one 1000-atom system (lmax 8, 590 points) along a trajectory of
+    Gaussian steps (0.05 angstrom per atom and frame).  Each guess mode runs
+    the same frames; the learned modes train on the first `--train` frames'
+    solutions (8 Adam steps per frame) and all modes are timed on the frames
+    after them.  value: solves/s of the learned guess."""
+    from ddpcm_amd import inputs
+    from ddpcm_amd.ddx import (DDCosmo, DDX_GUESS_LEARNED, DDX_GUESS_LEARNED_PREVIOUS,
+                               DDX_GUESS_PREVIOUS, DDX_GUESS_REFERENCE)
+    ntrain, neval = args.train, args.steps
+    frames = list(inputs.trajectory(n=1000, frames=ntrain + neval, seed=1000))
+    tb = 1.0 / inputs.TOANG
+    d = inputs.synthetic_input(1000, 1000, 8, 590)
+    modes = {}
+    for name, mode, train in [("reference", DDX_GUESS_REFERENCE, 0),
+                              ("previous", DDX_GUESS_PREVIOUS, 0),
+                              ("learned", DDX_GUESS_LEARNED, ntrain),
+                              ("learned_previous", DDX_GUESS_LEARNED_PREVIOUS, ntrain)]:
+        q, xyz, r = frames[0]
+        h = DDCosmo(xyz[:, 0] * tb, xyz[:, 1] * tb, xyz[:, 2] * tb, r * tb, lmax=8, ngrid=590)
+        h.set_guess(mode)
+        its, t_eval, es = [], 0.0, []
+        for f, (q, xyz, r) in enumerate(frames):
+            h.upload(xyz[:, 0] * tb, xyz[:, 1] * tb, xyz[:, 2] * tb, r * tb, q)
+            if f >= ntrain:
+                t0 = time.perf_counter()
+                h.run()
+                t_eval += time.perf_counter() - t0
+                st = h.stats()
+                its.append((st.niter_direct, st.niter_adjoint))
+                es.append(h.result()[0])
+            else:
+                h.run()
+                if train:
+                    h.guess_train(steps=8, lr=2e-3)
+        its = np.array(its)
+        modes[name] = {"solves_per_s": neval / t_eval, "ms_per_solve": t_eval / neval * 1e3,
+                       "iterations_direct": float(its[:, 0].mean()),
+                       "iterations_adjoint": float(its[:, 1].mean()), "energies": es}
+    ref_e = np.array(modes["reference"]["energies"])
+    for m in modes.values():
+        m["max_rel_energy_diff_vs_reference_guess"] = float(
+            np.max(np.abs(np.array(m.pop("energies")) - ref_e) / np.abs(ref_e)))
+    lr_ = modes["learned"]
+    out = {
+        "metric": METRIC, "value": lr_["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+        "steps": neval, "warmup": ntrain, "ms_per_step": lr_["ms_per_solve"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "c5: 1000 synthetic atoms, lmax 8, 590 Lebedev points, eps 78.39, "
+                               "eta 0.10, conv 1e-8, energy + forces, trajectory of 0.05 A "
+                               "Gaussian steps; %d training frames (warmup), %d timed frames"
+                               % (ntrain, neval), "precision": args.precision, "nsph": 1000},
+        "guess_modes": modes,
+        "roofline": None,
+    }
+    if not args.no_cpu_baseline:
+        q, xyz, r = frames[ntrain]
+        dd = inputs.DDInput(0, 1, 8, 590, 8, 1, 78.39, 0.1, q, xyz, r)
+        out["cpu_baseline"] = cpu_baseline(dd, args.cpu_cores)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2")
+    ap.add_argument("--train", type=int, default=200,
+                    help="c5: frames whose solutions train the guess network")
     ap.add_argument("--precision", default="fp64", choices=["fp64"])
     ap.add_argument("--cpu-cores", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -220,6 +287,9 @@ def main():
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "c5":
+        trajectory_bench(args)
+        return
     from ddpcm_amd import inputs
     from ddpcm_amd.ddx import DDCosmo, DDCosmoBatch
 
