@@ -212,6 +212,9 @@ struct ddx_handle {
   ddx::DevArray<int> jboff, eboff, fgrp, soff;
   ddx::DevArray<int4> xtile;      // tiles of a field between arbitrary point sets
   ddx::DevArray<int> xgrp;
+  std::vector<int4> h_ctile, h_ftile, h_xtile;   // host copies of the tile tables
+  std::vector<int> h_fgrp, h_xgrp;
+  int maps_n = -1;                // sphere count the block maps were built for
 
   // solver workspace: lane 0 runs on `stream`, lane 1 on a stream of its own
   ddx::Lane lane[2];

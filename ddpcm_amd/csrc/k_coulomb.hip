@@ -205,21 +205,21 @@ int upload_tiles(ddx_handle *h, DevArray<int4> &dt, const std::vector<int4> &t) 
 }  // namespace
 
 int build_coulomb_tiles(ddx_handle *h) {
-  std::vector<int4> t;
+  // the host tables live in the handle until the next setup, so the copies
+  // need no wait here
   std::vector<int> g;
   // potential (and solute field) at the cavity points from the spheres
-  coulomb_tiles(h->h_cavoff, h->h_soff, false, t, g);
-  h->nctile = (int)t.size();
-  DDX_TRY(upload_tiles(h, h->ctile, t));
+  coulomb_tiles(h->h_cavoff, h->h_soff, false, h->h_ctile, g);
+  h->nctile = (int)h->h_ctile.size();
+  DDX_TRY(upload_tiles(h, h->ctile, h->h_ctile));
   // field of the cavity charges at the nuclei
-  coulomb_tiles(h->h_soff, h->h_cavoff, true, t, g);
-  h->nftile = (int)t.size();
-  h->nfgrp = (int)g.size() - 1;
-  DDX_TRY(upload_tiles(h, h->ftile, t));
-  DDX_TRY(h->fgrp.reserve(g.size()));
-  DDX_HIP(hipMemcpyAsync(h->fgrp.p, g.data(), g.size() * sizeof(int), hipMemcpyHostToDevice,
-                         h->stream));
-  DDX_HIP(hipStreamSynchronize(h->stream));
+  coulomb_tiles(h->h_soff, h->h_cavoff, true, h->h_ftile, h->h_fgrp);
+  h->nftile = (int)h->h_ftile.size();
+  h->nfgrp = (int)h->h_fgrp.size() - 1;
+  DDX_TRY(upload_tiles(h, h->ftile, h->h_ftile));
+  DDX_TRY(h->fgrp.reserve(h->h_fgrp.size()));
+  DDX_HIP(hipMemcpyAsync(h->fgrp.p, h->h_fgrp.data(), h->h_fgrp.size() * sizeof(int),
+                         hipMemcpyHostToDevice, h->stream));
   return DDX_OK;
 }
 
@@ -257,8 +257,9 @@ int coulomb_field(ddx_handle *h, const int4 *tiles, int ntile, const int *grp, i
 int coulomb_field_points(ddx_handle *h, int nsrc, const double *q, const double *src3, int ntrg,
                          const double *trg3, double *ef) {
   if (ntrg <= 0) return DDX_OK;
-  std::vector<int4> t;
-  std::vector<int> g;
+  std::vector<int4> &t = h->h_xtile;
+  std::vector<int> &g = h->h_xgrp;
+  DDX_HIP(hipStreamSynchronize(h->stream));   // the previous tables' copies are done
   coulomb_tiles({0, ntrg}, {0, nsrc}, true, t, g);
   DDX_TRY(upload_tiles(h, h->xtile, t));
   DDX_TRY(h->xgrp.reserve(g.size()));

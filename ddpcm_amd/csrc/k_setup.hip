@@ -84,11 +84,19 @@ __global__ __launch_bounds__(kSetupThreads) void k_nbr_find(
   if (i < n) {
     const double4 si = sph[i];
     const int ki = inst ? inst[i] : 0;
-    for (int d = 0; d < 27; ++d) {
-      const unsigned b = sphere_bucket(si, ki, inv_h, mask, d);
-      bool dup = false;   // a bucket that an earlier cube maps to is done
-      for (int e = 0; e < d; ++e) dup |= sphere_bucket(si, ki, inv_h, mask, e) == b;
-      if (dup) continue;
+    // lane d < 27 takes cube d; a bucket that an earlier cube maps to is
+    // left out
+    const unsigned myb = lane < 27 ? sphere_bucket(si, ki, inv_h, mask, lane) : 0xffffffffu;
+    bool dup = false;
+    for (int e = 0; e < 27; ++e) {
+      const unsigned be = (unsigned)__shfl((int)myb, e, kWave);
+      dup = dup || (e < lane && be == myb);
+    }
+    unsigned long long todo = __ballot(lane < 27 && !dup);
+    while (todo) {
+      const int d = __ffsll(todo) - 1;
+      todo &= todo - 1;
+      const unsigned b = (unsigned)__shfl((int)myb, d, kWave);
       const int beg = boff[b], end = boff[b + 1];
       for (int k0 = beg; k0 < end; k0 += kWave) {
         const int k = k0 + lane;
@@ -444,19 +452,27 @@ int run_setup(ddx_handle *h) {
   hipLaunchKernelGGL(k_ccav, pg, dim3(kSetupThreads), 0, s, h->sph.p, h->grid.p, ng,
                      h->cavflag.p, h->cavidx.p, h->ccav.p);
   DDX_HIP(hipGetLastError());
-  // instance offsets of the cavity, the solver's block maps, Coulomb tiles
-  DDX_TRY(h->soff.reserve(h->ninst + 1));
-  DDX_TRY(h->cand.reserve((size_t)h->ninst + 1));
-  DDX_HIP(hipMemcpyAsync(h->soff.p, h->h_soff.data(), (h->ninst + 1) * sizeof(int),
-                         hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_cav_offsets, dim3((h->ninst + 256) / 256), dim3(256), 0, s, h->ninst, ng,
-                     h->soff.p, h->cavidx.p, h->cand.p);
-  DDX_HIP(hipGetLastError());
-  h->h_cavoff.resize(h->ninst + 1);
-  DDX_HIP(hipMemcpyAsync(h->h_cavoff.data(), h->cand.p, (h->ninst + 1) * sizeof(int),
-                         hipMemcpyDeviceToHost, s));
-  DDX_HIP(hipStreamSynchronize(s));
-  DDX_TRY(build_solver_maps(h));
+  // instance offsets of the cavity, the solver's block maps (once per
+  // handle: they depend on the sphere counts only), Coulomb tiles
+  if (h->maps_n != n) {
+    DDX_TRY(h->soff.reserve(h->ninst + 1));
+    DDX_HIP(hipMemcpyAsync(h->soff.p, h->h_soff.data(), (h->ninst + 1) * sizeof(int),
+                           hipMemcpyHostToDevice, s));
+    DDX_TRY(build_solver_maps(h));
+    h->maps_n = n;
+  }
+  if (h->ninst == 1) {
+    h->h_cavoff.assign({0, ncav});
+  } else {
+    DDX_TRY(h->cand.reserve((size_t)h->ninst + 1));
+    hipLaunchKernelGGL(k_cav_offsets, dim3((h->ninst + 256) / 256), dim3(256), 0, s, h->ninst,
+                       ng, h->soff.p, h->cavidx.p, h->cand.p);
+    DDX_HIP(hipGetLastError());
+    h->h_cavoff.resize(h->ninst + 1);
+    DDX_HIP(hipMemcpyAsync(h->h_cavoff.data(), h->cand.p, (h->ninst + 1) * sizeof(int),
+                           hipMemcpyDeviceToHost, s));
+    DDX_HIP(hipStreamSynchronize(s));
+  }
   DDX_TRY(build_coulomb_tiles(h));
   h->have_setup = true;
   h->stats.ntriples = ntri;

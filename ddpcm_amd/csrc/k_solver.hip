@@ -700,7 +700,7 @@ int build_solver_maps(ddx_handle *h) {
   DDX_HIP(hipMemcpyAsync(h->eblk.p, eb.data(), eb.size() * sizeof(int4), hipMemcpyHostToDevice, s));
   DDX_HIP(hipMemcpyAsync(h->jboff.p, jo.data(), jo.size() * sizeof(int), hipMemcpyHostToDevice, s));
   DDX_HIP(hipMemcpyAsync(h->eboff.p, eo.data(), eo.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  DDX_HIP(hipStreamSynchronize(s));
+  DDX_HIP(hipStreamSynchronize(s));   // (built once per handle)
   return DDX_OK;
 }
 
