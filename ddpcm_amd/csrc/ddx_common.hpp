@@ -215,6 +215,7 @@ struct ddx_handle {
   std::vector<int4> h_ctile, h_ftile, h_xtile;   // host copies of the tile tables
   std::vector<int> h_fgrp, h_xgrp;
   int maps_n = -1;                // sphere count the block maps were built for
+  int ysplit = 1;                 // partial projections of L X in the solver
 
   // solver workspace: lane 0 runs on `stream`, lane 1 on a stream of its own
   ddx::Lane lane[2];

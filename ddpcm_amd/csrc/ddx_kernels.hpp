@@ -17,7 +17,9 @@ int run_setup(ddx_handle *h);
 // disjoint point sets at y + j nylm nsph (j < ysplit)
 int project(ddx_handle *h, hipStream_t s, double alpha, const double *P, double *y,
             const DiisState *gate, int ysplit);
-constexpr int kProjSplit = 4;   // partial projections of L X inside the solver
+// partial projections of L X inside the solver: up to kProjSplit, as many
+// as keep about 1024 workgroups busy (h->ysplit, set with the block maps)
+constexpr int kProjSplit = 4;
 // xi(n, i) = [w_n] sum_lm Y(lm, n) S(lm, i)
 int expand(ddx_handle *h, hipStream_t s, const double *S, double *xi, bool wscale,
            const DiisState *gate);

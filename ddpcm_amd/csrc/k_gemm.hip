@@ -37,9 +37,13 @@ __global__ __launch_bounds__(256) void k_project(int nylm, int ngrid, int nsph, 
   if (gate && gate->done) return;
   __shared__ v4d red[3][kMaxTiles][kWave];
   const int lane = lane_id(), wave = threadIdx.x / kWave;
-  // blockIdx.y splits the points: wave w of split ks takes k-steps
-  // 4 ks + w, + 4 gridDim.y, ... and the split's sum goes to y + ks nylm nsph
-  const int nw = 4 * gridDim.y, gw = 4 * blockIdx.y + wave;
+  // blockIdx.y splits the points into contiguous ranges of whole k-steps
+  // (so that each line of P is read by one workgroup); wave w of the split
+  // takes its k-steps w, w + 4, ... and the split's sum goes to
+  // y + blockIdx.y nylm nsph
+  const int nsteps = (ngrid + 3) / 4;
+  const int per = (nsteps + gridDim.y - 1) / gridDim.y;
+  const int kbeg = 4 * per * blockIdx.y, kend = min(ngrid, 4 * per * (int)(blockIdx.y + 1));
   y += (size_t)blockIdx.y * nylm * nsph;
   const int i0 = blockIdx.x * 16;
   const int col = lane & 15, krow = lane >> 4;
@@ -52,13 +56,13 @@ __global__ __launch_bounds__(256) void k_project(int nylm, int ngrid, int nsph, 
 #pragma unroll
     for (int t = 0; t < kMaxTiles; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
     // k-steps of 4 points each
-    for (int kb = 4 * gw; kb < ngrid; kb += 4 * nw * kUnroll) {
+    for (int kb = kbeg + 4 * wave; kb < kend; kb += 16 * kUnroll) {
       double b[kUnroll];
       double a[kUnroll][kMaxTiles];
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
-        const int n = kb + 4 * nw * u + krow;
-        const bool okn = n < ngrid;
+        const int n = kb + 16 * u + krow;
+        const bool okn = n < kend;
         b[u] = (okn && sph < nsph) ? Pc[n] : 0.0;
 #pragma unroll
         for (int t = 0; t < kMaxTiles; ++t) {

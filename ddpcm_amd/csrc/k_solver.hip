@@ -23,6 +23,8 @@
 #include "ddx_device.hpp"
 #include "ddx_kernels.hpp"
 
+#include <algorithm>
+
 namespace ddx {
 namespace {
 
@@ -691,6 +693,8 @@ int build_solver_maps(ddx_handle *h) {
   }
   h->nbj = (int)jb.size();
   h->nbe = (int)eb.size();
+  const int nblk = (h->n + 15) / 16;   // workgroups of one projection pass
+  h->ysplit = nblk >= 1024 ? 1 : std::min(kProjSplit, (1024 + nblk - 1) / nblk);
   DDX_TRY(h->jblk.reserve(jb.size()));
   DDX_TRY(h->eblk.reserve(eb.size()));
   DDX_TRY(h->jboff.reserve(jo.size()));
@@ -709,7 +713,7 @@ SysArgs sys_args(const ddx_handle *h, Lane &ln) {
   SysArgs a;
   a.rhs = ln.rhs;
   a.y = ln.y.p;
-  a.ny = ln.star ? 1 : kProjSplit;
+  a.ny = ln.star ? 1 : h->ysplit;
   a.yp = ln.star ? ln.yp.p : nullptr;
   a.segoff = h->segoff.p;
   a.x = ln.x;
@@ -809,11 +813,11 @@ int solve_enqueue(ddx_handle *h, Lane *const *lanes, int count, int slot) {
   for (int g = 0; g < l0.group && l0.enqueued < l0.rounds; ++g, ++l0.enqueued) {
     h->tag_round = l0.enqueued;
     if (pair) {
-      DDX_TRY(apply_pair(h, l0, l0.y.p, *lanes[1], lanes[1]->x, kProjSplit));
+      DDX_TRY(apply_pair(h, l0, l0.y.p, *lanes[1], lanes[1]->x, h->ysplit));
     } else if (l0.star) {
       DDX_TRY(apply_lstarx(h, l0, l0.x, l0.y.p, solve_gate(h, l0), false));
     } else {
-      DDX_TRY(apply_lx(h, l0, l0.x, l0.y.p, solve_gate(h, l0), true, kProjSplit));
+      DDX_TRY(apply_lx(h, l0, l0.x, l0.y.p, solve_gate(h, l0), true, h->ysplit));
     }
     {
       TimedLaunch tl(h, DDX_K_JACOBI, s);
