@@ -209,7 +209,7 @@ struct ddx_handle {
   // each target group).
   int nbj = 0, nbe = 0, nctile = 0, nftile = 0, nfgrp = 0;
   ddx::DevArray<int4> jblk, eblk, ctile, ftile;
-  ddx::DevArray<int> jboff, eboff, fgrp, soff;
+  ddx::DevArray<int> jboff, eboff, fgrp, soff, cavoff;
   ddx::DevArray<int4> xtile;      // tiles of a field between arbitrary point sets
   ddx::DevArray<int> xgrp;
   std::vector<int4> h_ctile, h_ftile, h_xtile;   // host copies of the tile tables

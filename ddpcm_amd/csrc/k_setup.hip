@@ -464,12 +464,12 @@ int run_setup(ddx_handle *h) {
   if (h->ninst == 1) {
     h->h_cavoff.assign({0, ncav});
   } else {
-    DDX_TRY(h->cand.reserve((size_t)h->ninst + 1));
+    DDX_TRY(h->cavoff.reserve((size_t)h->ninst + 1));
     hipLaunchKernelGGL(k_cav_offsets, dim3((h->ninst + 256) / 256), dim3(256), 0, s, h->ninst,
-                       ng, h->soff.p, h->cavidx.p, h->cand.p);
+                       ng, h->soff.p, h->cavidx.p, h->cavoff.p);
     DDX_HIP(hipGetLastError());
     h->h_cavoff.resize(h->ninst + 1);
-    DDX_HIP(hipMemcpyAsync(h->h_cavoff.data(), h->cand.p, (h->ninst + 1) * sizeof(int),
+    DDX_HIP(hipMemcpyAsync(h->h_cavoff.data(), h->cavoff.p, (h->ninst + 1) * sizeof(int),
                            hipMemcpyDeviceToHost, s));
     DDX_HIP(hipStreamSynchronize(s));
   }

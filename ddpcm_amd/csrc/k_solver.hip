@@ -667,11 +667,15 @@ int energy_dot(ddx_handle *h, const double *a, const double *b, double scale, do
 
 // Block maps of the solver kernels, from the instances' sphere ranges: each
 // instance's elements (spheres) are split evenly over
-// ceil(elements / 512) (ceil(spheres / 4)) workgroups, at most 1024 each.
+// ceil(elements / 512) (ceil(spheres / sp)) workgroups, at most 1024 each.
+// sp, the spheres per k_extrap workgroup, is 4 (a wave per sphere) up to
+// about 8000 spheres in all and grows to 32 for larger systems and batches,
+// where the per-workgroup reads of the DIIS state would dominate.
 int build_solver_maps(ddx_handle *h) {
   std::vector<int4> jb, eb;
   std::vector<int> jo(h->ninst + 1), eo(h->ninst + 1);
   const long long ny = h->nylm;
+  const int sp = std::max(4, std::min(32, h->n / 2048 / 4 * 4));
   for (int q = 0; q < h->ninst; ++q) {
     const int s0 = h->h_soff[q], s1 = h->h_soff[q + 1];
     const long long e0 = s0 * ny, e1 = s1 * ny;
@@ -680,7 +684,7 @@ int build_solver_maps(ddx_handle *h) {
     for (int j = 0; j < nj; ++j)
       jb.push_back(make_int4(q, (int)(e0 + (e1 - e0) * j / nj), (int)(e0 + (e1 - e0) * (j + 1) / nj), 0));
     eo[q] = (int)eb.size();
-    const int ne = grid_blocks((size_t)(s1 - s0), kJThreads / kWave, 1024);
+    const int ne = grid_blocks((size_t)(s1 - s0), sp, 1024);
     for (int j = 0; j < ne; ++j)
       eb.push_back(make_int4(q, s0 + (int)((long long)(s1 - s0) * j / ne),
                              s0 + (int)((long long)(s1 - s0) * (j + 1) / ne), 0));
