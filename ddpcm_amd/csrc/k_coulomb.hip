@@ -23,7 +23,8 @@
 namespace ddx {
 namespace {
 
-constexpr int kCT = 256;
+constexpr int kCT = 256;          // threads per workgroup, sources per LDS tile
+constexpr int kTW = 2 * kCT;      // targets per tile: two per thread
 
 // 1/sqrt(x) for x > 0: v_rsq_f64 (relative error below 2^-24 on gfx950)
 // and one third-order correction: with e = 1 - x y^2, y (1 + e/2 + 3e^2/8)
@@ -35,7 +36,8 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
 }
 
 // phi(j) and, with FIELD, the field ef(:, j) of the same sources, from one
-// evaluation of 1/r per pair; tile (tb, te, sb, se) per workgroup
+// evaluation of 1/r per pair; tile (tb, te, sb, se) per workgroup, targets
+// tb + t and tb + t + kCT in thread t (each source read from LDS serves two)
 template <bool FIELD>
 __global__ __launch_bounds__(kCT) void k_potential(const int4 *__restrict__ tiles,
                                                    const double *__restrict__ q,
@@ -45,14 +47,20 @@ __global__ __launch_bounds__(kCT) void k_potential(const int4 *__restrict__ tile
                                                    double *__restrict__ ef) {
   __shared__ double4 tile[kCT];
   const int4 tl = tiles[blockIdx.x];
-  const int j = tl.x + threadIdx.x;
-  double tx = 0.0, ty = 0.0, tz = 0.0;
-  if (j < tl.y) {
-    tx = trg[3 * (size_t)j];
-    ty = trg[3 * (size_t)j + 1];
-    tz = trg[3 * (size_t)j + 2];
+  const int j0 = tl.x + threadIdx.x, j1 = j0 + kCT;
+  double ax = 0.0, ay = 0.0, az = 0.0, bx = 0.0, by = 0.0, bz = 0.0;
+  if (j0 < tl.y) {
+    ax = trg[3 * (size_t)j0];
+    ay = trg[3 * (size_t)j0 + 1];
+    az = trg[3 * (size_t)j0 + 2];
   }
-  double v = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0;
+  if (j1 < tl.y) {
+    bx = trg[3 * (size_t)j1];
+    by = trg[3 * (size_t)j1 + 1];
+    bz = trg[3 * (size_t)j1 + 2];
+  }
+  double va = 0.0, ea0 = 0.0, ea1 = 0.0, ea2 = 0.0;
+  double vb = 0.0, eb0 = 0.0, eb1 = 0.0, eb2 = 0.0;
   for (int i0 = tl.z; i0 < tl.w; i0 += kCT) {
     int i = i0 + threadIdx.x;
     if (i < tl.w) {
@@ -62,32 +70,54 @@ __global__ __launch_bounds__(kCT) void k_potential(const int4 *__restrict__ tile
     __syncthreads();
     int cnt = min(kCT, tl.w - i0);
     for (int k = 0; k < cnt; ++k) {
-      double4 s = tile[k];
-      double dx = tx - s.x, dy = ty - s.y, dz = tz - s.z;
-      double d2 = dx * dx + dy * dy + dz * dz;
-      double y = rsqrt_nr(d2);
-      double qy = s.w * y;
-      v = v + qy;
-      if (FIELD) {
-        double f = qy * (y * y);
-        e0 = e0 + f * dx;
-        e1 = e1 + f * dy;
-        e2 = e2 + f * dz;
+      const double4 s = tile[k];
+      {
+        const double dx = ax - s.x, dy = ay - s.y, dz = az - s.z;
+        const double y = rsqrt_nr(dx * dx + dy * dy + dz * dz);
+        const double qy = s.w * y;
+        va = va + qy;
+        if (FIELD) {
+          const double f = qy * (y * y);
+          ea0 = ea0 + f * dx;
+          ea1 = ea1 + f * dy;
+          ea2 = ea2 + f * dz;
+        }
+      }
+      {
+        const double dx = bx - s.x, dy = by - s.y, dz = bz - s.z;
+        const double y = rsqrt_nr(dx * dx + dy * dy + dz * dz);
+        const double qy = s.w * y;
+        vb = vb + qy;
+        if (FIELD) {
+          const double f = qy * (y * y);
+          eb0 = eb0 + f * dx;
+          eb1 = eb1 + f * dy;
+          eb2 = eb2 + f * dz;
+        }
       }
     }
     __syncthreads();
   }
-  if (j < tl.y) {
-    phi[j] = v;
+  if (j0 < tl.y) {
+    phi[j0] = va;
     if (FIELD) {
-      ef[3 * (size_t)j] = e0;
-      ef[3 * (size_t)j + 1] = e1;
-      ef[3 * (size_t)j + 2] = e2;
+      ef[3 * (size_t)j0] = ea0;
+      ef[3 * (size_t)j0 + 1] = ea1;
+      ef[3 * (size_t)j0 + 2] = ea2;
+    }
+  }
+  if (j1 < tl.y) {
+    phi[j1] = vb;
+    if (FIELD) {
+      ef[3 * (size_t)j1] = eb0;
+      ef[3 * (size_t)j1 + 1] = eb1;
+      ef[3 * (size_t)j1 + 2] = eb2;
     }
   }
 }
 
-// partial fields of tile (tb, te, sb, se) at its targets: out[(b kCT + t) 3]
+// partial fields of tile (tb, te, sb, se) at its targets (two per thread):
+// out[(b kTW + t) 3]
 __global__ __launch_bounds__(kCT) void k_field(const int4 *__restrict__ tiles,
                                                const double *__restrict__ q,
                                                const double *__restrict__ src,
@@ -95,14 +125,19 @@ __global__ __launch_bounds__(kCT) void k_field(const int4 *__restrict__ tiles,
                                                double *__restrict__ out) {
   __shared__ double4 tile[kCT];
   const int4 tl = tiles[blockIdx.x];
-  const int j = tl.x + threadIdx.x;
-  double tx = 0.0, ty = 0.0, tz = 0.0;
-  if (j < tl.y) {
-    tx = trg[3 * (size_t)j];
-    ty = trg[3 * (size_t)j + 1];
-    tz = trg[3 * (size_t)j + 2];
+  const int j0 = tl.x + threadIdx.x, j1 = j0 + kCT;
+  double ax = 0.0, ay = 0.0, az = 0.0, bx = 0.0, by = 0.0, bz = 0.0;
+  if (j0 < tl.y) {
+    ax = trg[3 * (size_t)j0];
+    ay = trg[3 * (size_t)j0 + 1];
+    az = trg[3 * (size_t)j0 + 2];
   }
-  double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+  if (j1 < tl.y) {
+    bx = trg[3 * (size_t)j1];
+    by = trg[3 * (size_t)j1 + 1];
+    bz = trg[3 * (size_t)j1 + 2];
+  }
+  double ea0 = 0.0, ea1 = 0.0, ea2 = 0.0, eb0 = 0.0, eb1 = 0.0, eb2 = 0.0;
   for (int i0 = tl.z; i0 < tl.w; i0 += kCT) {
     int i = i0 + threadIdx.x;
     if (i < tl.w)
@@ -111,21 +146,33 @@ __global__ __launch_bounds__(kCT) void k_field(const int4 *__restrict__ tiles,
     __syncthreads();
     int cnt = min(kCT, tl.w - i0);
     for (int k = 0; k < cnt; ++k) {
-      double4 s = tile[k];
-      double dx = tx - s.x, dy = ty - s.y, dz = tz - s.z;
-      double r2 = dx * dx + dy * dy + dz * dz;
-      double y = rsqrt_nr(r2);
-      double f = s.w * (y * y * y);
-      e0 = e0 + f * dx;
-      e1 = e1 + f * dy;
-      e2 = e2 + f * dz;
+      const double4 s = tile[k];
+      {
+        const double dx = ax - s.x, dy = ay - s.y, dz = az - s.z;
+        const double y = rsqrt_nr(dx * dx + dy * dy + dz * dz);
+        const double f = s.w * (y * y * y);
+        ea0 = ea0 + f * dx;
+        ea1 = ea1 + f * dy;
+        ea2 = ea2 + f * dz;
+      }
+      {
+        const double dx = bx - s.x, dy = by - s.y, dz = bz - s.z;
+        const double y = rsqrt_nr(dx * dx + dy * dy + dz * dz);
+        const double f = s.w * (y * y * y);
+        eb0 = eb0 + f * dx;
+        eb1 = eb1 + f * dy;
+        eb2 = eb2 + f * dz;
+      }
     }
     __syncthreads();
   }
-  const size_t o = ((size_t)blockIdx.x * kCT + threadIdx.x) * 3;
-  out[o] = e0;
-  out[o + 1] = e1;
-  out[o + 2] = e2;
+  const size_t o = ((size_t)blockIdx.x * kTW + threadIdx.x) * 3, o1 = o + 3 * (size_t)kCT;
+  out[o] = ea0;
+  out[o + 1] = ea1;
+  out[o + 2] = ea2;
+  out[o1] = eb0;
+  out[o1 + 1] = eb1;
+  out[o1 + 2] = eb2;
 }
 
 // ef of the targets of group g = sum of the partials of its tiles
@@ -136,18 +183,20 @@ __global__ __launch_bounds__(kCT) void k_field_reduce(const int *__restrict__ gr
                                                       double *__restrict__ ef) {
   const int b0 = grp[blockIdx.x], b1 = grp[blockIdx.x + 1];
   const int4 tl = tiles[b0];
-  const int j = tl.x + threadIdx.x;
-  if (j >= tl.y) return;
-  double e0 = 0.0, e1 = 0.0, e2 = 0.0;
-  for (int b = b0; b < b1; ++b) {
-    const size_t o = ((size_t)b * kCT + threadIdx.x) * 3;
-    e0 += part[o];
-    e1 += part[o + 1];
-    e2 += part[o + 2];
+  for (int t = threadIdx.x; t < kTW; t += kCT) {
+    const int j = tl.x + t;
+    if (j >= tl.y) break;
+    double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+    for (int b = b0; b < b1; ++b) {
+      const size_t o = ((size_t)b * kTW + t) * 3;
+      e0 += part[o];
+      e1 += part[o + 1];
+      e2 += part[o + 2];
+    }
+    ef[3 * (size_t)j] = e0;
+    ef[3 * (size_t)j + 1] = e1;
+    ef[3 * (size_t)j + 2] = e2;
   }
-  ef[3 * (size_t)j] = e0;
-  ef[3 * (size_t)j + 1] = e1;
-  ef[3 * (size_t)j + 2] = e2;
 }
 
 __global__ void k_psi(int n, int nylm, double fac, const double *__restrict__ q,
@@ -159,7 +208,7 @@ __global__ void k_psi(int n, int nylm, double fac, const double *__restrict__ q,
 
 }  // namespace
 
-// Tiles of kCT targets, per instance q: targets toff[q] .. toff[q+1]-1,
+// Tiles of kTW targets, per instance q: targets toff[q] .. toff[q+1]-1,
 // sources soff[q] .. soff[q+1]-1.  With `split`, the sources of each target
 // tile are cut into chunks (multiples of kCT, at least 4 kCT) so that there
 // are about 1024 tiles in all; grp then holds the first tile of each target
@@ -169,7 +218,7 @@ void coulomb_tiles(const std::vector<int> &toff, const std::vector<int> &soff, b
   const size_t ninst = toff.size() - 1;
   long long work = 0;   // target tiles x sources
   for (size_t q = 0; q < ninst; ++q)
-    work += (long long)((toff[q + 1] - toff[q] + kCT - 1) / kCT) * (soff[q + 1] - soff[q]);
+    work += (long long)((toff[q + 1] - toff[q] + kTW - 1) / kTW) * (soff[q + 1] - soff[q]);
   long long chunk = 0x7fffffff;
   if (split) {
     chunk = (work + 1023) / 1024;
@@ -180,8 +229,8 @@ void coulomb_tiles(const std::vector<int> &toff, const std::vector<int> &soff, b
   grp.clear();
   for (size_t q = 0; q < ninst; ++q) {
     const int sb = soff[q], se = soff[q + 1];
-    for (int tb = toff[q]; tb < toff[q + 1]; tb += kCT) {
-      const int te = std::min(tb + kCT, toff[q + 1]);
+    for (int tb = toff[q]; tb < toff[q + 1]; tb += kTW) {
+      const int te = std::min(tb + kTW, toff[q + 1]);
       grp.push_back((int)tiles.size());
       if (se <= sb) {
         tiles.push_back(make_int4(tb, te, sb, sb));
@@ -240,7 +289,7 @@ int coulomb_potential(ddx_handle *h, const int4 *tiles, int ntile, const double 
 int coulomb_field(ddx_handle *h, const int4 *tiles, int ntile, const int *grp, int ngrp,
                   const double *q, const double *src3, const double *trg3, double *ef) {
   if (ntile <= 0 || ngrp <= 0) return DDX_OK;
-  DDX_TRY(h->fpart.reserve((size_t)ntile * kCT * 3));
+  DDX_TRY(h->fpart.reserve((size_t)ntile * kTW * 3));
   {
     TimedLaunch tl(h, DDX_K_FIELD, h->stream);
     hipExtLaunchKernelGGL(k_field, dim3(ntile), dim3(kCT), 0, h->stream, tl.b, tl.e, 0, tiles, q,
