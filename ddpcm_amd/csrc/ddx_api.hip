@@ -206,7 +206,7 @@ int pipeline(ddx_handle *h) {
   }
   if (h->igrad) {
     DDX_TRY(forces_dd(h, h->phi.p, h->sigma.p, h->s.p, h->fx.p));
-    DDX_TRY(make_zeta(h, h->s.p, h->zeta.p));
+    DDX_TRY(make_zeta(h, h->s.p, h->zeta.p, true));
     DDX_TRY(field_forces(h, h->charge.p, h->zeta.p, h->fx.p, true));
     DDX_HIP(hipEventRecord(h->ev[5], s));
   }

@@ -84,7 +84,8 @@ int make_psi(ddx_handle *h, const double *charge, double *psi);
 // k_forces.hip — forces_dd, ddmkzeta and the field contractions
 int forces_dd(ddx_handle *h, const double *d_phi, const double *d_sigma,
               const double *d_s, double *d_fx);
-int make_zeta(ddx_handle *h, const double *d_s, double *d_zeta);
+// (xi_ready: h->xi already holds basis^T s, as forces_dd leaves it)
+int make_zeta(ddx_handle *h, const double *d_s, double *d_zeta, bool xi_ready = false);
 // solute_field: h->ef already holds the solute field at the cavity points
 // (from coulomb_potential)
 int field_forces(ddx_handle *h, const double *d_charge, const double *d_zeta,

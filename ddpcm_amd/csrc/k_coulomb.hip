@@ -176,26 +176,42 @@ __global__ __launch_bounds__(kCT) void k_field(const int4 *__restrict__ tiles,
 }
 
 // ef of the targets of group g = sum of the partials of its tiles
-// grp[g] .. grp[g+1]-1 (chunks of one target tile), in chunk order
+// grp[g] .. grp[g+1]-1 (chunks of one target tile), in a fixed order: each
+// target's chunks are split over kRq threads (chunk c to thread c mod kRq),
+// whose sums meet in LDS in thread order.  blockIdx.y picks the slice of
+// kCT / kRq targets.
+constexpr int kRq = 8;
 __global__ __launch_bounds__(kCT) void k_field_reduce(const int *__restrict__ grp,
                                                       const int4 *__restrict__ tiles,
                                                       const double *__restrict__ part,
                                                       double *__restrict__ ef) {
+  __shared__ double red[kCT][3];
   const int b0 = grp[blockIdx.x], b1 = grp[blockIdx.x + 1];
   const int4 tl = tiles[b0];
-  for (int t = threadIdx.x; t < kTW; t += kCT) {
-    const int j = tl.x + t;
-    if (j >= tl.y) break;
-    double e0 = 0.0, e1 = 0.0, e2 = 0.0;
-    for (int b = b0; b < b1; ++b) {
-      const size_t o = ((size_t)b * kTW + t) * 3;
-      e0 += part[o];
-      e1 += part[o + 1];
-      e2 += part[o + 2];
+  const int r = threadIdx.x % kRq;
+  const int t = blockIdx.y * (kCT / kRq) + threadIdx.x / kRq;
+  double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+  for (int b = b0 + r; b < b1; b += kRq) {
+    const size_t o = ((size_t)b * kTW + t) * 3;
+    e0 += part[o];
+    e1 += part[o + 1];
+    e2 += part[o + 2];
+  }
+  red[threadIdx.x][0] = e0;
+  red[threadIdx.x][1] = e1;
+  red[threadIdx.x][2] = e2;
+  __syncthreads();
+  const int j = tl.x + t;
+  if (r == 0 && j < tl.y) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int k = 0; k < kRq; ++k) {
+      s0 += red[threadIdx.x + k][0];
+      s1 += red[threadIdx.x + k][1];
+      s2 += red[threadIdx.x + k][2];
     }
-    ef[3 * (size_t)j] = e0;
-    ef[3 * (size_t)j + 1] = e1;
-    ef[3 * (size_t)j + 2] = e2;
+    ef[3 * (size_t)j] = s0;
+    ef[3 * (size_t)j + 1] = s1;
+    ef[3 * (size_t)j + 2] = s2;
   }
 }
 
@@ -296,7 +312,8 @@ int coulomb_field(ddx_handle *h, const int4 *tiles, int ntile, const int *grp, i
                           src3, trg3, h->fpart.p);
   }
   DDX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_field_reduce, dim3(ngrp), dim3(kCT), 0, h->stream, grp, tiles, h->fpart.p,
+  hipLaunchKernelGGL(k_field_reduce, dim3(ngrp, kTW / (kCT / kRq)), dim3(kCT), 0, h->stream, grp,
+                     tiles, h->fpart.p,
                      ef);
   DDX_HIP(hipGetLastError());
   return DDX_OK;

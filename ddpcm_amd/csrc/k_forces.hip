@@ -383,10 +383,11 @@ int forces_dd(ddx_handle *h, const double *d_phi, const double *d_sigma, const d
   return DDX_OK;
 }
 
-int make_zeta(ddx_handle *h, const double *d_s, double *d_zeta) {
+int make_zeta(ddx_handle *h, const double *d_s, double *d_zeta, bool xi_ready) {
   hipStream_t s = h->stream;
   DDX_TRY(h->xi.reserve(h->np));
-  DDX_TRY(expand(h, h->stream, d_s, h->xi.p, false, nullptr));
+  // xi = basis^T s, unless forces_dd left it for the same s
+  if (!xi_ready) DDX_TRY(expand(h, h->stream, d_s, h->xi.p, false, nullptr));
   double scale = 0.5 * ((h->eps - 1.0) / h->eps);
   hipLaunchKernelGGL(k_zeta, dim3((h->np + 255) / 256), dim3(256), 0, s, h->np, h->ngrid, scale,
                      h->grid.p, h->ui.p, h->cavidx.p, h->xi.p, d_zeta);
