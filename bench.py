@@ -1,6 +1,6 @@
 """Energy + forces ddCOSMO solves per second on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload c2|c3|c4|c5] [--precision fp64]
+    python bench.py [--steps K] [--warmup W] [--workload c2|c3|c4|c5] [--precision fp64|mixed]
 
 One step is one complete solve from coordinates already resident in HBM:
 device setup (neighbours, fi/ui/zi, cavity, triple lists), RHS (solute
@@ -37,6 +37,7 @@ sys.path.insert(0, ROOT)
 METRIC = "E+forces solves/s at conv 1e-8: mixed+learned vs plain FP64 GPU; HBM GB/s"
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6      # MI355X FP64, vector and dense matrix alike
+FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 dense matrix (vector FP32 reaches it only packed)
 KERNELS = ("sweep_l", "sweep_lstar", "project", "expand", "jacobi", "diis", "extrap", "field",
            "potential", "forces", "sweep_pair")
 
@@ -206,7 +207,8 @@ def trajectory_bench(args):
     Gaussian steps (0.05 angstrom per atom and frame).  Each guess mode runs
     the same frames; the learned modes train on the first `--train` frames'
     solutions (8 Adam steps per frame) and all modes are timed on the frames
-    after them.  value: solves/s of the learned guess."""
+    after them; the last mode adds the mixed precision to the previous +
+    learned guess.  value: solves/s of the learned guess."""
     from ddpcm_amd import inputs
     from ddpcm_amd.ddx import (DDCosmo, DDX_GUESS_LEARNED, DDX_GUESS_LEARNED_PREVIOUS,
                                DDX_GUESS_PREVIOUS, DDX_GUESS_REFERENCE)
@@ -215,13 +217,16 @@ def trajectory_bench(args):
     tb = 1.0 / inputs.TOANG
     d = inputs.synthetic_input(1000, 1000, 8, 590)
     modes = {}
-    for name, mode, train in [("reference", DDX_GUESS_REFERENCE, 0),
-                              ("previous", DDX_GUESS_PREVIOUS, 0),
-                              ("learned", DDX_GUESS_LEARNED, ntrain),
-                              ("learned_previous", DDX_GUESS_LEARNED_PREVIOUS, ntrain)]:
+    for name, mode, train, prec in [("reference", DDX_GUESS_REFERENCE, 0, 0),
+                                    ("previous", DDX_GUESS_PREVIOUS, 0, 0),
+                                    ("learned", DDX_GUESS_LEARNED, ntrain, 0),
+                                    ("learned_previous", DDX_GUESS_LEARNED_PREVIOUS, ntrain, 0),
+                                    ("mixed_learned_previous", DDX_GUESS_LEARNED_PREVIOUS,
+                                     ntrain, 1)]:
         q, xyz, r = frames[0]
         h = DDCosmo(xyz[:, 0] * tb, xyz[:, 1] * tb, xyz[:, 2] * tb, r * tb, lmax=8, ngrid=590)
         h.set_guess(mode)
+        h.set_precision(prec)
         its, t_eval, es = [], 0.0, []
         for f, (q, xyz, r) in enumerate(frames):
             h.upload(xyz[:, 0] * tb, xyz[:, 1] * tb, xyz[:, 2] * tb, r * tb, q)
@@ -264,6 +269,38 @@ def trajectory_bench(args):
     print(json.dumps(out))
 
 
+def mixed_beside(h, args, sec64, es64, fx64, batch):
+    """The same workload with FP32 sweeps and harmonic products inside FP64
+    iterative refinement: solves/s, its speed-up over the FP64 solves timed
+    just before, iterations, and the differences of the energies and forces
+    from the FP64 results."""
+    h.set_precision(1)
+    for _ in range(max(1, args.warmup)):
+        h.run()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        h.run()
+    sec = (time.perf_counter() - t0) / args.steps
+    es, fx = h.result()
+    st = h.stats()
+    h.set_precision(0)
+    es, es64 = np.atleast_1d(es), np.atleast_1d(es64)
+    out = {"ms_per_step": sec * 1e3, "speedup_vs_fp64": sec64 / sec,
+           "energy_max_rel_diff_vs_fp64": float(np.max(np.abs(es - es64) / np.abs(es64))),
+           "force_max_abs_diff_vs_fp64": float(np.max(np.abs(fx - fx64))),
+           "stages_ms": {"solves": st.ms_direct + st.ms_adjoint}}
+    if batch:
+        nd, na = h.iterations()
+        out["value"] = h.count / sec
+        out["iterations"] = {"direct": [int(nd.min()), int(nd.max())],
+                             "adjoint": [int(na.min()), int(na.max())]}
+    else:
+        out["value"] = 1.0 / sec
+        out["iterations"] = {"direct": st.niter_direct, "adjoint": st.niter_adjoint}
+    out["unit"] = "solves/s"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,7 +309,9 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--train", type=int, default=200,
                     help="c5: frames whose solutions train the guess network")
-    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
+                    help="precision of the timed solves; an fp64 run also times the mixed "
+                         "precision beside it (\"mixed_precision\" in the JSON line)")
     ap.add_argument("--cpu-cores", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -316,6 +355,8 @@ def main():
         h = DDCosmo.from_input(d)
         h.upload(x, y, z, r, d.charge)
         tag = {"c2": "2000atoms", "c3": "20000atoms"}.get(args.workload, args.workload)
+    if args.precision == "mixed":
+        h.set_precision(1)
     for _ in range(args.warmup):
         h.run()
     # which kernel class dominates one solve
@@ -341,8 +382,10 @@ def main():
         avg_s = kms * 1e-3 / launches
         if unit == "flop":
             ach = per_launch / avg_s / 1e12
-            roofline = {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
-                        "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None}
+            # the FP32 operators of the mixed precision against the FP32 peak
+            peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
+            roofline = {"bound": "mfma", "achieved": ach, "peak": peak,
+                        "unit": "TFLOP/s", "frac": ach / peak, "traffic": None}
         else:
             ach = per_launch / avg_s / 1e9
             roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -351,6 +394,10 @@ def main():
         roofline.update({"kernel": dominant, "launches": launches,
                          "avg_launch_us": avg_s * 1e6, "work_per_launch": per_launch,
                          "work_unit": unit})
+
+    mixed = None
+    if args.precision == "fp64":
+        mixed = mixed_beside(h, args, sec, es, fx, batch)
 
     parity = None
     gpath = os.path.join(ROOT, "tests", "golden", args.workload + ".npz")
@@ -385,7 +432,8 @@ def main():
         "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
+        "dtype": "f64" if args.precision == "fp64" else "f32 sweeps in f64 refinement",
+        "data": "synthetic",
         "config": config,
         "stages_ms": {"setup": st.ms_setup, "rhs": st.ms_rhs,
                       "solves": st.ms_direct + st.ms_adjoint, "forces": st.ms_forces},
@@ -395,6 +443,7 @@ def main():
         "roofline": roofline,
         "kernels": table,
         "hbm_gbs": {k: table[k].get("gbs") for k in ("jacobi", "extrap")},
+        "mixed_precision": mixed,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = (cpu_baseline_batch(decks, args.cpu_cores) if batch

@@ -136,7 +136,7 @@ int run_direct(ddx_handle *h, const double *d_psi) {
   Lane *ln = &h->lane[0];
   int ok = 0;
   DDX_TRY(begin_direct(h, *ln));
-  DDX_TRY(solve_run(h, &ln, 1));
+  DDX_TRY(solve_lanes(h, &ln, 1));
   DDX_TRY(end_direct(h, *ln, d_psi, &ok));
   return ok ? DDX_OK : noconv(false);
 }
@@ -145,7 +145,7 @@ int run_adjoint(ddx_handle *h, const double *d_psi) {
   Lane *ln = &h->lane[0];
   int ok = 0;
   DDX_TRY(begin_adjoint(h, *ln, d_psi));
-  DDX_TRY(solve_run(h, &ln, 1));
+  DDX_TRY(solve_lanes(h, &ln, 1));
   DDX_TRY(end_adjoint(h, *ln, &ok));
   return ok ? DDX_OK : noconv(true);
 }
@@ -184,13 +184,13 @@ int pipeline(ddx_handle *h) {
     Lane *lanes[2] = {&h->lane[0], &h->lane[1]};
     DDX_TRY(begin_direct(h, h->lane[0]));
     DDX_TRY(begin_adjoint(h, h->lane[1], h->psi.p));
-    DDX_TRY(solve_run(h, lanes, 2));
+    DDX_TRY(solve_lanes(h, lanes, 2));
     DDX_TRY(end_direct(h, h->lane[0], h->psi.p, &ok_d));
     DDX_TRY(end_adjoint(h, h->lane[1], &ok_a));
   } else {
     Lane *ln = &h->lane[0];
     DDX_TRY(begin_direct(h, *ln));
-    DDX_TRY(solve_run(h, &ln, 1));
+    DDX_TRY(solve_lanes(h, &ln, 1));
     DDX_TRY(end_direct(h, *ln, h->psi.p, &ok_d));
   }
   DDX_HIP(hipEventRecord(h->ev[3], s));
@@ -413,6 +413,7 @@ int ddx_free(ddx_handle *h) {
   }
   for (auto &e : h->gev)
     if (e) (void)hipEventDestroy(e);
+  drop_round_graphs(h);
   for (auto &t : h->ktimer) {
     for (auto &r : t.recs) {
       (void)hipEventDestroy(r.b);
@@ -468,12 +469,16 @@ int ddx_get_array(const ddx_handle *hc, const char *name, double *dst) {
     if (!h->igrad) { set_error("zi exists only with igrad = 1"); return DDX_ERR_STATE; }
     src = h->zi.p; cnt = 3 * np;
   } else if (!strcmp(name, "ccav")) { src = h->ccav.p; cnt = 3 * (size_t)h->ncav; }
-  else if (!strcmp(name, "rhs") || !strcmp(name, "xi")) {
+  else if (!strcmp(name, "rhs") || !strcmp(name, "xi") || !strcmp(name, "sigma") ||
+           !strcmp(name, "s")) {
     // the last RHS a direct solve used (cosmo.f90:102-128), the last
-    // xi = basis^T S of the forces (forces_dd.f90:108-114)
-    const bool rhs = name[0] == 'r';
-    src = rhs ? h->rhs.p : h->xi.p;
-    cnt = rhs ? (size_t)h->nylm * h->n : np;
+    // xi = basis^T S of the forces (forces_dd.f90:108-114), the last
+    // solutions sigma (direct) and S (adjoint)
+    const bool xi = !strcmp(name, "xi");
+    src = xi ? h->xi.p : name[0] == 'r' ? h->rhs.p : name[1] == 'i' ? h->sigma.p : h->s.p;
+    if (!strcmp(name, "sigma") && !h->have_sigma) src = nullptr;
+    if (!strcmp(name, "s") && !h->have_s) src = nullptr;
+    cnt = xi ? np : (size_t)h->nylm * h->n;
     if (!src) {
       set_error("'%s' is not available yet", name);
       return DDX_ERR_STATE;
@@ -590,9 +595,18 @@ static int op_apply(ddx_handle *h, int which, const double *x, double *y) {
   DDX_TRY(reserve_vectors(h));
   DDX_TRY(h->y.reserve(nn));
   DDX_TRY(upload(h->x, x, nn, s));
-  if (which == 0) DDX_TRY(apply_lx(h, h->lane[0], h->x.p, h->y.p, nullptr, false, 1));
-  else if (which == 1) DDX_TRY(apply_lstarx(h, h->lane[0], h->x.p, h->y.p, nullptr, true));
-  else DDX_TRY(device_ldm1x(h, s, h->x.p, h->y.p));
+  // in the handle's precision: the FP32 operators under mixed precision
+  Lane &l0 = h->lane[0];
+  if (which < 2 && h->prec == DDX_PREC_MIXED) {
+    DDX_TRY(make_tables32(h));
+    l0.f32 = true;
+  }
+  int rc = DDX_OK;
+  if (which == 0) rc = apply_lx(h, l0, h->x.p, h->y.p, nullptr, false, 1);
+  else if (which == 1) rc = apply_lstarx(h, l0, h->x.p, h->y.p, nullptr, true);
+  else rc = device_ldm1x(h, s, h->x.p, h->y.p);
+  l0.f32 = false;
+  DDX_TRY(rc);
   DDX_TRY(download(y, h->y.p, nn, s));
   return sync(h);
 }
@@ -705,11 +719,15 @@ int ddx_batch_sizes(const ddx_handle *h, int *count, int *sph_off, int *cav_off)
 
 int ddx_set_precision(ddx_handle *h, int prec) {
   if (!h || (prec != DDX_PREC_FP64 && prec != DDX_PREC_MIXED)) return DDX_ERR_ARG;
-  if (prec == DDX_PREC_MIXED) {
-    set_error("mixed precision is not built yet");
-    return DDX_ERR_ARG;
-  }
+  DDX_TRY(sync(h));
   h->prec = prec;
+  return DDX_OK;
+}
+
+int ddx_set_graphs(ddx_handle *h, int enable) {
+  if (!h) return DDX_ERR_ARG;
+  DDX_TRY(sync(h));
+  h->use_graphs = enable != 0;
   return DDX_OK;
 }
 

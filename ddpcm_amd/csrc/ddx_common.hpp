@@ -84,6 +84,7 @@ struct DiisState {
   double coef[kDiisMax];    // extrapolation weights, by logical index
   double b[(kDiisMax + 1) * (kDiisMax + 1)];  // bordered Gram matrix, rows of 25, 0-based
   double norm;              // increment norm of the last iteration
+  double tol;               // convergence threshold of this instance
   double hist[kMaxIter + 1];
 };
 
@@ -139,6 +140,9 @@ struct Lane {
   const double *rhs = nullptr;
   double *x = nullptr;
   bool active = false;
+  bool f32 = false;               // operators in FP32 (inner solves of the refinement)
+  DevArray<double> rres, dcorr;   // refinement: FP64 residual and correction
+  DevArray<double> dnorm;         // refinement: per-instance norms of the correction
 };
 
 }  // namespace ddx
@@ -169,6 +173,8 @@ struct ddx_handle {
   // tables
   ddx::DevArray<double4> grid;    // (x, y, z, w) per Lebedev point
   ddx::DevArray<double> Y, Yw;    // basis (nylm, ngrid) and basis * w
+  ddx::DevArray<float> Y32, Yw32; // their float copies (mixed precision)
+  bool have_tables32 = false;
   ddx::DevArray<double> facl, cf; // (2l+1)/4pi and facs * 4pi/(2l+1)
   ddx::DevArray<double> facs;
 
@@ -215,6 +221,14 @@ struct ddx_handle {
   std::vector<int4> h_ctile, h_ftile, h_xtile;   // host copies of the tile tables
   std::vector<int> h_fgrp, h_xgrp;
   int maps_n = -1;                // sphere count the block maps were built for
+  bool mute_timers = false;       // launches outside the timed classes' meaning
+  // groups of solver rounds captured as HIP graphs (ddx_set_graphs)
+  bool use_graphs = false;
+  struct RoundGraph {
+    std::vector<uintptr_t> key;
+    hipGraphExec_t exec;
+  };
+  std::vector<RoundGraph> round_graphs;
   int ysplit = 1;                 // partial projections of L X in the solver
 
   // solver workspace: lane 0 runs on `stream`, lane 1 on a stream of its own
@@ -251,7 +265,7 @@ struct TimedLaunch {
   hipEvent_t b = nullptr, e = nullptr;
   TimedLaunch(ddx_handle *h, int kid, hipStream_t) {
     KernelTimer &t = h->ktimer[kid];
-    if (!t.on) return;
+    if (!t.on || h->mute_timers) return;
     hipEvent_t ev[2] = {nullptr, nullptr};
     for (auto &x : ev) {
       if (!t.pool.empty()) {

@@ -23,6 +23,14 @@ constexpr int kProjSplit = 4;
 // xi(n, i) = [w_n] sum_lm Y(lm, n) S(lm, i)
 int expand(ddx_handle *h, hipStream_t s, const double *S, double *xi, bool wscale,
            const DiisState *gate);
+// the same two products in FP32 (v_mfma_f32_16x16x4f32, FP32 tables Yw32 /
+// Y32) for the mixed precision: P and xi in float, S and y in double
+int project32(ddx_handle *h, hipStream_t s, double alpha, const float *P, double *y,
+              const DiisState *gate, int ysplit);
+int expand32(ddx_handle *h, hipStream_t s, const double *S, float *xi, bool wscale,
+             const DiisState *gate);
+// float copies of the harmonic tables (made once per handle)
+int make_tables32(ddx_handle *h);
 
 // k_sweep.hip — off-diagonal operators (matvec.f90:94-235), on the scratch
 // and stream of a lane.  L X reads X' = cf X from ln.xs: apply_lx makes it
@@ -37,6 +45,11 @@ int apply_lstarx(ddx_handle *h, Lane &ln, const double *x, double *y, const Diis
 int apply_pair(ddx_handle *h, Lane &ld, double *yd, Lane &la, const double *xa, int ysplit);
 // xs = cf x
 int cf_scale(ddx_handle *h, hipStream_t s, const double *x, double *xs);
+// y = sum of the per-segment partial sums of L* S
+int seg_sum(ddx_handle *h, hipStream_t s, const double *yp, double *y, const DiisState *gate);
+// ungated FP64 operators for residuals: L xd into ld->y and L* xa into
+// la->y (either lane may be null)
+int apply_fp64(ddx_handle *h, Lane *ld, const double *xd, Lane *la, const double *xa);
 
 // k_solver.hip — Jacobi/DIIS (jacobi_diis.f90:43-438), RHS, norms
 // Solves on lanes: solve_begin sets one up (x holds the guess); solve_run
@@ -50,9 +63,14 @@ inline const DiisState *solve_gate(const ddx_handle *h, const Lane &ln) {
   return ln.diis.p + (h->ninst > 1 ? h->ninst : 0);
 }
 int build_solver_maps(ddx_handle *h);
+// releases the captured graphs of solver rounds (after a new setup)
+void drop_round_graphs(ddx_handle *h);
 int solve_begin(ddx_handle *h, Lane &ln, int star, int diis_max, double tol,
                 const double *d_rhs, double *d_x, int max_iter);
 int solve_run(ddx_handle *h, Lane *const *lanes, int count);
+// solve_run in the handle's precision: FP64, or the refinement of the mixed
+// precision (FP32 operators inside FP64 iterative refinement)
+int solve_lanes(ddx_handle *h, Lane *const *lanes, int count);
 int solve_end(ddx_handle *h, Lane &ln, int *n_iter, int *ok, std::vector<double> *hist);
 int solve_jacobi_diis(ddx_handle *h, Lane &ln, int star, int diis_max, double tol,
                       const double *d_rhs, double *d_x, int max_iter,

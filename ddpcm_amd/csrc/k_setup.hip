@@ -324,6 +324,7 @@ int run_setup(ddx_handle *h) {
   int n = h->n, ng = h->ngrid;
   size_t np = (size_t)n * ng;
   h->np = np;
+  drop_round_graphs(h);   // captured solver rounds hold the old buffers
 
   // neighbour list on the spatial hash
   unsigned T = 64;

@@ -24,6 +24,8 @@
 #include "ddx_kernels.hpp"
 
 #include <algorithm>
+#include <cstring>
+#include <iterator>
 
 namespace ddx {
 namespace {
@@ -169,12 +171,13 @@ __device__ __forceinline__ void instance_done(DiisState *sum, int it, int ok) {
 
 __global__ __launch_bounds__(kSmallThreads) void k_diis_small(
     int diis_max, int nbj, const int *__restrict__ jboff, const int *__restrict__ eboff,
-    const int *__restrict__ soff, int ninst, double tol, int maxit, Systems sys) {
+    const int *__restrict__ soff, int ninst, int maxit, Systems sys) {
 #pragma clang fp contract(off)
   const SysArgs &q0 = sys.s[blockIdx.y];
   const int inst = blockIdx.x;
   DiisState *__restrict__ st = q0.st + inst;
   if (st->done) return;
+  const double tol = st->tol;
   DiisState *sum = ninst > 1 ? q0.st + ninst : nullptr;
   // this instance's workgroup partials
   const int jb = jboff[inst], eb = eboff[inst];
@@ -490,10 +493,11 @@ __global__ __launch_bounds__(kJThreads) void k_extrap(int nylm, size_t nn, int d
 // jacobi_diis.f90:405-438; jacobi_diis.f90:153-156)
 __global__ __launch_bounds__(kWave) void k_norm(const int *__restrict__ eboff,
                                                 const int *__restrict__ soff, int ninst,
-                                                double tol, int maxit, Systems sys) {
+                                                int maxit, Systems sys) {
   const int inst = blockIdx.x;
   DiisState *__restrict__ st = sys.s[blockIdx.y].st + inst;
   if (st->done) return;
+  const double tol = st->tol;
   const double *__restrict__ part = sys.s[blockIdx.y].part_e;
   double s = 0.0;
   for (int b = eboff[inst] + threadIdx.x; b < eboff[inst + 1]; b += kWave) s += part[b];
@@ -512,7 +516,7 @@ __global__ __launch_bounds__(kWave) void k_norm(const int *__restrict__ eboff,
 }
 
 // fresh states for the ninst instances and the summary of a batch
-__global__ void k_diis_init(int ninst, DiisState *__restrict__ st) {
+__global__ void k_diis_init(int ninst, double tol, DiisState *__restrict__ st) {
   const int q = blockIdx.x;
   DiisState *d = st + q;
   const int t = threadIdx.x;
@@ -527,7 +531,55 @@ __global__ void k_diis_init(int ninst, DiisState *__restrict__ st) {
     d->ncoef = 0;
     d->remaining = q < ninst ? 0 : ninst;
     d->norm = 0.0;
+    d->tol = tol;
   }
+}
+
+// Norms of the refinement, per instance: sqrt(sum over its spheres of
+// sum_lm x^2 / (l + 1), divided by the sphere count), the norm of the
+// convergence test (rmsvec, jacobi_diis.f90:405-438; the per-sphere square
+// root and square cancel).  Partial sums over the Jacobi block map (every
+// block inside one instance), then one wave per instance.
+__global__ __launch_bounds__(kJThreads) void k_sq_part(const int4 *__restrict__ jblk, int nylm,
+                                                       const double *__restrict__ invl1,
+                                                       const double *__restrict__ x,
+                                                       double *__restrict__ xadd,
+                                                       double *__restrict__ part) {
+  __shared__ double lds[kJThreads / kWave];
+  const int4 blk = jblk[blockIdx.x];
+  double s = 0.0;
+  for (int k = blk.y + threadIdx.x; k < blk.z; k += kJThreads) {
+    const double v = x[k];
+    s += invl1[k % nylm] * v * v;
+    if (xadd) xadd[k] += v;   // refinement: x += d
+  }
+  const double tot = block_sum_256(s, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// mode 0: st[q].tol = max(st[q].tol, rel norm_q); mode 1: out[q] = norm_q
+__global__ __launch_bounds__(kWave) void k_inst_norm(const int *__restrict__ jboff,
+                                                     const int *__restrict__ soff,
+                                                     const double *__restrict__ part, double rel,
+                                                     DiisState *__restrict__ st,
+                                                     double *__restrict__ out) {
+  const int q = blockIdx.x;
+  double s = 0.0;
+  for (int b = jboff[q] + threadIdx.x; b < jboff[q + 1]; b += kWave) s += part[b];
+  s = wave_sum(s);
+  if (threadIdx.x == 0) {
+    const double nrm = sqrt(s / (double)max(1, soff[q + 1] - soff[q]));
+    if (st) st[q].tol = fmax(st[q].tol, rel * nrm);
+    else out[q] = nrm;
+  }
+}
+
+// refinement residual r = g - D x - y, y = L x (or L* x); D = 1 / facl
+__global__ void k_residual(size_t nn, int nylm, const double *__restrict__ facl,
+                           const double *__restrict__ g, const double *__restrict__ x,
+                           const double *__restrict__ y, double *__restrict__ r) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nn) r[k] = (g[k] - x[k] / facl[k % nylm]) - y[k];
 }
 
 // per instance: iterations as the reference reports them and the ok flag
@@ -788,24 +840,19 @@ int solve_begin(ddx_handle *h, Lane &ln, int star, int diis_max, double tol, con
   ln.active = true;
 
   const int nst = h->ninst > 1 ? h->ninst + 1 : 1;
-  hipLaunchKernelGGL(k_diis_init, dim3(nst), dim3(kWave), 0, ln.stream, h->ninst, ln.diis.p);
+  hipLaunchKernelGGL(k_diis_init, dim3(nst), dim3(kWave), 0, ln.stream, h->ninst, tol,
+                     ln.diis.p);
   DDX_HIP(hipGetLastError());
   // the L X sweep reads cf x: from here on k_extrap keeps it
   if (!star) DDX_TRY(cf_scale(h, ln.stream, d_x, ln.xs.p));
   return DDX_OK;
 }
 
-// Enqueues the next group of rounds and the copy of each lane's state header
-// (nmat, it, done, ok) into slot `slot` of its pinned h_head, marked by
-// h->gev[slot].
-int solve_enqueue(ddx_handle *h, Lane *const *lanes, int count, int slot) {
+namespace {
+// `nr` rounds of one solve, or of a direct and an adjoint solve in lockstep
+int enqueue_rounds(ddx_handle *h, Lane *const *lanes, int count, int nr) {
   Lane &l0 = *lanes[0];
   const bool pair = count == 2;
-  if (pair && (l0.star != 0 || lanes[1]->star != 1 || lanes[1]->stream != l0.stream ||
-               lanes[1]->diis_max != l0.diis_max || lanes[1]->rounds != l0.rounds)) {
-    set_error("paired solves need a direct and an adjoint solve with equal settings");
-    return DDX_ERR_ARG;
-  }
   hipStream_t s = l0.stream;
   size_t nn = (size_t)h->nylm * h->n;
   const int nbj = h->nbj, nbe = h->nbe, ninst = h->ninst;
@@ -813,8 +860,7 @@ int solve_enqueue(ddx_handle *h, Lane *const *lanes, int count, int slot) {
   const int diis_max = l0.diis_max;
   Systems sys{};
   for (int k = 0; k < count; ++k) sys.s[k] = sys_args(h, *lanes[k]);
-  h->tag_solve = l0.serial;
-  for (int g = 0; g < l0.group && l0.enqueued < l0.rounds; ++g, ++l0.enqueued) {
+  for (int g = 0; g < nr; ++g, ++l0.enqueued) {
     h->tag_round = l0.enqueued;
     if (pair) {
       DDX_TRY(apply_pair(h, l0, l0.y.p, *lanes[1], lanes[1]->x, h->ysplit));
@@ -833,7 +879,7 @@ int solve_enqueue(ddx_handle *h, Lane *const *lanes, int count, int slot) {
       TimedLaunch tl(h, DDX_K_DIIS, s);
       hipExtLaunchKernelGGL(k_diis_small, dim3(ninst, count), dim3(kSmallThreads), 0, s, tl.b,
                             tl.e, 0, diis_max, nbj, h->jboff.p, h->eboff.p, h->soff.p, ninst,
-                            l0.tol, l0.max_iter, sys);
+                            l0.max_iter, sys);
       DDX_HIP(hipGetLastError());
     }
     {
@@ -844,10 +890,98 @@ int solve_enqueue(ddx_handle *h, Lane *const *lanes, int count, int slot) {
     DDX_HIP(hipGetLastError());
     if (diis_max == 0) {
       hipLaunchKernelGGL(k_norm, dim3(ninst, count), dim3(kWave), 0, s, h->eboff.p, h->soff.p,
-                         ninst, l0.tol, l0.max_iter, sys);
+                         ninst, l0.max_iter, sys);
       DDX_HIP(hipGetLastError());
     }
   }
+  return DDX_OK;
+}
+
+// what a captured group of rounds depends on: every pointer and setting the
+// launches take
+std::vector<uintptr_t> round_key(const ddx_handle *h, Lane *const *lanes, int count, int nr) {
+  std::vector<uintptr_t> k = {(uintptr_t)count, (uintptr_t)nr, (uintptr_t)h->ysplit,
+                              (uintptr_t)h->nbj, (uintptr_t)h->nbe, (uintptr_t)h->nworker};
+  for (int c = 0; c < count; ++c) {
+    const Lane &l = *lanes[c];
+    const uintptr_t v[] = {(uintptr_t)l.star, (uintptr_t)l.f32, (uintptr_t)l.diis_max,
+                           (uintptr_t)l.max_iter,
+                           (uintptr_t)l.rhs, (uintptr_t)l.x, (uintptr_t)l.y.p, (uintptr_t)l.xnew.p,
+                           (uintptr_t)l.xs.p, (uintptr_t)l.P.p, (uintptr_t)l.xiw.p,
+                           (uintptr_t)l.rw.p, (uintptr_t)l.yp.p, (uintptr_t)l.xd.p,
+                           (uintptr_t)l.ed.p, (uintptr_t)l.part.p, (uintptr_t)l.diis.p,
+                           (uintptr_t)l.misc.p, (uintptr_t)l.stream};
+    k.insert(k.end(), std::begin(v), std::end(v));
+    uint64_t tb;
+    std::memcpy(&tb, &l.tol, 8);
+    k.push_back((uintptr_t)tb);
+  }
+  return k;
+}
+
+bool timers_on(const ddx_handle *h) {
+  for (const auto &t : h->ktimer)
+    if (t.on) return true;
+  return false;
+}
+
+// the group of nr rounds as a HIP graph: captured on first use with these
+// lanes, replayed afterwards (every round's state lives in device memory)
+int launch_round_graph(ddx_handle *h, Lane *const *lanes, int count, int nr) {
+  Lane &l0 = *lanes[0];
+  const std::vector<uintptr_t> key = round_key(h, lanes, count, nr);
+  hipGraphExec_t exec = nullptr;
+  for (auto &g : h->round_graphs)
+    if (g.key == key) exec = g.exec;
+  if (!exec) {
+    hipGraph_t graph = nullptr;
+    const int first = l0.enqueued;
+    DDX_HIP(hipStreamBeginCapture(l0.stream, hipStreamCaptureModeRelaxed));
+    const int rc = enqueue_rounds(h, lanes, count, nr);
+    const hipError_t ec = hipStreamEndCapture(l0.stream, &graph);
+    l0.enqueued = first;
+    if (rc != DDX_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    DDX_HIP(ec);
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    DDX_HIP(ei);
+    // keyed on the buffers as they stand after the capture
+    h->round_graphs.push_back({round_key(h, lanes, count, nr), exec});
+  }
+  DDX_HIP(hipGraphLaunch(exec, l0.stream));
+  l0.enqueued += nr;
+  return DDX_OK;
+}
+}  // namespace
+
+void drop_round_graphs(ddx_handle *h) {
+  for (auto &g : h->round_graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  h->round_graphs.clear();
+}
+
+// Enqueues the next group of rounds and the copy of each lane's state header
+// (nmat, it, done, ok) into slot `slot` of its pinned h_head, marked by
+// h->gev[slot].  The first group of a solve runs eagerly; with graphs on
+// (and no kernel timer) the following groups are one graph launch each.
+int solve_enqueue(ddx_handle *h, Lane *const *lanes, int count, int slot) {
+  Lane &l0 = *lanes[0];
+  const bool pair = count == 2;
+  if (pair && (l0.star != 0 || lanes[1]->star != 1 || lanes[1]->stream != l0.stream ||
+               lanes[1]->diis_max != l0.diis_max || lanes[1]->rounds != l0.rounds)) {
+    set_error("paired solves need a direct and an adjoint solve with equal settings");
+    return DDX_ERR_ARG;
+  }
+  hipStream_t s = l0.stream;
+  h->tag_solve = l0.serial;
+  const int nr = std::min(l0.group, l0.rounds - l0.enqueued);
+  if (h->use_graphs && l0.group == 2 && nr == 2 && !timers_on(h))
+    DDX_TRY(launch_round_graph(h, lanes, count, nr));
+  else
+    DDX_TRY(enqueue_rounds(h, lanes, count, nr));
   h->tag_solve = -1;
   h->tag_round = 0;
   l0.group = 2;
@@ -922,6 +1056,153 @@ int solve_run(ddx_handle *h, Lane *const *lanes, int count) {
     drop_rounds(h, star ? DDX_K_EXPAND : DDX_K_PROJECT, serial, it[0]);
   }
   return DDX_OK;
+}
+
+namespace {
+// Inner solves of the refinement stop at max(tol, kRelTol |start|): the
+// increments of FP32 operators cannot fall much below 1e-7 of the iterate.
+constexpr double kRelTol = 1e-5;
+// The refinement stops once every instance's correction is below tol, the
+// reference's own test on the increment (jacobi_diis.f90:153-156); the
+// solve of that last correction starts below its threshold and takes one
+// or two iterations.
+constexpr double kStop = 1.0;
+constexpr int kMaxRefine = 6;
+// The corrections are solved to kFinal tol, so that the error left in x
+// (about the last threshold times rho / (1 - rho), rho the Jacobi/DIIS
+// contraction) stays below the reference's own.
+constexpr double kFinal = 0.2;
+
+// iterations and ok of each instance in the solve that just ended on ln
+void add_iterations(const ddx_handle *h, const Lane &ln, std::vector<int> &its,
+                    std::vector<int> &ok) {
+  for (int q = 0; q < h->ninst; ++q) {
+    const bool oq = h->ninst > 1 ? ln.h_res[2 * q + 1] != 0 : ln.h_diis->ok != 0;
+    const int iq = h->ninst > 1 ? (oq ? ln.h_res[2 * q] : ln.max_iter) : ln.h_diis->it;
+    its[q] += iq;
+    ok[q] = ok[q] && oq;
+  }
+}
+
+// per-instance norms of x into st[q].tol (raised to rel |x|_q) or out[q];
+// with xadd, xadd += x on the way
+int inst_norms(ddx_handle *h, Lane &ln, const double *x, double *xadd, double rel,
+               DiisState *st, double *out) {
+  hipLaunchKernelGGL(k_sq_part, dim3(h->nbj), dim3(kJThreads), 0, ln.stream, h->jblk.p, h->nylm,
+                     h->cf.p + h->nylm, x, xadd, ln.part.p);
+  DDX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_inst_norm, dim3(h->ninst), dim3(kWave), 0, ln.stream, h->jboff.p,
+                     h->soff.p, ln.part.p, rel, st, out);
+  DDX_HIP(hipGetLastError());
+  return DDX_OK;
+}
+
+// runs the inner solve on the lanes with their FP32 operators
+int run_f32(ddx_handle *h, Lane *const *lanes, int count) {
+  for (int k = 0; k < count; ++k) lanes[k]->f32 = true;
+  const int rc = solve_run(h, lanes, count);
+  for (int k = 0; k < count; ++k) lanes[k]->f32 = false;
+  return rc;
+}
+}  // namespace
+
+// Mixed precision: iterative refinement in FP64 around Jacobi/DIIS solves
+// whose sweeps and harmonic products run in FP32.  The lanes have been
+// begun (solve_begin) with the guess in x.  First the FP32 operator is
+// solved for x itself; then, until the corrections are small enough, the
+// FP64 residual r = g - (D + L) x is formed and (D + L) d = r is solved with
+// the FP32 operator, from D^-1 r, and x += d.  The Jacobi/DIIS kernels
+// (update, Gram row, bordered solve, extrapolation, norms) stay FP64.
+// Reported iterations are the inner iterations of all stages.
+int solve_run_mixed(ddx_handle *h, Lane *const *lanes, int count) {
+  DDX_TRY(make_tables32(h));
+  const int ninst = h->ninst;
+  const size_t nn = (size_t)h->nylm * h->n;
+  const double *keep_rhs[2];
+  double *keep_x[2];
+  double keep_tol[2];
+  std::vector<int> its[2], ok[2];
+  for (int k = 0; k < count; ++k) {
+    Lane &ln = *lanes[k];
+    keep_rhs[k] = ln.rhs;
+    keep_x[k] = ln.x;
+    keep_tol[k] = ln.tol;
+    its[k].assign(ninst, 0);
+    ok[k].assign(ninst, 1);
+    DDX_TRY(ln.rres.reserve(nn + 1));
+    DDX_TRY(ln.dcorr.reserve(nn + 1));
+    DDX_TRY(ln.dnorm.reserve(ninst + 1));
+    DDX_TRY(ln.part.reserve((size_t)h->nbj * kPartStride + h->nbe + 1));
+    DDX_TRY(inst_norms(h, ln, ln.x, nullptr, kRelTol, ln.diis.p, nullptr));
+  }
+  DDX_TRY(run_f32(h, lanes, count));
+  bool all_ok = true;
+  for (int k = 0; k < count; ++k) {
+    add_iterations(h, *lanes[k], its[k], ok[k]);
+    for (int q = 0; q < ninst; ++q) all_ok = all_ok && ok[k][q];
+  }
+  std::vector<double> dn(ninst);
+  for (int pass = 0; all_ok && pass < kMaxRefine; ++pass) {
+    // FP64 residuals of the current x
+    Lane *ld = nullptr, *la = nullptr;
+    for (int k = 0; k < count; ++k) (lanes[k]->star ? la : ld) = lanes[k];
+    h->mute_timers = true;
+    const int rc = apply_fp64(h, ld, ld ? keep_x[ld == lanes[0] ? 0 : 1] : nullptr, la,
+                              la ? keep_x[la == lanes[0] ? 0 : 1] : nullptr);
+    h->mute_timers = false;
+    DDX_TRY(rc);
+    for (int k = 0; k < count; ++k) {
+      Lane &ln = *lanes[k];
+      hipLaunchKernelGGL(k_residual, dim3((nn + 255) / 256), dim3(256), 0, ln.stream, nn,
+                         h->nylm, h->facl.p, keep_rhs[k], keep_x[k], ln.y.p, ln.rres.p);
+      DDX_HIP(hipGetLastError());
+      DDX_TRY(device_ldm1x(h, ln.stream, ln.rres.p, ln.dcorr.p));
+      DDX_TRY(solve_begin(h, ln, ln.star, ln.diis_max, kFinal * keep_tol[k], ln.rres.p,
+                          ln.dcorr.p, ln.max_iter));
+      DDX_TRY(inst_norms(h, ln, ln.dcorr.p, nullptr, kRelTol, ln.diis.p, nullptr));
+    }
+    DDX_TRY(run_f32(h, lanes, count));
+    bool small = true;
+    for (int k = 0; k < count; ++k) {
+      Lane &ln = *lanes[k];
+      add_iterations(h, ln, its[k], ok[k]);
+      DDX_TRY(inst_norms(h, ln, ln.dcorr.p, keep_x[k], 0.0, nullptr, ln.dnorm.p));
+      DDX_HIP(hipMemcpyAsync(dn.data(), ln.dnorm.p, sizeof(double) * ninst,
+                             hipMemcpyDeviceToHost, ln.stream));
+      DDX_HIP(hipStreamSynchronize(ln.stream));
+      for (int q = 0; q < ninst; ++q) {
+        all_ok = all_ok && ok[k][q];
+        small = small && dn[q] <= kStop * keep_tol[k];
+      }
+    }
+    if (small) break;
+    if (pass + 1 == kMaxRefine) all_ok = false;
+  }
+  // the lanes report the whole refinement
+  for (int k = 0; k < count; ++k) {
+    Lane &ln = *lanes[k];
+    ln.rhs = keep_rhs[k];
+    ln.x = keep_x[k];
+    ln.tol = keep_tol[k];
+    int itmax = 0;
+    for (int q = 0; q < ninst; ++q) {
+      const bool oq = all_ok && ok[k][q];
+      itmax = std::max(itmax, its[k][q]);
+      if (ninst > 1) {
+        ln.h_res[2 * q] = oq ? its[k][q] : ln.max_iter + 1;
+        ln.h_res[2 * q + 1] = oq;
+      }
+    }
+    ln.h_diis->it = itmax;
+    ln.h_diis->ok = all_ok ? 1 : 0;
+    if (!ln.star) DDX_TRY(cf_scale(h, ln.stream, ln.x, ln.xs.p));
+  }
+  return DDX_OK;
+}
+
+int solve_lanes(ddx_handle *h, Lane *const *lanes, int count) {
+  return h->prec == DDX_PREC_MIXED ? solve_run_mixed(h, lanes, count)
+                                   : solve_run(h, lanes, count);
 }
 
 int solve_jacobi_diis(ddx_handle *h, Lane &ln, int star, int diis_max, double tol,

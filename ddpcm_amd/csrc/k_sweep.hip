@@ -39,9 +39,12 @@ __host__ __device__ constexpr int lmi(int l, int m) { return l * l + l + m; }
 // paired sweep also evaluates L X and gets a smaller budget.  The budgets
 // are the largest that the register allocator fits in 256 VGPRs without
 // spilling (its resource report, at two waves per SIMD).
-__host__ __device__ constexpr int acc_cap(int lmax) { return (lmax == 13 || lmax == 14) ? 64 : 72; }
-constexpr int kAccPairLess = 16;   // budget taken by L X in the paired pass
-constexpr int kAccOne = 81;        // up to lmax 8 all entries in one block
+// FP32 accumulators (F) take one register each instead of two.
+__host__ __device__ constexpr int acc_cap(int lmax, bool F) {
+  return F ? 128 : ((lmax == 13 || lmax == 14) ? 64 : 72);
+}
+__host__ __device__ constexpr int acc_pair_less(bool F) { return F ? 32 : 16; }  // taken by L X
+__host__ __device__ constexpr int acc_one(bool F) { return F ? 144 : 81; }  // one block up to here
 
 // entries (l, +-m), l = m..lmax
 __host__ __device__ constexpr int mcount(int lmax, int m) {
@@ -54,11 +57,11 @@ __host__ __device__ constexpr int mspan(int lmax, int m0, int m1) {
   return c;
 }
 // first m of block b; lmax + 1 once the blocks are used up
-__host__ __device__ constexpr int mblock_begin(int lmax, bool dir, int b) {
-  if ((lmax + 1) * (lmax + 1) <= kAccOne) return b == 0 ? 0 : lmax + 1;
+__host__ __device__ constexpr int mblock_begin(int lmax, bool dir, int b, bool F) {
+  if ((lmax + 1) * (lmax + 1) <= acc_one(F)) return b == 0 ? 0 : lmax + 1;
   int m = 0;
   for (int k = 0; k < b && m <= lmax; ++k) {
-    const int cap = acc_cap(lmax) - ((k == 0 && dir) ? kAccPairLess : 0);
+    const int cap = acc_cap(lmax, F) - ((k == 0 && dir) ? acc_pair_less(F) : 0);
     int cnt = 0;
     while (m <= lmax && cnt + mcount(lmax, m) <= cap) cnt += mcount(lmax, m++);
   }
@@ -76,36 +79,35 @@ __host__ __device__ constexpr int aidx(int lmax, int m0, int m1, int l, int m, b
 // coef p~_lm trig_m to the accumulators of AM0 <= |m| < AM1 (trig_m =
 // cos(m phi) for m >= 0, sin(|m| phi) for m < 0).  Without DIR the
 // recurrences in m only step up to AM0 before the l recurrence starts.
-template <int LMAX, bool DIR, int AM0, int AM1>
-__device__ __forceinline__ double sweep_triple(double cthe, double sthe, double cphi, double sphi,
-                                               double t, const double *Xc,
-                                               double coef, double *acc) {
+template <typename R, int LMAX, bool DIR, int AM0, int AM1>
+__device__ __forceinline__ R sweep_triple(R cthe, R sthe, R cphi, R sphi, R t, const R *Xc, R coef,
+                                          R *acc) {
   constexpr bool ADJ = AM0 < AM1;
   constexpr int MB = DIR ? 0 : AM0;          // first m with the l recurrence
   constexpr int ME = DIR ? LMAX + 1 : AM1;   // one past the last m needed
-  const double ct = cthe * t, st = sthe * t, t2 = t * t;
-  double s = 0.0;
-  double pmm = 1.0;   // t^m P_m^m
-  double cm = 1.0, sm = 0.0, cprev = 0.0, sprev = 0.0;
+  const R ct = cthe * t, st = sthe * t, t2 = t * t;
+  R s = 0.0;
+  R pmm = 1.0;   // t^m P_m^m
+  R cm = 1.0, sm = 0.0, cprev = 0.0, sprev = 0.0;
 #pragma unroll
   for (int m = 0; m < ME; ++m) {
     if (m >= MB) {
       const bool acc_m = ADJ && m >= AM0 && m < AM1;
-      double qc = 0.0, qs = 0.0;
+      R qc = 0.0, qs = 0.0;
       if (acc_m) {
         qc = (m == 0) ? coef : coef * cm;
-        qs = (m == 0) ? 0.0 : coef * sm;
+        qs = (m == 0) ? R(0) : coef * sm;
       }
-      double a = 0.0, b = 0.0;
+      R a = 0.0, b = 0.0;
       // X' of this m is read from LDS only after the previous m's sum: the
       // compiler would otherwise keep all nylm values in registers
-      const double *Xm = Xc;
+      const R *Xm = Xc;
       if (DIR) {
         int off = 0;
         asm volatile("" : "+v"(off) : "v"(s));
         Xm = Xc + off;
       }
-      auto visit = [&](int l, double p) {
+      auto visit = [&](int l, R p) {
         if (DIR) {
           a += p * Xm[lmi(l, m)];
           if (m > 0) b += p * Xm[lmi(l, -m)];
@@ -115,17 +117,17 @@ __device__ __forceinline__ double sweep_triple(double cthe, double sthe, double 
           if (m > 0) acc[aidx(LMAX, AM0, AM1, l, m, true)] += p * qs;
         }
       };
-      double p0 = pmm;
+      R p0 = pmm;
       visit(m, p0);
       if (m < LMAX) {
-        double p1 = ((double)(2 * m + 1) * ct) * p0;
+        R p1 = ((R)(2 * m + 1) * ct) * p0;
         visit(m + 1, p1);
 #pragma unroll
         for (int l = m + 2; l <= LMAX; ++l) {
-          constexpr double one = 1.0;
-          const double ca = (double)(2 * l - 1) * (one / (double)(l - m));
-          const double cb = (double)(l + m - 1) * (one / (double)(l - m));
-          const double p2 = ca * (ct * p1) - cb * (t2 * p0);
+          constexpr R one = 1;
+          const R ca = (R)(2 * l - 1) * (one / (R)(l - m));
+          const R cb = (R)(l + m - 1) * (one / (R)(l - m));
+          const R p2 = ca * (ct * p1) - cb * (t2 * p0);
           visit(l, p2);
           p0 = p1;
           p1 = p2;
@@ -134,10 +136,10 @@ __device__ __forceinline__ double sweep_triple(double cthe, double sthe, double 
       if (DIR) s += (m == 0) ? a : (cm * a + sm * b);
     }
     if (m + 1 < ME) {
-      pmm = (-(double)(2 * m + 1) * st) * pmm;
-      double cn, sn;
+      pmm = (-(R)(2 * m + 1) * st) * pmm;
+      R cn, sn;
       if (m == 0) { cn = cphi; sn = sphi; }
-      else { cn = 2.0 * cphi * cm - cprev; sn = 2.0 * cphi * sm - sprev; }
+      else { cn = R(2) * cphi * cm - cprev; sn = R(2) * cphi * sm - sprev; }
       cprev = cm; sprev = sm; cm = cn; sm = sn;
     }
   }
@@ -167,15 +169,34 @@ __device__ __forceinline__ double dpp_partner(double v) {
 // pairs lanes 32 apart and stage 1 lanes 16 apart by the gfx950 row swaps,
 // stages 2..5 use DPP.  After six stages each lane holds the wave sums of
 // columns base .. base + len - 1.
-template <int S, int N>
-__device__ __forceinline__ void wave_reduce_split(double *v, int &base, int &len) {
+// FP32 values move as one 32-bit word.
+template <int S>
+__device__ __forceinline__ float dpp_partner(float v) {
+  constexpr int ctrl = S == 2 ? 0x140 : S == 3 ? 0x141 : S == 4 ? 0x4E : 0xB1;
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xf, 0xf, false));
+}
+
+template <int S, int N, typename R>
+__device__ __forceinline__ void wave_reduce_split(R *v, int &base, int &len) {
   constexpr int H = (N + 1) / 2;
   const bool side = (threadIdx.x >> (5 - S)) & 1;
 #pragma unroll
   for (int k = 0; k < H; ++k) {
-    const double x = v[k];
-    const double y = (k + H < N) ? v[k + H] : 0.0;
-    if constexpr (S < 2) {
+    const R x = v[k];
+    const R y = (k + H < N) ? v[k + H] : R(0);
+    if constexpr (S < 2 && sizeof(R) == 4) {
+      const unsigned xb = __float_as_uint(x), yb = __float_as_uint(y);
+      unsigned nxb, nyb;
+      if constexpr (S == 0) {
+        auto r = __builtin_amdgcn_permlane32_swap(xb, yb, false, false);
+        nxb = r[0]; nyb = r[1];
+      } else {
+        auto r = __builtin_amdgcn_permlane16_swap(xb, yb, false, false);
+        nxb = r[0]; nyb = r[1];
+      }
+      v[k] = __uint_as_float(nxb) + __uint_as_float(nyb);
+    } else if constexpr (S < 2) {
       const long long bx = __double_as_longlong(x), by = __double_as_longlong(y);
       const unsigned xl = (unsigned)(bx & 0xffffffffll), xh = (unsigned)(bx >> 32);
       const unsigned yl = (unsigned)(by & 0xffffffffll), yh = (unsigned)(by >> 32);
@@ -206,24 +227,24 @@ __device__ __forceinline__ void wave_reduce_split(double *v, int &base, int &len
   } else {
     len = len < H ? len : H;
   }
-  if constexpr (S < 5) wave_reduce_split<S + 1, H>(v, base, len);
+  if constexpr (S < 5) wave_reduce_split<S + 1, H, R>(v, base, len);
 }
 
 // One pass over triples beg..end-1 of a container: rw[tslot] = W (R . X_c)
 // when DIR (X' in xsh), and when AM0 < AM1 the (l, +-m), AM0 <= m < AM1,
 // entries of the chunk's share of (L*S)_c, -cf * sum_t w_n xi(n, o) W p~
 // trig, into the segment row yp.
-template <int LMAX, bool DIR, int AM0, int AM1>
+template <typename R, int LMAX, bool DIR, int AM0, int AM1>
 __device__ __forceinline__ void sweep_pass(
     int beg, int end, const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
-    const int *__restrict__ tslot, const int *__restrict__ tp, const double *xsh,
-    double *__restrict__ rw, const double *__restrict__ xiw, const double *__restrict__ cf,
+    const int *__restrict__ tslot, const int *__restrict__ tp, const R *xsh,
+    R *__restrict__ rw, const R *__restrict__ xiw, const double *__restrict__ cf,
     double *__restrict__ yp) {
   constexpr bool ADJ = AM0 < AM1;
   constexpr int NA = ADJ ? mspan(LMAX, AM0, AM1) : 1;
-  double acc[NA];
+  R acc[NA];
 #pragma unroll
-  for (int k = 0; k < NA; ++k) acc[k] = 0.0;
+  for (int k = 0; k < NA; ++k) acc[k] = R(0);
   // software pipeline, two triples deep: geometry, slot and owner point of
   // triple k + 128 and xi of triple k + 64 load while triple k is evaluated
   int k = beg + (int)threadIdx.x;
@@ -232,7 +253,7 @@ __device__ __forceinline__ void sweep_pass(
   double4 g0 = z4, g1 = z4;
   double2 w0 = z2, w1 = z2;
   int s0 = 0, s1 = 0, p1 = 0;
-  double x0 = 0.0;
+  R x0 = R(0);
   if (k < end) {
     g0 = tgeo[k];
     w0 = ttw[k];
@@ -256,12 +277,12 @@ __device__ __forceinline__ void sweep_pass(
       if (DIR) s2 = tslot[k2];
       if (ADJ) p2 = tp[k2];
     }
-    double x1 = 0.0;
+    R x1 = R(0);
     if (ADJ && k + kWave < end) x1 = xiw[p1];
-    const double coef = ADJ ? x0 * w0.y : 0.0;
-    const double v = sweep_triple<LMAX, DIR, AM0, AM1>(g0.x, g0.y, g0.z, g0.w, w0.x, xsh, coef,
-                                                       acc);
-    if (DIR) rw[s0] = w0.y * v;
+    const R coef = ADJ ? x0 * (R)w0.y : R(0);
+    const R v = sweep_triple<R, LMAX, DIR, AM0, AM1>((R)g0.x, (R)g0.y, (R)g0.z, (R)g0.w,
+                                                     (R)w0.x, xsh, coef, acc);
+    if (DIR) rw[s0] = (R)w0.y * v;
     g0 = g1; g1 = g2;
     w0 = w1; w1 = w2;
     s0 = s1; s1 = s2;
@@ -270,7 +291,7 @@ __device__ __forceinline__ void sweep_pass(
   if constexpr (ADJ) {
     // sum of acc over the 64 lanes, entry by entry, spread over the lanes
     int base = 0, len = NA;
-    wave_reduce_split<0, NA>(acc, base, len);
+    wave_reduce_split<0, NA, R>(acc, base, len);
     constexpr int NF = reduced_len(NA, 6);
 #pragma unroll
     for (int j = 0; j < NF; ++j) {
@@ -287,45 +308,47 @@ __device__ __forceinline__ void sweep_pass(
             lm = lmi(l, (m > 0 && (r & 1)) ? -m : m);
           }
         }
-        yp[lm] = -(cf[lm] * acc[j]);
+        yp[lm] = -(cf[lm] * (double)acc[j]);
       }
     }
   }
 }
 
 // The adjoint passes of blocks B.. (the first also evaluates L X when DIR)
-template <int LMAX, bool DIR, int B>
+template <typename R, int LMAX, bool DIR, int B>
 __device__ __forceinline__ void sweep_blocks(
     int beg, int end, const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
-    const int *__restrict__ tslot, const int *__restrict__ tp, const double *xsh,
-    double *__restrict__ rw, const double *__restrict__ xiw, const double *__restrict__ cf,
+    const int *__restrict__ tslot, const int *__restrict__ tp, const R *xsh,
+    R *__restrict__ rw, const R *__restrict__ xiw, const double *__restrict__ cf,
     double *__restrict__ yp) {
-  constexpr int M0 = mblock_begin(LMAX, DIR, B), M1 = mblock_begin(LMAX, DIR, B + 1);
-  sweep_pass<LMAX, DIR && B == 0, M0, M1>(beg, end, tgeo, ttw, tslot, tp, xsh, rw, xiw, cf, yp);
-  if constexpr (M1 <= LMAX) sweep_blocks<LMAX, DIR, B + 1>(beg, end, tgeo, ttw, tslot, tp, xsh,
-                                                          rw, xiw, cf, yp);
+  constexpr bool F = sizeof(R) == 4;
+  constexpr int M0 = mblock_begin(LMAX, DIR, B, F), M1 = mblock_begin(LMAX, DIR, B + 1, F);
+  sweep_pass<R, LMAX, DIR && B == 0, M0, M1>(beg, end, tgeo, ttw, tslot, tp, xsh, rw, xiw, cf,
+                                             yp);
+  if constexpr (M1 <= LMAX) sweep_blocks<R, LMAX, DIR, B + 1>(beg, end, tgeo, ttw, tslot, tp,
+                                                             xsh, rw, xiw, cf, yp);
 }
 
 // Triples beg..end-1 of container c: rw[tslot] = W (R . X_c) when DIR, and
 // when ADJ the chunk's share of (L*S)_c into the segment row yp.
-template <int LMAX, bool DIR, bool ADJ>
+template <typename R, int LMAX, bool DIR, bool ADJ>
 __device__ __forceinline__ void sweep_chunk(
     int c, int beg, int end, const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
     const int *__restrict__ tslot, const int *__restrict__ tp, const double *__restrict__ Xs,
-    double *__restrict__ rw, const double *__restrict__ xiw, const double *__restrict__ cf,
+    R *__restrict__ rw, const R *__restrict__ xiw, const double *__restrict__ cf,
     double *__restrict__ yp) {
   constexpr int NY = (LMAX + 1) * (LMAX + 1);
-  // X'_c in LDS: every lane reads the same word (broadcast)
-  __shared__ double xsh[DIR ? NY : 1];
+  // X'_c in LDS (rounded to R): every lane reads the same word (broadcast)
+  __shared__ R xsh[DIR ? NY : 1];
   if (DIR) {
     __syncthreads();   // reads of the previous segment's X' are done
-    for (int k = threadIdx.x; k < NY; k += kWave) xsh[k] = Xs[(size_t)c * NY + k];
+    for (int k = threadIdx.x; k < NY; k += kWave) xsh[k] = (R)Xs[(size_t)c * NY + k];
     __syncthreads();
   }
   if constexpr (ADJ)
-    sweep_blocks<LMAX, DIR, 0>(beg, end, tgeo, ttw, tslot, tp, xsh, rw, xiw, cf, yp);
+    sweep_blocks<R, LMAX, DIR, 0>(beg, end, tgeo, ttw, tslot, tp, xsh, rw, xiw, cf, yp);
   else
-    sweep_pass<LMAX, true, 0, 0>(beg, end, tgeo, ttw, tslot, tp, xsh, rw, xiw, cf, yp);
+    sweep_pass<R, LMAX, true, 0, 0>(beg, end, tgeo, ttw, tslot, tp, xsh, rw, xiw, cf, yp);
 }
 
 // One wave per worker (blockIdx.x of G): units floor(w U / G) ..
@@ -336,13 +359,13 @@ __device__ __forceinline__ void sweep_chunk(
 // container of an instance that has converged.
 // (two waves per SIMD: the register budget that keeps a block of
 // accumulators and the two-deep load pipeline without spilling)
-template <int LMAX, bool DIR, bool ADJ>
+template <typename R, int LMAX, bool DIR, bool ADJ>
 __global__ __launch_bounds__(kWave, 2) void k_sweep(
     long long U, long long G, const int *__restrict__ uoff, const int *__restrict__ segoff,
     const int *__restrict__ inl, const int *__restrict__ pairoff, int n,
     const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
     const int *__restrict__ tslot, const int *__restrict__ tp, const double *__restrict__ Xs,
-    double *__restrict__ rw, const double *__restrict__ xiw, const double *__restrict__ cf,
+    R *__restrict__ rw, const R *__restrict__ xiw, const double *__restrict__ cf,
     double *__restrict__ yp, const DiisState *__restrict__ gdir,
     const DiisState *__restrict__ gadj, const int *__restrict__ inst, int ninst) {
   const bool dir = DIR && !(gdir && gdir->done);
@@ -384,17 +407,17 @@ __global__ __launch_bounds__(kWave, 2) void k_sweep(
     }
     if constexpr (DIR && ADJ) {
       if (dc && ac)
-        sweep_chunk<LMAX, true, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
+        sweep_chunk<R, LMAX, true, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
       else if (dc)
-        sweep_chunk<LMAX, true, false>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
+        sweep_chunk<R, LMAX, true, false>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
       else if (ac)
-        sweep_chunk<LMAX, false, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
+        sweep_chunk<R, LMAX, false, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
     } else if constexpr (DIR) {
       if (dc)
-        sweep_chunk<LMAX, true, false>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
+        sweep_chunk<R, LMAX, true, false>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
     } else {
       if (ac)
-        sweep_chunk<LMAX, false, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
+        sweep_chunk<R, LMAX, false, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
     }
     u = ue;
     ++seg;
@@ -404,14 +427,15 @@ __global__ __launch_bounds__(kWave, 2) void k_sweep(
 // P(n,o) = sum of the point's weighted terms, in neighbour order.  Points
 // with ui = 1, which calcv skips (ddcosmo.f90:1698), are exactly the points
 // without triples (ui < 1 iff some t < 1), so their P is zero here as well.
+template <typename R>
 __global__ __launch_bounds__(256) void k_gather_points(size_t np, const int *__restrict__ ptoff,
-                                                       const double *__restrict__ rw,
-                                                       double *__restrict__ P,
+                                                       const R *__restrict__ rw,
+                                                       R *__restrict__ P,
                                                        const DiisState *__restrict__ gate) {
   if (gate && gate->done) return;
   size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  double s = 0.0;
+  R s = R(0);
   const int b = ptoff[p], e = ptoff[p + 1];
   for (int k = b; k < e; ++k) s += rw[k];
   P[p] = s;
@@ -437,61 +461,116 @@ __global__ void k_cf_scale(size_t nn, int nylm, const double *__restrict__ cf,
   if (k < nn) xs[k] = cf[k % nylm] * x[k];
 }
 
-template <int LMAX, bool DIR, bool ADJ>
-int launch_sweep(ddx_handle *h, hipStream_t s, const double *Xs, double *rw, const double *xiw,
-                 double *yp, const DiisState *gdir, const DiisState *gadj) {
+template <typename R, int LMAX, bool DIR, bool ADJ>
+int launch_sweep(ddx_handle *h, hipStream_t s, const double *Xs, R *rw, const R *xiw, double *yp,
+                 const DiisState *gdir, const DiisState *gadj) {
   if (h->nworker == 0) return DDX_OK;
   TimedLaunch tl(h, (DIR && ADJ) ? DDX_K_SWEEP_PAIR : (DIR ? DDX_K_SWEEP_L : DDX_K_SWEEP_LSTAR), s);
-  hipExtLaunchKernelGGL((k_sweep<LMAX, DIR, ADJ>), dim3(h->nworker), dim3(kWave), 0, s, tl.b,
-                        tl.e, 0, (long long)h->nunit, (long long)h->nworker, h->uoff.p, h->segoff.p, h->inl.p,
-                     h->pairoff.p, h->n, h->tgeo.p, h->ttw.p, h->tslot.p, h->tp.p, Xs, rw, xiw,
-                     h->cf.p, yp, gdir, gadj, h->ninst > 1 ? h->inst.p : nullptr, h->ninst);
+  hipExtLaunchKernelGGL((k_sweep<R, LMAX, DIR, ADJ>), dim3(h->nworker), dim3(kWave), 0, s, tl.b,
+                        tl.e, 0, (long long)h->nunit, (long long)h->nworker, h->uoff.p,
+                        h->segoff.p, h->inl.p, h->pairoff.p, h->n, h->tgeo.p, h->ttw.p,
+                        h->tslot.p, h->tp.p, Xs, rw, xiw, h->cf.p, yp, gdir, gadj,
+                        h->ninst > 1 ? h->inst.p : nullptr, h->ninst);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
 
-template <int LMAX>
-int sweep_lmax(ddx_handle *h, hipStream_t s, int mode, const double *Xs, double *rw,
-               const double *xiw, double *y, const DiisState *gdir, const DiisState *gadj) {
-  if (mode == 3) return launch_sweep<LMAX, true, true>(h, s, Xs, rw, xiw, y, gdir, gadj);
-  if (mode == 1) return launch_sweep<LMAX, true, false>(h, s, Xs, rw, xiw, y, gdir, gadj);
-  return launch_sweep<LMAX, false, true>(h, s, Xs, rw, xiw, y, gdir, gadj);
+template <typename R, int LMAX>
+int sweep_lmax(ddx_handle *h, hipStream_t s, int mode, const double *Xs, R *rw, const R *xiw,
+               double *y, const DiisState *gdir, const DiisState *gadj) {
+  if (mode == 3) return launch_sweep<R, LMAX, true, true>(h, s, Xs, rw, xiw, y, gdir, gadj);
+  if (mode == 1) return launch_sweep<R, LMAX, true, false>(h, s, Xs, rw, xiw, y, gdir, gadj);
+  return launch_sweep<R, LMAX, false, true>(h, s, Xs, rw, xiw, y, gdir, gadj);
 }
 
-// mode: 1 = L X terms, 2 = L* S shares per segment, 3 = both
-int sweep(ddx_handle *h, hipStream_t s, int mode, const double *Xs, double *rw,
-          const double *xiw, double *y, const DiisState *gdir, const DiisState *gadj) {
+// mode: 1 = L X terms, 2 = L* S shares per segment, 3 = both.  R: the
+// arithmetic of the sweep (double, or float inside the refinement of the
+// mixed precision); X' and the L* S shares stay double.
+template <typename R>
+int sweep(ddx_handle *h, hipStream_t s, int mode, const double *Xs, R *rw, const R *xiw,
+          double *y, const DiisState *gdir, const DiisState *gadj) {
   if (h->n == 0) return DDX_OK;
   switch (h->lmax) {
-    case 0: return sweep_lmax<0>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
-    case 1: return sweep_lmax<1>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
-    case 2: return sweep_lmax<2>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
-    case 3: return sweep_lmax<3>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
-    case 4: return sweep_lmax<4>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
-    case 5: return sweep_lmax<5>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
-    case 6: return sweep_lmax<6>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
-    case 7: return sweep_lmax<7>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
-    case 8: return sweep_lmax<8>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
-    case 9: return sweep_lmax<9>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
-    case 10: return sweep_lmax<10>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
-    case 11: return sweep_lmax<11>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
-    case 12: return sweep_lmax<12>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
-    case 13: return sweep_lmax<13>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
-    case 14: return sweep_lmax<14>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
-    case 15: return sweep_lmax<15>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
-    case 16: return sweep_lmax<16>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 0: return sweep_lmax<R, 0>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 1: return sweep_lmax<R, 1>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 2: return sweep_lmax<R, 2>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 3: return sweep_lmax<R, 3>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 4: return sweep_lmax<R, 4>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 5: return sweep_lmax<R, 5>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 6: return sweep_lmax<R, 6>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 7: return sweep_lmax<R, 7>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 8: return sweep_lmax<R, 8>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 9: return sweep_lmax<R, 9>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 10: return sweep_lmax<R, 10>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 11: return sweep_lmax<R, 11>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 12: return sweep_lmax<R, 12>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 13: return sweep_lmax<R, 13>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 14: return sweep_lmax<R, 14>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 15: return sweep_lmax<R, 15>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
+    case 16: return sweep_lmax<R, 16>(h, s, mode, Xs, rw, xiw, y, gdir, gadj);
     default: set_error("lmax %d not supported (0..16)", h->lmax); return DDX_ERR_ARG;
   }
 }
 
 // P = gathered L X terms of each point; y = - Yw^T-projection of P
+template <typename R>
 int finish_lx(ddx_handle *h, Lane &ln, double *y, const DiisState *gate, int ysplit) {
   hipStream_t s = ln.stream;
-  hipLaunchKernelGGL(k_gather_points, dim3((h->np + 255) / 256), dim3(256), 0, s, h->np,
-                     h->ptoff.p, ln.rw.p, ln.P.p, gate);
+  R *rw = reinterpret_cast<R *>(ln.rw.p);
+  R *P = reinterpret_cast<R *>(ln.P.p);
+  hipLaunchKernelGGL(k_gather_points<R>, dim3((h->np + 255) / 256), dim3(256), 0, s, h->np,
+                     h->ptoff.p, rw, P, gate);
   DDX_HIP(hipGetLastError());
   // y(lm, o) = - sum_n Yw(lm, n) P(n, o)
-  return project(h, s, -1.0, ln.P.p, y, gate, ysplit);
+  if constexpr (sizeof(R) == 4) return project32(h, s, -1.0, P, y, gate, ysplit);
+  else return project(h, s, -1.0, P, y, gate, ysplit);
+}
+
+// the lane's point values of the L* S sweep: w_n xi(n, o) in R
+template <typename R>
+int expand_xi(ddx_handle *h, Lane &ln, const double *x, const DiisState *gate) {
+  R *xiw = reinterpret_cast<R *>(ln.xiw.p);
+  if constexpr (sizeof(R) == 4) return expand32(h, ln.stream, x, xiw, true, gate);
+  else return expand(h, ln.stream, x, xiw, true, gate);
+}
+
+template <typename R>
+int lx_r(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate,
+         bool xs_ready, int ysplit) {
+  if (!xs_ready) DDX_TRY(cf_scale(h, ln.stream, x, ln.xs.p));
+  DDX_TRY(sweep<R>(h, ln.stream, 1, ln.xs.p, reinterpret_cast<R *>(ln.rw.p), nullptr, nullptr,
+                   gate, nullptr));
+  return finish_lx<R>(h, ln, y, gate, ysplit);
+}
+
+template <typename R>
+int lstarx_r(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate,
+             bool want_y) {
+  // w_n xi(n, o) = w_n sum_lm Y(lm, n) x(lm, o)
+  DDX_TRY(expand_xi<R>(h, ln, x, gate));
+  DDX_TRY(sweep<R>(h, ln.stream, 2, nullptr, nullptr, reinterpret_cast<const R *>(ln.xiw.p),
+                   ln.yp.p, nullptr, gate));
+  return want_y ? seg_sum(h, ln.stream, ln.yp.p, y, gate) : DDX_OK;
+}
+
+template <typename R>
+int pair_r(ddx_handle *h, Lane &ld, double *yd, const DiisState *gd, Lane &la, const double *xa,
+           const DiisState *ga, int ysplit) {
+  DDX_TRY(expand_xi<R>(h, la, xa, ga));
+  DDX_TRY(sweep<R>(h, ld.stream, 3, ld.xs.p, reinterpret_cast<R *>(ld.rw.p),
+                   reinterpret_cast<const R *>(la.xiw.p), la.yp.p, gd, ga));
+  return finish_lx<R>(h, ld, yd, gd, ysplit);
+}
+
+int reserve_lx(ddx_handle *h, Lane &ln) {
+  DDX_TRY(ln.rw.reserve(h->ntri + 1));
+  DDX_TRY(ln.P.reserve(h->np));
+  return ln.xs.reserve((size_t)h->nylm * h->n + 1);
+}
+
+int reserve_lstarx(ddx_handle *h, Lane &ln) {
+  DDX_TRY(ln.xiw.reserve(h->np));
+  return ln.yp.reserve((size_t)h->nseg_cap * h->nylm + 1);
 }
 
 }  // namespace
@@ -505,18 +584,6 @@ int cf_scale(ddx_handle *h, hipStream_t s, const double *x, double *xs) {
   return DDX_OK;
 }
 
-int apply_lx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate,
-             bool xs_ready, int ysplit) {
-  hipStream_t s = ln.stream;
-  size_t nn = (size_t)h->nylm * h->n;
-  DDX_TRY(ln.rw.reserve(h->ntri + 1));
-  DDX_TRY(ln.P.reserve(h->np));
-  DDX_TRY(ln.xs.reserve(nn + 1));
-  if (!xs_ready) DDX_TRY(cf_scale(h, s, x, ln.xs.p));
-  DDX_TRY(sweep(h, s, 1, ln.xs.p, ln.rw.p, nullptr, nullptr, gate, nullptr));
-  return finish_lx(h, ln, y, gate, ysplit);
-}
-
 int seg_sum(ddx_handle *h, hipStream_t s, const double *yp, double *y, const DiisState *gate) {
   size_t nn = (size_t)h->nylm * h->n;
   if (nn == 0) return DDX_OK;
@@ -526,27 +593,48 @@ int seg_sum(ddx_handle *h, hipStream_t s, const double *yp, double *y, const Dii
   return DDX_OK;
 }
 
+// The applies below run in the lane's precision: ln.f32 selects the FP32
+// sweeps, gather and MFMA products of the mixed precision.
+int apply_lx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate,
+             bool xs_ready, int ysplit) {
+  DDX_TRY(reserve_lx(h, ln));
+  return ln.f32 ? lx_r<float>(h, ln, x, y, gate, xs_ready, ysplit)
+                : lx_r<double>(h, ln, x, y, gate, xs_ready, ysplit);
+}
+
 int apply_lstarx(ddx_handle *h, Lane &ln, const double *x, double *y, const DiisState *gate,
                  bool want_y) {
-  hipStream_t s = ln.stream;
-  DDX_TRY(ln.xiw.reserve(h->np));
-  DDX_TRY(ln.yp.reserve((size_t)h->nseg_cap * h->nylm + 1));
-  // w_n xi(n, o) = w_n sum_lm Y(lm, n) x(lm, o)
-  DDX_TRY(expand(h, s, x, ln.xiw.p, true, gate));
-  DDX_TRY(sweep(h, s, 2, nullptr, nullptr, ln.xiw.p, ln.yp.p, nullptr, gate));
-  return want_y ? seg_sum(h, s, ln.yp.p, y, gate) : DDX_OK;
+  DDX_TRY(reserve_lstarx(h, ln));
+  return ln.f32 ? lstarx_r<float>(h, ln, x, y, gate, want_y)
+                : lstarx_r<double>(h, ln, x, y, gate, want_y);
 }
 
 int apply_pair(ddx_handle *h, Lane &ld, double *yd, Lane &la, const double *xa, int ysplit) {
-  hipStream_t s = ld.stream;
-  DDX_TRY(ld.rw.reserve(h->ntri + 1));
-  DDX_TRY(ld.P.reserve(h->np));
-  DDX_TRY(la.xiw.reserve(h->np));
-  DDX_TRY(la.yp.reserve((size_t)h->nseg_cap * h->nylm + 1));
+  DDX_TRY(reserve_lx(h, ld));
+  DDX_TRY(reserve_lstarx(h, la));
   const DiisState *gd = solve_gate(h, ld), *ga = solve_gate(h, la);
-  DDX_TRY(expand(h, s, xa, la.xiw.p, true, ga));
-  DDX_TRY(sweep(h, s, 3, ld.xs.p, ld.rw.p, la.xiw.p, la.yp.p, gd, ga));
-  return finish_lx(h, ld, yd, gd, ysplit);
+  return ld.f32 ? pair_r<float>(h, ld, yd, gd, la, xa, ga, ysplit)
+                : pair_r<double>(h, ld, yd, gd, la, xa, ga, ysplit);
+}
+
+// FP64 operators of the lanes' current iterates for the residuals of the
+// refinement, ungated: L x into ld.y (x' = cf x rebuilt in ld.xs) and
+// L* s into la.y, in one sweep when both are asked for.
+int apply_fp64(ddx_handle *h, Lane *ld, const double *xd, Lane *la, const double *xa) {
+  if (ld) {
+    DDX_TRY(reserve_lx(h, *ld));
+    DDX_TRY(cf_scale(h, ld->stream, xd, ld->xs.p));
+  }
+  if (la) DDX_TRY(reserve_lstarx(h, *la));
+  if (ld && la) {
+    DDX_TRY(pair_r<double>(h, *ld, ld->y.p, nullptr, *la, xa, nullptr, 1));
+  } else if (ld) {
+    DDX_TRY(lx_r<double>(h, *ld, xd, ld->y.p, nullptr, true, 1));
+  } else if (la) {
+    DDX_TRY(lstarx_r<double>(h, *la, xa, nullptr, nullptr, false));
+  }
+  if (la) DDX_TRY(seg_sum(h, la->stream, la->yp.p, la->y.p, nullptr));
+  return DDX_OK;
 }
 
 }  // namespace ddx

@@ -29,7 +29,7 @@ EXPORTS = (
     "ddx_set_precision", "ddx_get_stats", "ddx_kernel_timer", "ddx_kernel_timer_read",
     "ddx_batch_init", "ddx_batch_energy_forces", "ddx_batch_result", "ddx_batch_iterations",
     "ddx_batch_sizes", "ddx_gemm_bf16", "ddx_set_guess", "ddx_guess_train", "ddx_guess_reset",
-    "ddx_mlp_grad",
+    "ddx_mlp_grad", "ddx_set_graphs",
 )
 DDX_GUESS_REFERENCE, DDX_GUESS_PREVIOUS, DDX_GUESS_LEARNED, DDX_GUESS_LEARNED_PREVIOUS = 0, 1, 2, 3
 KERNELS = ("sweep_l", "sweep_lstar", "project", "expand", "jacobi", "diis", "extrap", "field",
@@ -102,6 +102,7 @@ def load_library(path: str = LIB_PATH):
     _u16 = C.POINTER(C.c_uint16)
     L.ddx_gemm_bf16.argtypes = [C.c_int, C.c_int, C.c_int, _u16, _u16, C.POINTER(C.c_float)]
     L.ddx_set_guess.argtypes = [C.c_void_p, C.c_int]
+    L.ddx_set_graphs.argtypes = [C.c_void_p, C.c_int]
     L.ddx_guess_train.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp]
     L.ddx_guess_reset.argtypes = [C.c_void_p, C.c_uint]
     _fp = C.POINTER(C.c_float)
@@ -175,7 +176,9 @@ class DDCosmo:
         shapes = {"grid": (self.ngrid, 3), "w": (self.ngrid,), "basis": (self.ngrid, self.nylm),
                   "facl": (self.nylm,), "facs": (self.nylm,), "fi": (self.nsph, self.ngrid),
                   "ui": (self.nsph, self.ngrid), "zi": (self.nsph, self.ngrid, 3),
-                  "ccav": (self.ncav, 3), "csph": (self.nsph, 3), "rsph": (self.nsph,)}
+                  "ccav": (self.ncav, 3), "csph": (self.nsph, 3), "rsph": (self.nsph,),
+                  "rhs": (self.nsph, self.nylm), "xi": (self.nsph, self.ngrid),
+                  "sigma": (self.nsph, self.nylm), "s": (self.nsph, self.nylm)}
         a = np.zeros(shapes[name])
         self._check(self.L.ddx_get_array(self.h, name.encode(), _p(a)))
         return a
@@ -324,6 +327,10 @@ class DDCosmo:
         loss = np.zeros(2)
         self._check(self.L.ddx_guess_train(self.h, int(steps), float(lr), _p(loss)))
         return float(loss[0]), float(loss[1])
+
+    def set_graphs(self, enable=True):
+        """Replay the solver's iteration groups as captured HIP graphs."""
+        self._check(self.L.ddx_set_graphs(self.h, int(bool(enable))))
 
     def guess_reset(self, seed=1):
         self._check(self.L.ddx_guess_reset(self.h, int(seed)))

@@ -67,7 +67,8 @@ int ddx_get_sizes(const ddx_handle *h, int *nsph, int *ncav, int *nylm,
  * "fi", "ui" (ngrid,nsph), "zi" (3,ngrid,nsph); and the intermediates the
  * reference prints at iprint >= 4: "rhs" (nylm,nsph) of the last direct
  * solve (cosmo.f90:131), "xi" (ngrid,nsph) of the last forces
- * (forces_dd.f90:116). */
+ * (forces_dd.f90:116); and "sigma", "s" (nylm,nsph), the last direct and
+ * adjoint solutions. */
 int ddx_get_array(const ddx_handle *h, const char *name, double *dst);
 /* inl (nsph+1, 1-based as in the reference) and nl (inl[nsph]-1 entries) */
 int ddx_get_neighbours(const ddx_handle *h, int *inl, int *nl);
@@ -156,6 +157,15 @@ int ddx_batch_result(ddx_handle *h, double *esolv, double *fx);
 int ddx_batch_iterations(const ddx_handle *h, int *direct, int *adjoint);
 int ddx_batch_sizes(const ddx_handle *h, int *count, int *sph_off, int *cav_off);
 
+/* Precision of the solves (no counterpart in the reference, which is FP64
+ * throughout).  DDX_PREC_MIXED runs the L X and L* S sweeps and the
+ * harmonic products (projection, expansion) in FP32 with FP32 accumulation,
+ * inside FP64 iterative refinement: the residual g - (D + L) x, the
+ * Jacobi/DIIS update, Gram row and bordered solve, the energies and the
+ * forces stay FP64.  Results agree with FP64 to well inside the
+ * convergence threshold; iteration counts are those of the inner solves
+ * and differ from the reference's.  ddx_lx / ddx_lstarx apply the FP32
+ * operators while it is set; ddx_jacobi_diis always runs FP64. */
 int ddx_set_precision(ddx_handle *h, int prec);
 
 /* Initial guess of the solves (no counterpart in the reference beyond
@@ -203,6 +213,14 @@ int ddx_mlp_grad(int n, int in, int out, const unsigned short *F, const float *T
 int ddx_gemm_bf16(int M, int N, int K, const unsigned short *a, const unsigned short *b,
                   float *c);
 int ddx_get_stats(const ddx_handle *h, ddx_stats *st);
+
+/* Graph replay of the solver (no counterpart in the reference): with it on,
+ * after the first group of iterations of a solve every further group (two
+ * rounds of sweep, Jacobi, DIIS and extrapolation kernels) is one launch of
+ * a HIP graph captured on first use, which saves the host-side launch cost
+ * of small systems.  Results are identical to the eager launches.  Kernel
+ * timers, while on, force eager launches. */
+int ddx_set_graphs(ddx_handle *h, int enable);
 
 /* Kernel timing (instrumentation; no counterpart in the reference).  While
  * enabled for a kernel class, every launch of that class is bracketed by HIP

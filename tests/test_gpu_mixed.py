@@ -1,0 +1,127 @@
+"""Mixed precision (ddx_set_precision(DDX_PREC_MIXED)): FP32 sweeps and
+harmonic products inside FP64 iterative refinement.
+
+The reference (and the FP64 path, which follows its iterations exactly)
+stops once the Jacobi/DIIS increment is below 1e-8, so its results sit at a
+distance from the exact solution of the linear system: up to 1.1e-7 in the
+H-norm of sigma and S and 3e-8 Eh/bohr in the forces of its test inputs.
+The refinement stops by a test of the same kind on its corrections.  So
+the mixed results are checked against the oracle converged to 1e-13 (the
+exact solution, far below these tolerances), and must be as close to it as
+the reference's own outputs are, plus 1e-9:
+
+  FP32 operators L x, L* x ......... within 2e-6 of the reference's, relative
+                                     to the largest entry, and not equal to
+                                     the FP64 operators (the FP32 kernels ran)
+  energy ........................... |E - E_exact| <= |E_ref - E_exact|
+                                     + 1e-9 (eps-1)/(2 eps) sum |sigma psi|
+  sigma, S (H-norm), forces ........ distance from exact <= the reference's
+                                     + 1e-9
+"""
+import numpy as np
+import pytest
+
+from conftest import deck, golden, hsnorm_rms
+
+pytestmark = pytest.mark.gpu
+
+MOLS = ["formaldehyde", "benzene", "caffeine", "taxol", "ala25", "crambin"]
+# lmax 10 / 1202 points, lmax 12 / 590, lmax 16 / 1202: the FP32 m-blocks
+HIGH = ["c3_small", "l12", "l16"]
+MIXED = 1
+
+
+def make(name):
+    from ddpcm_amd.ddx import DDCosmo
+    d = deck(name)
+    h = DDCosmo.from_input(d)
+    h.set_precision(MIXED)
+    return d, h
+
+
+def relmax(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("name", ["caffeine", "c2_small"] + HIGH)
+def test_fp32_operators(name):
+    d, h = make(name)
+    g = golden(name)
+    v = g["probe_v"]
+    lx, lsx = h.lx(v), h.lstarx(v)
+    assert relmax(lx, g["probe_lx"]) < 2e-6
+    assert relmax(lsx, g["probe_lstarx"]) < 2e-6
+    assert relmax(lx, g["probe_lx"]) > 1e-12 and relmax(lsx, g["probe_lstarx"]) > 1e-12
+    h.set_precision(0)
+    assert relmax(h.lx(v), g["probe_lx"]) < 1e-12
+
+
+_EXACT = {}
+
+
+def exact(oracle_mod, name):
+    """The oracle's energy, forces, sigma and S converged to 1e-13."""
+    if name not in _EXACT:
+        d = deck(name)
+        o = oracle_mod.Oracle.from_input(d, iconv=13)
+        e, f, _, _ = o.energy_forces(d.charge)
+        phi, psi = o.mkrhs(d.charge)
+        sigma, _, _, rc1 = o.cosmo(False, phi=phi, psi=psi)
+        s, _, _, rc2 = o.cosmo(True, psi=psi)
+        assert rc1 == 0 and rc2 == 0
+        scale = 0.5 * (d.eps - 1.0) / d.eps * float(np.sum(np.abs(sigma * psi)))
+        _EXACT[name] = (e, f, sigma, s, scale)
+    return _EXACT[name]
+
+
+@pytest.mark.parametrize("name", MOLS + ["c2_small", "ion"] + HIGH)
+def test_mixed_energy_forces(name, oracle_mod):
+    d, h = make(name)
+    g = golden(name)
+    ex, fxx, sx, ssx, scale = exact(oracle_mod, name)
+    es, fx = h.energy_forces(d.charge)
+    st = h.stats()
+    err = dict(e=abs(es - ex) / scale, sig=hsnorm_rms(h.get("sigma") - sx, h.lmax),
+               s=hsnorm_rms(h.get("s") - ssx, h.lmax), f=float(np.max(np.abs(fx - fxx))),
+               it=(st.niter_direct, st.niter_adjoint))
+    print(name, err)
+    ref = dict(e=abs(g["esolv"] - ex) / scale, sig=hsnorm_rms(g["sigma"] - sx, h.lmax),
+               s=hsnorm_rms(g["s"] - ssx, h.lmax), f=float(np.max(np.abs(g["fx"] - fxx))))
+    for k in ("e", "sig", "s", "f"):
+        assert err[k] <= ref[k] + 1e-9, (k, err, ref)
+    assert 0 < st.niter_direct < 200 and 0 < st.niter_adjoint < 200
+    assert np.max(np.abs(fx.sum(axis=0))) < 1e-8 * max(1.0, np.max(np.abs(fx)))
+
+
+def test_mixed_single_solves(oracle_mod):
+    d, h = make("taxol")
+    g = golden("taxol")
+    ex, _, sx, ssx, scale = exact(oracle_mod, "taxol")
+    phi, psi = h.mkrhs(d.charge)
+    sigma, es = h.cosmo(False, True, phi=phi, psi=psi)
+    assert abs(es - ex) <= abs(g["esolv"] - ex) + 1e-9 * scale
+    assert hsnorm_rms(sigma - sx, h.lmax) <= hsnorm_rms(g["sigma"] - sx, h.lmax) + 1e-9
+    s, _ = h.cosmo(True, psi=psi)
+    assert hsnorm_rms(s - ssx, h.lmax) <= hsnorm_rms(g["s"] - ssx, h.lmax) + 1e-9
+
+
+def test_mixed_batch_against_oracle(oracle_mod):
+    from ddpcm_amd import inputs
+    from ddpcm_amd.ddx import DDCosmoBatch
+    decks = inputs.batch_inputs(count=10, lo=6, hi=90, seed=4400)
+    b = DDCosmoBatch.from_inputs(decks)
+    b.set_precision(MIXED)
+    es, fx = b.energy_forces([d.charge for d in decks])
+    nd, na = b.iterations()
+    assert np.all(nd > 0) and np.all(na > 0) and np.all(nd < 200) and np.all(na < 200)
+    for q, (d, f) in enumerate(zip(decks, b.split(fx))):
+        o = oracle_mod.Oracle.from_input(d, iconv=13)
+        eo, fo, _, _ = o.energy_forces(d.charge)
+        phi, psi = o.mkrhs(d.charge)
+        so, _, _, _ = o.cosmo(False, phi=phi, psi=psi)
+        scale = 0.5 * (d.eps - 1.0) / d.eps * float(np.sum(np.abs(so * psi)))
+        # the oracle at the molecule's own threshold: the reference's result
+        e8, f8, _, _ = oracle_mod.Oracle.from_input(d).energy_forces(d.charge)
+        assert abs(es[q] - eo) <= abs(e8 - eo) + 1e-9 * scale, q
+        assert np.max(np.abs(f - fo)) <= np.max(np.abs(f8 - fo)) + 1e-9, q
