@@ -312,6 +312,9 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
                     help="precision of the timed solves; an fp64 run also times the mixed "
                          "precision beside it (\"mixed_precision\" in the JSON line)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick sweep workers per CU and the projection split by timing the "
+                         "workload first (ddx_autotune)")
     ap.add_argument("--cpu-cores", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -357,6 +360,10 @@ def main():
         tag = {"c2": "2000atoms", "c3": "20000atoms"}.get(args.workload, args.workload)
     if args.precision == "mixed":
         h.set_precision(1)
+    tuned = None
+    if args.autotune:
+        w, sp = h.autotune()
+        tuned = {"sweep_workers_per_cu": w, "projection_split": sp}
     for _ in range(args.warmup):
         h.run()
     # which kernel class dominates one solve
@@ -444,6 +451,7 @@ def main():
         "kernels": table,
         "hbm_gbs": {k: table[k].get("gbs") for k in ("jacobi", "extrap")},
         "mixed_precision": mixed,
+        "autotune": tuned,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = (cpu_baseline_batch(decks, args.cpu_cores) if batch

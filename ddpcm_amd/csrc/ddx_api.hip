@@ -1,4 +1,5 @@
 // C-ABI of libddx: the reference module's procedures on device-resident state.
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -721,6 +722,52 @@ int ddx_set_precision(ddx_handle *h, int prec) {
   if (!h || (prec != DDX_PREC_FP64 && prec != DDX_PREC_MIXED)) return DDX_ERR_ARG;
   DDX_TRY(sync(h));
   h->prec = prec;
+  return DDX_OK;
+}
+
+// The launch shapes of the solver, picked by timing the whole pipeline on
+// the uploaded system: sweep workers per compute unit (4, 8, 16), then the
+// point split of the L X projection (1, 2, 4).  Each candidate runs twice,
+// the second run timed; the last run uses the choice.
+int ddx_autotune(ddx_handle *h, int *workers_per_cu, int *proj_split) {
+  if (!h) return DDX_ERR_ARG;
+  if (!h->have_charge) {
+    set_error("ddx_autotune needs ddx_upload first");
+    return DDX_ERR_STATE;
+  }
+  auto run_ms = [h](double *ms) -> int {
+    DDX_TRY(timed_setup(h));
+    DDX_TRY(pipeline(h));
+    DDX_TRY(sync(h));
+    const auto t0 = std::chrono::steady_clock::now();
+    DDX_TRY(timed_setup(h));
+    DDX_TRY(pipeline(h));
+    DDX_TRY(sync(h));
+    *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return DDX_OK;
+  };
+  static const int kWorkers[] = {4, 8, 16};
+  static const int kSplits[] = {1, 2, 4};
+  double best = 1e300, t = 0.0;
+  int bw = h->workers_per_cu, bs = 1;
+  for (int w : kWorkers) {
+    h->workers_per_cu = w;
+    DDX_TRY(run_ms(&t));
+    if (t < best) best = t, bw = w;
+  }
+  h->workers_per_cu = bw;
+  best = 1e300;
+  for (int sp : kSplits) {
+    h->ysplit = sp;   // the block maps exist after the runs above
+    DDX_TRY(run_ms(&t));
+    if (t < best) best = t, bs = sp;
+  }
+  h->ysplit = bs;
+  DDX_TRY(timed_setup(h));
+  DDX_TRY(pipeline(h));
+  DDX_TRY(sync(h));
+  if (workers_per_cu) *workers_per_cu = bw;
+  if (proj_split) *proj_split = bs;
   return DDX_OK;
 }
 

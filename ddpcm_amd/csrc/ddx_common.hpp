@@ -222,6 +222,7 @@ struct ddx_handle {
   std::vector<int> h_fgrp, h_xgrp;
   int maps_n = -1;                // sphere count the block maps were built for
   bool mute_timers = false;       // launches outside the timed classes' meaning
+  int workers_per_cu = ddx::kSweepWorkersPerCU;   // sweep workers (ddx_autotune)
   // groups of solver rounds captured as HIP graphs (ddx_set_graphs)
   bool use_graphs = false;
   struct RoundGraph {

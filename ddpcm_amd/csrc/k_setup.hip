@@ -423,8 +423,9 @@ int run_setup(ddx_handle *h) {
   h->ntri = ntri;
   h->ncav = ncav;
   h->nunit = nunit;
-  // workers: enough one-wave workgroups to fill every SIMD twice
-  long long G = (long long)kSweepWorkersPerCU * h->ncu;
+  // workers: enough one-wave workgroups to fill every SIMD twice (or as
+  // many per compute unit as ddx_autotune chose)
+  long long G = (long long)h->workers_per_cu * h->ncu;
   if (G > nunit) G = nunit;
   h->nworker = (int)G;
   DDX_TRY(h->segcnt.reserve(n + 1));

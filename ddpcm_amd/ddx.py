@@ -29,7 +29,7 @@ EXPORTS = (
     "ddx_set_precision", "ddx_get_stats", "ddx_kernel_timer", "ddx_kernel_timer_read",
     "ddx_batch_init", "ddx_batch_energy_forces", "ddx_batch_result", "ddx_batch_iterations",
     "ddx_batch_sizes", "ddx_gemm_bf16", "ddx_set_guess", "ddx_guess_train", "ddx_guess_reset",
-    "ddx_mlp_grad", "ddx_set_graphs",
+    "ddx_mlp_grad", "ddx_set_graphs", "ddx_autotune",
 )
 DDX_GUESS_REFERENCE, DDX_GUESS_PREVIOUS, DDX_GUESS_LEARNED, DDX_GUESS_LEARNED_PREVIOUS = 0, 1, 2, 3
 KERNELS = ("sweep_l", "sweep_lstar", "project", "expand", "jacobi", "diis", "extrap", "field",
@@ -103,6 +103,7 @@ def load_library(path: str = LIB_PATH):
     L.ddx_gemm_bf16.argtypes = [C.c_int, C.c_int, C.c_int, _u16, _u16, C.POINTER(C.c_float)]
     L.ddx_set_guess.argtypes = [C.c_void_p, C.c_int]
     L.ddx_set_graphs.argtypes = [C.c_void_p, C.c_int]
+    L.ddx_autotune.argtypes = [C.c_void_p, _ip, _ip]
     L.ddx_guess_train.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp]
     L.ddx_guess_reset.argtypes = [C.c_void_p, C.c_uint]
     _fp = C.POINTER(C.c_float)
@@ -327,6 +328,13 @@ class DDCosmo:
         loss = np.zeros(2)
         self._check(self.L.ddx_guess_train(self.h, int(steps), float(lr), _p(loss)))
         return float(loss[0]), float(loss[1])
+
+    def autotune(self):
+        """Pick sweep workers per CU and the projection split by timing the
+        uploaded system's pipeline; returns (workers_per_cu, proj_split)."""
+        w, sp = C.c_int(), C.c_int()
+        self._check(self.L.ddx_autotune(self.h, C.byref(w), C.byref(sp)))
+        return w.value, sp.value
 
     def set_graphs(self, enable=True):
         """Replay the solver's iteration groups as captured HIP graphs."""
