@@ -402,6 +402,8 @@ def main():
                          "avg_launch_us": avg_s * 1e6, "work_per_launch": per_launch,
                          "work_unit": unit})
 
+    if batch:
+        nd, na = h.iterations()
     mixed = None
     if args.precision == "fp64":
         mixed = mixed_beside(h, args, sec, es, fx, batch)
@@ -416,7 +418,6 @@ def main():
                                           st.niter_adjoint == int(g["niter_adjoint"]))}
 
     if batch:
-        nd, na = h.iterations()
         config = {"workload": "c4: %d synthetic molecules of %d..%d atoms (%d in all), lmax %d, "
                               "%d Lebedev points, eps %.2f, eta %.2f, conv 1e-%d, energy + forces, "
                               "one batched handle" % (h.count, int(np.diff(h.sph_off).min()),
