@@ -94,6 +94,22 @@ def test_mixed_energy_forces(name, oracle_mod):
     assert np.max(np.abs(fx.sum(axis=0))) < 1e-8 * max(1.0, np.max(np.abs(fx)))
 
 
+def test_mixed_energy_only(oracle_mod):
+    """The 12-atom energy-only input (igrad 0): the direct solve alone."""
+    d, h = make("c1")
+    g = golden("c1")
+    assert d.igrad == 0
+    o = oracle_mod.Oracle.from_input(d, iconv=13)
+    phi, psi = o.mkrhs(d.charge)
+    sx, ex, _, rc = o.cosmo(False, phi=phi, psi=psi)
+    assert rc == 0
+    scale = 0.5 * (d.eps - 1.0) / d.eps * float(np.sum(np.abs(sx * psi)))
+    es, _ = h.energy_forces(d.charge)
+    assert abs(es - ex) <= abs(g["esolv"] - ex) + 1e-9 * scale
+    assert hsnorm_rms(h.get("sigma") - sx, h.lmax) <= hsnorm_rms(g["sigma"] - sx, h.lmax) + 1e-9
+    assert 0 < h.stats().niter_direct < 200
+
+
 def test_mixed_single_solves(oracle_mod):
     d, h = make("taxol")
     g = golden("taxol")
@@ -104,6 +120,10 @@ def test_mixed_single_solves(oracle_mod):
     assert hsnorm_rms(sigma - sx, h.lmax) <= hsnorm_rms(g["sigma"] - sx, h.lmax) + 1e-9
     s, _ = h.cosmo(True, psi=psi)
     assert hsnorm_rms(s - ssx, h.lmax) <= hsnorm_rms(g["s"] - ssx, h.lmax) + 1e-9
+    # the QM path: the right-hand side given as multipoles (cosmo.f90:125-128)
+    sigma2, es2 = h.cosmo(False, False, glm=g["rhs"], psi=psi)
+    assert abs(es2 - ex) <= abs(g["esolv"] - ex) + 1e-9 * scale
+    assert hsnorm_rms(sigma2 - sx, h.lmax) <= hsnorm_rms(g["sigma"] - sx, h.lmax) + 1e-9
 
 
 def test_mixed_batch_against_oracle(oracle_mod):
