@@ -52,6 +52,7 @@ contains
     real*8, dimension(n), intent(in) :: x, y, z, rvdw
     integer(c_int) :: rc, ns, nc, ny, ng, lm, nnl
     integer :: istatus
+    character(len=16) :: prec_env
     if (nproc == 0) nproc = 1
     pi = four*atan(one)
     sq2 = sqrt(two)
@@ -59,6 +60,11 @@ contains
     rc = ddx_init(handle, int(n, c_int), x, y, z, rvdw, int(iprint, c_int), int(lmax, c_int), &
                   int(ngrid, c_int), int(iconv, c_int), int(igrad, c_int), eps, eta)
     call ddx_check(rc, 'ddinit')
+    ! DDX_PRECISION=mixed in the environment selects the mixed precision
+    ! (FP32 sweeps inside FP64 refinement); the default is the reference's FP64
+    call get_environment_variable('DDX_PRECISION', prec_env)
+    if (trim(prec_env) == 'mixed') &
+      call ddx_check(ddx_set_precision(handle, DDX_PREC_MIXED), 'ddinit')
     rc = ddx_get_sizes(handle, ns, nc, ny, ng, lm)
     nsph = ns ; ncav = nc ; nylm = ny ; ngrid = ng
     grad = igrad /= 0

@@ -4,6 +4,7 @@ module ddx_cabi
   implicit none
 
   integer(c_int), parameter :: DDX_OK = 0, DDX_ERR_NOCONV = 3
+  integer(c_int), parameter :: DDX_PREC_FP64 = 0, DDX_PREC_MIXED = 1
 
   interface
     function ddx_last_error() bind(C, name='ddx_last_error')
@@ -23,6 +24,12 @@ module ddx_cabi
     integer(c_int) function ddx_free(h) bind(C, name='ddx_free')
       import :: c_ptr, c_int
       type(c_ptr), value :: h
+    end function
+
+    integer(c_int) function ddx_set_precision(h, prec) bind(C, name='ddx_set_precision')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+      integer(c_int), value :: prec
     end function
 
     integer(c_int) function ddx_get_sizes(h, nsph, ncav, nylm, ngrid, lmax) &

@@ -142,3 +142,25 @@ def test_driver_log_equals_golden_log_line_by_line(name, tmp_path):
         rel = _NORM_REL if b[:1] == ["iter="] else 0.0
         assert len(a) == len(b) and all(_same_token(x, y, rel) for x, y in zip(a, b)), \
             "line %d: %s != %s" % (k, " ".join(map(str, a)), " ".join(map(str, b)))
+
+
+@pytest.mark.parametrize("name", ["caffeine", "crambin"])
+def test_driver_mixed_precision(name, tmp_path):
+    """DDX_PRECISION=mixed in the environment: the Fortran host runs the
+    mixed precision; energy and forces stay within 1e-6 relative and 1e-7
+    Eh/bohr of the reference's log."""
+    if not os.path.exists(DRIVER):
+        pytest.fail("ddx_main is not built; run __graft_entry__.build()")
+    deck = os.path.join(GOLDEN, "inputs", name + ".txt")
+    work = tmp_path / name
+    work.mkdir()
+    env = dict(os.environ, DDX_PRECISION="mixed")
+    r = subprocess.run([DRIVER, deck], cwd=str(work), capture_output=True, text=True,
+                       timeout=120, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    e, f, it = parse(r.stdout)
+    e0, f0, it0 = parse(open(os.path.join(GOLDEN, "logs", name + ".log")).read())
+    assert abs(e - e0) <= 1e-6 * abs(e0) + 1.5e-6
+    np.testing.assert_allclose(f, f0, rtol=0, atol=1e-7)
+    # the refinement's iteration counts, not the reference's: it ran
+    assert it != it0
