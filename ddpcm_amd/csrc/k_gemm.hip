@@ -28,6 +28,7 @@ constexpr int kMaxTiles = 8;   // row tiles of 16 per projection pass
 
 constexpr int kUnroll = 4;   // k-steps whose operands are loaded before their MFMAs
 constexpr int kExpandUnroll = 8;
+constexpr size_t kExpandWide = 2048;   // blocks from which a wave takes 4 sphere tiles
 constexpr int kUnroll32 = 2;   // k-steps in flight in the FP32 projection
 
 __global__ __launch_bounds__(256) void k_project(int nylm, int ngrid, int nsph, double alpha,
@@ -102,8 +103,12 @@ __global__ __launch_bounds__(256) void k_project(int nylm, int ngrid, int nsph, 
   }
 }
 
-// xi(n, i) = scale_n * sum_lm Y(lm, n) S(lm, i); one wave per 16 x 16 tile,
-// blockIdx.x over point tiles of 64, blockIdx.y over sphere tiles of 16.
+// xi(n, i) = scale_n * sum_lm Y(lm, n) S(lm, i); one wave per 16 points x
+// J tiles of 16 spheres (J independent accumulator chains that share the Y
+// operand), blockIdx.x over point tiles of 64, blockIdx.y over sphere tiles
+// of 16 J.  J = 4 pays where the grid has blocks to spare (large systems,
+// batches), J = 1 keeps small systems' grids full.
+template <int J>
 __global__ __launch_bounds__(256) void k_expand(int nylm, int ngrid, int nsph,
                                                 const double *__restrict__ Y,
                                                 const double *__restrict__ S,
@@ -113,34 +118,45 @@ __global__ __launch_bounds__(256) void k_expand(int nylm, int ngrid, int nsph,
   if (gate && gate->done) return;
   const int lane = lane_id(), wave = threadIdx.x / kWave;
   const int n0 = blockIdx.x * 64 + 16 * wave;
-  const int i0 = blockIdx.y * 16;
+  const int i0 = blockIdx.y * 16 * J;
   if (n0 >= ngrid) return;
   const int row = lane & 15, kq = lane >> 4;
   const int na = n0 + row;          // A row: grid point
-  const int sb = i0 + row;          // B column: sphere
-  v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
+  v4d acc[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) acc[j] = v4d{0.0, 0.0, 0.0, 0.0};
   // the operands of kExpandUnroll k-steps are loaded before their MFMAs
   for (int k0 = 0; k0 < nylm; k0 += 4 * kExpandUnroll) {
-    double a[kExpandUnroll], b[kExpandUnroll];
+    double a[kExpandUnroll], b[kExpandUnroll][J];
 #pragma unroll
     for (int u = 0; u < kExpandUnroll; ++u) {
       const int k = k0 + 4 * u + kq;
       a[u] = (na < ngrid && k < nylm) ? Y[k + (size_t)nylm * na] : 0.0;
-      b[u] = (sb < nsph && k < nylm) ? S[k + (size_t)nylm * sb] : 0.0;
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int sb = i0 + 16 * j + row;   // B column: sphere
+        b[u][j] = (sb < nsph && k < nylm) ? S[k + (size_t)nylm * sb] : 0.0;
+      }
     }
 #pragma unroll
     for (int u = 0; u < kExpandUnroll; ++u)
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
-  }
-  const int sph = i0 + row;
-  if (sph >= nsph) return;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int n = n0 + kq + 4 * q;
-    if (n < ngrid) {
-      double v = acc[q];
-      if (wscale) v *= grid[n].w;
-      xi[n + (size_t)ngrid * sph] = v;
+      for (int j = 0; j < J; ++j)
+        acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u][j], acc[j], 0, 0, 0);
+  }
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int sph = i0 + 16 * j + row;
+    if (sph < nsph) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int n = n0 + kq + 4 * q;
+        if (n < ngrid) {
+          double v = acc[j][q];
+          if (wscale) v *= grid[n].w;
+          xi[n + (size_t)ngrid * sph] = v;
+        }
+      }
     }
   }
 }
@@ -317,9 +333,11 @@ int project(ddx_handle *h, hipStream_t s, double alpha, const double *P, double 
 int expand(ddx_handle *h, hipStream_t s, const double *S, double *xi, bool wscale,
            const DiisState *gate) {
   if (h->n <= 0) return DDX_OK;
-  dim3 g((h->ngrid + 63) / 64, (h->n + 15) / 16);
+  const int pt = (h->ngrid + 63) / 64;
+  const bool wide = (size_t)pt * ((h->n + 63) / 64) >= kExpandWide;
+  dim3 g(pt, wide ? (h->n + 63) / 64 : (h->n + 15) / 16);
   TimedLaunch tl(h, DDX_K_EXPAND, s);
-  hipExtLaunchKernelGGL(k_expand, g, dim3(256), 0, s, tl.b, tl.e, 0, h->nylm, h->ngrid, h->n,
+  hipExtLaunchKernelGGL(wide ? k_expand<4> : k_expand<1>, g, dim3(256), 0, s, tl.b, tl.e, 0, h->nylm, h->ngrid, h->n,
                         h->Y.p, S, h->grid.p, wscale ? 1 : 0, xi, gate);
   DDX_HIP(hipGetLastError());
   return DDX_OK;

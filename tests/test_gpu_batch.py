@@ -70,3 +70,28 @@ def test_batch_resident_path_and_single_calls_refused():
     assert np.array_equal(es, es2) and np.array_equal(fx, fx2)
     with pytest.raises(DDXError):
         DDCosmoBatch.__mro__[1].energy_forces(b, np.concatenate([dk.charge for dk in decks]))
+
+
+def test_large_batch_matches_single_handles():
+    """A batch large enough (about 30 000 spheres) for the expansion's wide
+    launch shape (four sphere tiles per wave): each molecule as on its own
+    handle, whose small grid keeps one tile per wave.  At this size the
+    all-pairs source chunks and the extrapolation blocks are wider than a
+    single molecule's, so partial sums group differently; late DIIS steps
+    amplify that to about 1e-10 of the energy (1e-9 and 1e-8 Eh/bohr here,
+    iteration counts within one)."""
+    from ddpcm_amd import inputs
+    from ddpcm_amd.ddx import DDCosmo, DDCosmoBatch
+    decks = inputs.batch_inputs(count=200, lo=100, hi=200, seed=4600)
+    b = DDCosmoBatch.from_inputs(decks)
+    assert ((b.ngrid + 63) // 64) * ((b.nsph + 63) // 64) >= 2048
+    es, fx = b.energy_forces([d.charge for d in decks])
+    nd, na = b.iterations()
+    for q in range(0, 200, 7):
+        d = decks[q]
+        h = DDCosmo.from_input(d)
+        e1, f1 = h.energy_forces(d.charge)
+        st = h.stats()
+        assert abs(es[q] - e1) <= 1e-9 * abs(e1), q
+        np.testing.assert_allclose(b.split(fx)[q], f1, rtol=0, atol=1e-8, err_msg=str(q))
+        assert abs(nd[q] - st.niter_direct) <= 1 and abs(na[q] - st.niter_adjoint) <= 1, q
