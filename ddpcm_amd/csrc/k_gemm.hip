@@ -237,7 +237,9 @@ __global__ __launch_bounds__(256) void k_project32(int nylm, int ngrid, int nsph
   }
 }
 
-// xi(n, i) = scale_n * sum_lm Y(lm, n) S(lm, i) in FP32 (S rounded on load)
+// xi(n, i) = scale_n * sum_lm Y(lm, n) S(lm, i) in FP32 (S rounded on load),
+// with the launch shapes of k_expand (J sphere tiles per wave)
+template <int J>
 __global__ __launch_bounds__(256) void k_expand32(int nylm, int ngrid, int nsph,
                                                   const float *__restrict__ Y,
                                                   const double *__restrict__ S,
@@ -247,33 +249,44 @@ __global__ __launch_bounds__(256) void k_expand32(int nylm, int ngrid, int nsph,
   if (gate && gate->done) return;
   const int lane = lane_id(), wave = threadIdx.x / kWave;
   const int n0 = blockIdx.x * 64 + 16 * wave;
-  const int i0 = blockIdx.y * 16;
+  const int i0 = blockIdx.y * 16 * J;
   if (n0 >= ngrid) return;
   const int row = lane & 15, kq = lane >> 4;
   const int na = n0 + row;
-  const int sb = i0 + row;
-  v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
+  v4f acc[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < nylm; k0 += 4 * kExpandUnroll) {
-    float a[kExpandUnroll], b[kExpandUnroll];
+    float a[kExpandUnroll], b[kExpandUnroll][J];
 #pragma unroll
     for (int u = 0; u < kExpandUnroll; ++u) {
       const int k = k0 + 4 * u + kq;
       a[u] = (na < ngrid && k < nylm) ? Y[k + (size_t)nylm * na] : 0.f;
-      b[u] = (sb < nsph && k < nylm) ? (float)S[k + (size_t)nylm * sb] : 0.f;
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int sb = i0 + 16 * j + row;
+        b[u][j] = (sb < nsph && k < nylm) ? (float)S[k + (size_t)nylm * sb] : 0.f;
+      }
     }
 #pragma unroll
     for (int u = 0; u < kExpandUnroll; ++u)
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u], acc, 0, 0, 0);
-  }
-  const int sph = i0 + row;
-  if (sph >= nsph) return;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int n = n0 + 4 * kq + q;
-    if (n < ngrid) {
-      float v = acc[q];
-      if (wscale) v *= (float)grid[n].w;
-      xi[n + (size_t)ngrid * sph] = v;
+      for (int j = 0; j < J; ++j)
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u][j], acc[j], 0, 0, 0);
+  }
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int sph = i0 + 16 * j + row;
+    if (sph < nsph) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int n = n0 + 4 * kq + q;
+        if (n < ngrid) {
+          float v = acc[j][q];
+          if (wscale) v *= (float)grid[n].w;
+          xi[n + (size_t)ngrid * sph] = v;
+        }
+      }
     }
   }
 }
@@ -312,9 +325,11 @@ int project32(ddx_handle *h, hipStream_t s, double alpha, const float *P, double
 int expand32(ddx_handle *h, hipStream_t s, const double *S, float *xi, bool wscale,
              const DiisState *gate) {
   if (h->n <= 0) return DDX_OK;
-  dim3 g((h->ngrid + 63) / 64, (h->n + 15) / 16);
+  const int pt = (h->ngrid + 63) / 64;
+  const bool wide = (size_t)pt * ((h->n + 63) / 64) >= kExpandWide;
+  dim3 g(pt, wide ? (h->n + 63) / 64 : (h->n + 15) / 16);
   TimedLaunch tl(h, DDX_K_EXPAND, s);
-  hipExtLaunchKernelGGL(k_expand32, g, dim3(256), 0, s, tl.b, tl.e, 0, h->nylm, h->ngrid, h->n,
+  hipExtLaunchKernelGGL(wide ? k_expand32<4> : k_expand32<1>, g, dim3(256), 0, s, tl.b, tl.e, 0, h->nylm, h->ngrid, h->n,
                         h->Y32.p, S, h->grid.p, wscale ? 1 : 0, xi, gate);
   DDX_HIP(hipGetLastError());
   return DDX_OK;

@@ -95,3 +95,8 @@ def test_large_batch_matches_single_handles():
         assert abs(es[q] - e1) <= 1e-9 * abs(e1), q
         np.testing.assert_allclose(b.split(fx)[q], f1, rtol=0, atol=1e-8, err_msg=str(q))
         assert abs(nd[q] - st.niter_direct) <= 1 and abs(na[q] - st.niter_adjoint) <= 1, q
+    # the FP32 expansion of the mixed precision in the same wide shape
+    b.set_precision(1)
+    em, fm = b.energy_forces([d.charge for d in decks])
+    assert np.max(np.abs(em - es) / np.abs(es)) < 1e-7
+    assert np.max(np.abs(fm - fx)) < 1e-7
