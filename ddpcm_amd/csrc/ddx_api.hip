@@ -280,6 +280,9 @@ int create(ddx_handle **out, int n, int count, const int *nsph, const double *x,
   ddx_handle *h = new ddx_handle();
   h->n = n;
   h->ninst = count;
+  // batches: twice the sweep workers (what ddx_autotune picks for them; a
+  // batch's many short containers balance better over more workers)
+  if (count > 1) h->workers_per_cu = 2 * kSweepWorkersPerCU;
   h->h_soff.assign(1, 0);
   for (int q = 0; q < count; ++q) h->h_soff.push_back(h->h_soff.back() + (nsph ? nsph[q] : n));
   h->lmax = lmax;
