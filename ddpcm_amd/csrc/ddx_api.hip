@@ -280,9 +280,6 @@ int create(ddx_handle **out, int n, int count, const int *nsph, const double *x,
   ddx_handle *h = new ddx_handle();
   h->n = n;
   h->ninst = count;
-  // batches: twice the sweep workers (what ddx_autotune picks for them; a
-  // batch's many short containers balance better over more workers)
-  if (count > 1) h->workers_per_cu = 2 * kSweepWorkersPerCU;
   h->h_soff.assign(1, 0);
   for (int q = 0; q < count; ++q) h->h_soff.push_back(h->h_soff.back() + (nsph ? nsph[q] : n));
   h->lmax = lmax;
@@ -729,7 +726,7 @@ int ddx_set_precision(ddx_handle *h, int prec) {
 }
 
 // The launch shapes of the solver, picked by timing the whole pipeline on
-// the uploaded system: sweep workers per compute unit (4, 8, 16), then the
+// the uploaded system: sweep workers per compute unit (4 .. 64), then the
 // point split of the L X projection (1, 2, 4).  Each candidate runs twice,
 // the second run timed; the last run uses the choice.
 int ddx_autotune(ddx_handle *h, int *workers_per_cu, int *proj_split) {
@@ -749,10 +746,11 @@ int ddx_autotune(ddx_handle *h, int *workers_per_cu, int *proj_split) {
     *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return DDX_OK;
   };
-  static const int kWorkers[] = {4, 8, 16};
+  static const int kWorkers[] = {4, 8, 16, 32, 64};
   static const int kSplits[] = {1, 2, 4};
   double best = 1e300, t = 0.0;
   int bw = h->workers_per_cu, bs = 1;
+  h->workers_fixed = true;
   for (int w : kWorkers) {
     h->workers_per_cu = w;
     DDX_TRY(run_ms(&t));

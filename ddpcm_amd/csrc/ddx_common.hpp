@@ -17,7 +17,8 @@ constexpr int kMaxLmax = 16;       // template instantiations of the sweeps
 constexpr int kMaxDegree = 64;     // neighbours per sphere (64-bit point masks)
 constexpr int kDiisMax = 25;       // ndiis of the reference (ddcosmo.f90:87)
 constexpr int kMaxIter = 200;      // n_iter of cosmo (cosmo.f90:81)
-constexpr int kSweepWorkersPerCU = 8;  // one-wave sweep workers per compute unit
+constexpr int kSweepWorkersPerCU = 8;  // one-wave sweep workers per compute unit (at least)
+constexpr int kUnitsPerWorker = 12;    // units of 64 triples per sweep worker (default)
 
 void set_error(const char *fmt, ...);
 
@@ -222,7 +223,10 @@ struct ddx_handle {
   std::vector<int> h_fgrp, h_xgrp;
   int maps_n = -1;                // sphere count the block maps were built for
   bool mute_timers = false;       // launches outside the timed classes' meaning
-  int workers_per_cu = ddx::kSweepWorkersPerCU;   // sweep workers (ddx_autotune)
+  // sweep workers: by default enough for about kUnitsPerWorker units each (at
+  // least kSweepWorkersPerCU per CU); ddx_autotune fixes workers_per_cu
+  int workers_per_cu = ddx::kSweepWorkersPerCU;
+  bool workers_fixed = false;
   // groups of solver rounds captured as HIP graphs (ddx_set_graphs)
   bool use_graphs = false;
   struct RoundGraph {

@@ -19,6 +19,9 @@
 #include "ddx_device.hpp"
 #include "ddx_kernels.hpp"
 
+#include <algorithm>
+#include <cmath>
+
 namespace ddx {
 namespace {
 
@@ -423,9 +426,17 @@ int run_setup(ddx_handle *h) {
   h->ntri = ntri;
   h->ncav = ncav;
   h->nunit = nunit;
-  // workers: enough one-wave workgroups to fill every SIMD twice (or as
-  // many per compute unit as ddx_autotune chose)
+  // workers: whole multiples of the R = 8 per CU one-wave workgroups that
+  // are resident at once (two waves per SIMD), as many as give about
+  // kUnitsPerWorker units each (or as many per compute unit as ddx_autotune
+  // chose).  Short ranges keep waves queued behind the resident ones, which
+  // hides the memory latency of their triples; a partial last round of
+  // workgroups would leave most CUs idle at the end.
   long long G = (long long)h->workers_per_cu * h->ncu;
+  if (!h->workers_fixed) {
+    const double rounds = (double)nunit / ((double)kUnitsPerWorker * (double)G);
+    G *= std::max(1LL, std::llround(rounds));
+  }
   if (G > nunit) G = nunit;
   h->nworker = (int)G;
   DDX_TRY(h->segcnt.reserve(n + 1));

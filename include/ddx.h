@@ -215,8 +215,8 @@ int ddx_gemm_bf16(int M, int N, int K, const unsigned short *a, const unsigned s
 int ddx_get_stats(const ddx_handle *h, ddx_stats *st);
 
 /* Autotuner (no counterpart in the reference): times the whole pipeline of
- * the uploaded system (ddx_upload first) for 4, 8 and 16 sweep workers per
- * compute unit and then for 1, 2 and 4 point splits of the L X projection,
+ * the uploaded system (ddx_upload first) for 4, 8, 16, 32 and 64 sweep
+ * workers per compute unit and then for 1, 2 and 4 point splits of the L X projection,
  * keeps the fastest of each on the handle, and leaves the results of a run
  * with them.  The worker count changes how the L* S sums are split into
  * partial sums, so results may differ from the untuned handle's in the last
