@@ -237,7 +237,9 @@ void coulomb_tiles(const std::vector<int> &toff, const std::vector<int> &soff, b
     work += (long long)((toff[q + 1] - toff[q] + kTW - 1) / kTW) * (soff[q + 1] - soff[q]);
   long long chunk = 0x7fffffff;
   if (split) {
-    chunk = (work + 1023) / 1024;
+    // about 8192 tiles: several rounds of the resident workgroups, so that
+    // the uneven last chunks of each target tile do not leave CUs idle
+    chunk = (work + 8191) / 8192;
     chunk = ((chunk + kCT - 1) / kCT) * kCT;
     if (chunk < 4 * kCT) chunk = 4 * kCT;
   }
