@@ -309,6 +309,9 @@ int create(ddx_handle **out, int n, int count, const int *nsph, const double *x,
       return fail(DDX_ERR_HIP);
     }
   }
+  // the scratch address of ddx_efld_noalloc on this device, looked up here
+  // so that a later call under graph capture finds it ready
+  if ((rc = noalloc_setup(nullptr)) != DDX_OK) return fail(rc);
   for (auto &e : h->ev)
     if (hipEventCreate(&e) != hipSuccess) {
       set_error("hipEventCreate failed");

@@ -98,6 +98,9 @@ int coulomb_field(ddx_handle *h, const int4 *tiles, int ntile, const int *grp, i
 int coulomb_field_points(ddx_handle *h, int nsrc, const double *q, const double *src3, int ntrg,
                          const double *trg3, double *ef);
 int make_psi(ddx_handle *h, const double *charge, double *psi);
+// device address of the scratch array of ddx_efld_noalloc on the current
+// device (looked up once per device; out may be null)
+int noalloc_setup(double **out);
 
 // k_forces.hip — forces_dd, ddmkzeta and the field contractions
 int forces_dd(ddx_handle *h, const double *d_phi, const double *d_sigma,

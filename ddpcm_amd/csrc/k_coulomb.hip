@@ -19,6 +19,7 @@
 #include "ddx_kernels.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 namespace ddx {
 namespace {
@@ -116,15 +117,13 @@ __global__ __launch_bounds__(kCT) void k_potential(const int4 *__restrict__ tile
   }
 }
 
-// partial fields of tile (tb, te, sb, se) at its targets (two per thread):
-// out[(b kTW + t) 3]
-__global__ __launch_bounds__(kCT) void k_field(const int4 *__restrict__ tiles,
-                                               const double *__restrict__ q,
-                                               const double *__restrict__ src,
-                                               const double *__restrict__ trg,
-                                               double *__restrict__ out) {
-  __shared__ double4 tile[kCT];
-  const int4 tl = tiles[blockIdx.x];
+// fields of the sources sb..se-1 of tile tl at its targets tb..te-1 (two per
+// thread, targets tb + t and tb + t + kCT); the sums of thread t come back in
+// e[0..2] (first target) and e[3..5] (second), zero for targets past te
+__device__ __forceinline__ void field_tile(const int4 tl, const double *__restrict__ q,
+                                           const double *__restrict__ src,
+                                           const double *__restrict__ trg, double4 *tile,
+                                           double e[6]) {
   const int j0 = tl.x + threadIdx.x, j1 = j0 + kCT;
   double ax = 0.0, ay = 0.0, az = 0.0, bx = 0.0, by = 0.0, bz = 0.0;
   if (j0 < tl.y) {
@@ -166,13 +165,33 @@ __global__ __launch_bounds__(kCT) void k_field(const int4 *__restrict__ tiles,
     }
     __syncthreads();
   }
+  if (j0 >= tl.y) ea0 = ea1 = ea2 = 0.0;
+  if (j1 >= tl.y) eb0 = eb1 = eb2 = 0.0;
+  e[0] = ea0;
+  e[1] = ea1;
+  e[2] = ea2;
+  e[3] = eb0;
+  e[4] = eb1;
+  e[5] = eb2;
+}
+
+// partial fields of tile (tb, te, sb, se) at its targets (two per thread):
+// out[(b kTW + t) 3]
+__global__ __launch_bounds__(kCT) void k_field(const int4 *__restrict__ tiles,
+                                               const double *__restrict__ q,
+                                               const double *__restrict__ src,
+                                               const double *__restrict__ trg,
+                                               double *__restrict__ out) {
+  __shared__ double4 tile[kCT];
+  double e[6];
+  field_tile(tiles[blockIdx.x], q, src, trg, tile, e);
   const size_t o = ((size_t)blockIdx.x * kTW + threadIdx.x) * 3, o1 = o + 3 * (size_t)kCT;
-  out[o] = ea0;
-  out[o + 1] = ea1;
-  out[o + 2] = ea2;
-  out[o1] = eb0;
-  out[o1 + 1] = eb1;
-  out[o1 + 2] = eb2;
+  out[o] = e[0];
+  out[o + 1] = e[1];
+  out[o + 2] = e[2];
+  out[o1] = e[3];
+  out[o1 + 1] = e[4];
+  out[o1 + 2] = e[5];
 }
 
 // ef of the targets of group g = sum of the partials of its tiles
@@ -181,13 +200,9 @@ __global__ __launch_bounds__(kCT) void k_field(const int4 *__restrict__ tiles,
 // whose sums meet in LDS in thread order.  blockIdx.y picks the slice of
 // kCT / kRq targets.
 constexpr int kRq = 8;
-__global__ __launch_bounds__(kCT) void k_field_reduce(const int *__restrict__ grp,
-                                                      const int4 *__restrict__ tiles,
-                                                      const double *__restrict__ part,
-                                                      double *__restrict__ ef) {
-  __shared__ double red[kCT][3];
-  const int b0 = grp[blockIdx.x], b1 = grp[blockIdx.x + 1];
-  const int4 tl = tiles[b0];
+__device__ __forceinline__ void field_reduce(int b0, int b1, int tb, int te,
+                                             const double *__restrict__ part,
+                                             double *__restrict__ ef, double (*red)[3]) {
   const int r = threadIdx.x % kRq;
   const int t = blockIdx.y * (kCT / kRq) + threadIdx.x / kRq;
   double e0 = 0.0, e1 = 0.0, e2 = 0.0;
@@ -201,8 +216,8 @@ __global__ __launch_bounds__(kCT) void k_field_reduce(const int *__restrict__ gr
   red[threadIdx.x][1] = e1;
   red[threadIdx.x][2] = e2;
   __syncthreads();
-  const int j = tl.x + t;
-  if (r == 0 && j < tl.y) {
+  const int j = tb + t;
+  if (r == 0 && j < te) {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     for (int k = 0; k < kRq; ++k) {
       s0 += red[threadIdx.x + k][0];
@@ -213,6 +228,68 @@ __global__ __launch_bounds__(kCT) void k_field_reduce(const int *__restrict__ gr
     ef[3 * (size_t)j + 1] = s1;
     ef[3 * (size_t)j + 2] = s2;
   }
+}
+
+__global__ __launch_bounds__(kCT) void k_field_reduce(const int *__restrict__ grp,
+                                                      const int4 *__restrict__ tiles,
+                                                      const double *__restrict__ part,
+                                                      double *__restrict__ ef) {
+  __shared__ double red[kCT][3];
+  const int b0 = grp[blockIdx.x], b1 = grp[blockIdx.x + 1];
+  const int4 tl = tiles[b0];
+  field_reduce(b0, b1, tl.x, tl.y, part, ef, red);
+}
+
+// The same field on a uniform tiling that needs no table: tile b covers
+// target tile b / nchunk (kTW targets) and source chunk b % nchunk (`chunk`
+// sources).  With one chunk the sums go straight to ef; with more, to
+// part[(b kTW + t) 3], which k_field_reduce_uniform sums in chunk order.
+__device__ __forceinline__ int4 uniform_tile(int b, int nchunk, int chunk, int ntrg, int nsrc) {
+  const int tt = b / nchunk, c = b - tt * nchunk;
+  const int tb = tt * kTW, sb = (int)min((long long)c * chunk, (long long)nsrc);
+  return make_int4(tb, min(tb + kTW, ntrg), sb, (int)min((long long)sb + chunk, (long long)nsrc));
+}
+
+__global__ __launch_bounds__(kCT) void k_field_uniform(int ntrg, int nsrc, int nchunk, int chunk,
+                                                       const double *__restrict__ q,
+                                                       const double *__restrict__ src,
+                                                       const double *__restrict__ trg,
+                                                       double *__restrict__ part,
+                                                       double *__restrict__ ef) {
+  __shared__ double4 tile[kCT];
+  double e[6];
+  const int4 tl = uniform_tile(blockIdx.x, nchunk, chunk, ntrg, nsrc);
+  field_tile(tl, q, src, trg, tile, e);
+  if (nchunk == 1) {
+    const int j0 = tl.x + threadIdx.x, j1 = j0 + kCT;
+    if (j0 < tl.y) {
+      ef[3 * (size_t)j0] = e[0];
+      ef[3 * (size_t)j0 + 1] = e[1];
+      ef[3 * (size_t)j0 + 2] = e[2];
+    }
+    if (j1 < tl.y) {
+      ef[3 * (size_t)j1] = e[3];
+      ef[3 * (size_t)j1 + 1] = e[4];
+      ef[3 * (size_t)j1 + 2] = e[5];
+    }
+    return;
+  }
+  const size_t o = ((size_t)blockIdx.x * kTW + threadIdx.x) * 3, o1 = o + 3 * (size_t)kCT;
+  part[o] = e[0];
+  part[o + 1] = e[1];
+  part[o + 2] = e[2];
+  part[o1] = e[3];
+  part[o1 + 1] = e[4];
+  part[o1 + 2] = e[5];
+}
+
+__global__ __launch_bounds__(kCT) void k_field_reduce_uniform(int ntrg, int nchunk,
+                                                              const double *__restrict__ part,
+                                                              double *__restrict__ ef) {
+  __shared__ double red[kCT][3];
+  const int tb = blockIdx.x * kTW;
+  field_reduce(blockIdx.x * nchunk, (blockIdx.x + 1) * nchunk, tb, min(tb + kTW, ntrg), part,
+               ef, red);
 }
 
 __global__ void k_psi(int n, int nylm, double fac, const double *__restrict__ q,
@@ -346,4 +423,78 @@ int make_psi(ddx_handle *h, const double *charge, double *psi) {
   return DDX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// The field without allocation.  Its partial sums live in `scratch`, a
+// file-scope device array of kScratchBytes on every GPU the library runs on
+// (held until the process ends).  In host code the array's name is only a
+// stand-in: its device address on the current GPU comes from
+// hipGetSymbolAddress, once per device, and only that address is used.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+static double *g_scratch[kMaxDevices];
+
+int noalloc_setup(double **out) {
+  int dev = 0;
+  DDX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("device %d beyond the %d devices the scratch table covers", dev, kMaxDevices);
+    return DDX_ERR_ARG;
+  }
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+      return DDX_ERR_HIP;
+    }
+    g_scratch[dev] = static_cast<double *>(p);
+  }
+  if (out) *out = g_scratch[dev];
+  return DDX_OK;
+}
+
 }  // namespace ddx
+
+extern "C" int ddx_efld_noalloc(int nsrc, const double *src, const double *csrc, int ntrg,
+                                const double *ctrg, double *ef, void *stream) {
+  using namespace ddx;
+  if (nsrc < 0 || ntrg < 0 || (nsrc && (!src || !csrc)) || (ntrg && (!ctrg || !ef))) {
+    set_error("ddx_efld_noalloc: invalid arguments");
+    return DDX_ERR_ARG;
+  }
+  double *part = nullptr;
+  DDX_TRY(noalloc_setup(&part));
+  if (ntrg == 0) return DDX_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // chunks as the table-driven path cuts them (about 8192 tiles, at least
+  // 4 kCT sources each), then fewer where the partials would not fit
+  const long long ntt = (ntrg + kTW - 1) / kTW;
+  const long long max_tiles = (long long)(kScratchBytes / (sizeof(double) * 3 * kTW));
+  long long chunk = (ntt * nsrc + 8191) / 8192;
+  chunk = std::max<long long>(4 * kCT, ((chunk + kCT - 1) / kCT) * kCT);
+  long long nchunk = std::max<long long>(1, (nsrc + chunk - 1) / chunk);
+  if (ntt * nchunk > max_tiles) {
+    nchunk = std::max<long long>(1, max_tiles / ntt);
+    chunk = std::max<long long>(1, (nsrc + nchunk - 1) / nchunk);
+    chunk = ((chunk + kCT - 1) / kCT) * kCT;
+    nchunk = std::max<long long>(1, (nsrc + chunk - 1) / chunk);
+  }
+  const size_t need = nchunk > 1 ? (size_t)(ntt * nchunk) * kTW * 3 * sizeof(double) : 0;
+  if (need > kScratchBytes || ntt * nchunk > 0x7fffffffll) {
+    set_error("ddx_efld_noalloc: %lld partial tiles exceed the %zu-byte scratch",
+              ntt * nchunk, kScratchBytes);
+    return DDX_ERR_CAPACITY;
+  }
+  hipLaunchKernelGGL(k_field_uniform, dim3((unsigned)(ntt * nchunk)), dim3(kCT), 0, s, ntrg,
+                     nsrc, (int)nchunk, (int)chunk, src, csrc, ctrg, part, ef);
+  DDX_HIP(hipGetLastError());
+  if (nchunk > 1) {
+    hipLaunchKernelGGL(k_field_reduce_uniform, dim3((unsigned)ntt, kTW / (kCT / kRq)), dim3(kCT),
+                       0, s, ntrg, (int)nchunk, part, ef);
+    DDX_HIP(hipGetLastError());
+  }
+  return DDX_OK;
+}

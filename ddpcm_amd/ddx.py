@@ -29,7 +29,7 @@ EXPORTS = (
     "ddx_set_precision", "ddx_get_stats", "ddx_kernel_timer", "ddx_kernel_timer_read",
     "ddx_batch_init", "ddx_batch_energy_forces", "ddx_batch_result", "ddx_batch_iterations",
     "ddx_batch_sizes", "ddx_gemm_bf16", "ddx_set_guess", "ddx_guess_train", "ddx_guess_reset",
-    "ddx_mlp_grad", "ddx_set_graphs", "ddx_autotune",
+    "ddx_mlp_grad", "ddx_set_graphs", "ddx_autotune", "ddx_efld_noalloc",
 )
 DDX_GUESS_REFERENCE, DDX_GUESS_PREVIOUS, DDX_GUESS_LEARNED, DDX_GUESS_LEARNED_PREVIOUS = 0, 1, 2, 3
 KERNELS = ("sweep_l", "sweep_lstar", "project", "expand", "jacobi", "diis", "extrap", "field",
@@ -80,6 +80,9 @@ def load_library(path: str = LIB_PATH):
     L.ddx_forces_dd.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
     L.ddx_mkzeta.argtypes = [C.c_void_p, _dp, _dp]
     L.ddx_efld.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int, _dp, _dp]
+    # device pointers and a hipStream_t, as integers
+    L.ddx_efld_noalloc.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]
     for f in ("ddx_lx", "ddx_lstarx", "ddx_ldm1x"):
         getattr(L, f).argtypes = [C.c_void_p, _dp, _dp]
     L.ddx_hnorm.argtypes = [C.c_void_p, _dp, _dp]

@@ -95,6 +95,19 @@ int ddx_mkzeta(ddx_handle *h, const double *s, double *zeta);
 int ddx_efld(ddx_handle *h, int nsrc, const double *src, const double *csrc,
              int ntrg, const double *ctrg, double *ef);
 
+/* efld without allocation (no counterpart in the reference; for callers
+ * that cannot allocate device memory: code under HIP graph capture, tight
+ * serving loops).  The same field as ddx_efld, with every pointer in device
+ * memory, on `stream` (a hipStream_t; NULL = the null stream), and no handle.
+ * It calls no hipMalloc, hipFree or allocating library routine: its partial
+ * sums go to a 32 MiB scratch array of the library on each device, which
+ * holds the last call's contents (every value a call reads, it first
+ * writes).  Calls that overlap on one device race on that array.  The source
+ * chunks are sized so that the partials always fit (DDX_ERR_CAPACITY
+ * otherwise); results agree with ddx_efld to rounding. */
+int ddx_efld_noalloc(int nsrc, const double *src, const double *csrc, int ntrg,
+                     const double *ctrg, double *ef, void *stream);
+
 /* operator layer (matvec.f90): lx (:94), lstarx (:161), ldm1x (:246),
  * hnorm (:281).  x, y are (nylm, nsph). */
 int ddx_lx(ddx_handle *h, const double *x, double *y);
