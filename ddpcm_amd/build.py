@@ -17,6 +17,8 @@ FSRC = os.path.join(PKG, "fortran")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libddx.so")
 DRIVER = os.path.join(PKG, "ddx_main")
+TEST_PROG = os.path.join(ROOT, "tests", "fortran", "jd_module")
+PKG_RPATH = "$ORIGIN/../../ddpcm_amd"
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 FLANG = "/opt/rocm/lib/llvm/bin/flang"
@@ -30,7 +32,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["ddx_api.hip", "k_scan.hip", "k_setup.hip", "k_gemm.hip", "k_sweep.hip",
-           "k_solver.hip", "k_coulomb.hip", "k_forces.hip", "k_guess.hip", "tables.cpp"]
+           "k_solver.hip", "k_coulomb.hip", "k_forces.hip", "k_guess.hip", "k_ops.hip", "tables.cpp"]
 
 
 def _compile(src: str) -> str:
@@ -76,6 +78,19 @@ def build_fortran(verbose: bool = False) -> str | None:
         raise RuntimeError("flang failed:\n%s\n%s" % (r.stdout, r.stderr))
     if verbose:
         print("built", DRIVER)
+    # the module's test program (tests/fortran), linked the same way
+    tsrc = os.path.join(ROOT, "tests", "fortran", "jd_module.f90")
+    if os.path.exists(tsrc):
+        tmod = os.path.join(OBJ, "tests")
+        os.makedirs(tmod, exist_ok=True)
+        cmd = [FLANG, "-O2", "-module-dir", tmod] + srcs[:2] + [tsrc, "-o", TEST_PROG,
+                                                                "-L" + PKG, "-lddx",
+                                                                "-Wl,-rpath," + PKG_RPATH]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("flang failed:\n%s\n%s" % (r.stdout, r.stderr))
+        if verbose:
+            print("built", TEST_PROG)
     return DRIVER
 
 

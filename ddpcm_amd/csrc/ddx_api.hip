@@ -115,6 +115,7 @@ int end_direct(ddx_handle *h, Lane &ln, const double *d_psi, int *ok) {
   inst_iterations(h, ln, nit, h->iters_direct);
   DDX_TRY(energy_dot(h, h->sigma.p, d_psi, 0.5 * ((h->eps - 1.0) / h->eps), h->energy.p));
   h->have_sigma = true;
+  h->conv_sigma = *ok != 0;
   return DDX_OK;
 }
 
@@ -124,6 +125,7 @@ int end_adjoint(ddx_handle *h, Lane &ln, int *ok) {
   h->stats.niter_adjoint = nit;
   inst_iterations(h, ln, nit, h->iters_adjoint);
   h->have_s = true;
+  h->conv_s = *ok != 0;
   return DDX_OK;
 }
 
@@ -521,6 +523,36 @@ int ddx_mkrhs(ddx_handle *h, const double *charge, double *phi, double *psi) {
   return sync(h);
 }
 
+int ddx_mkrhs_at(ddx_handle *h, int n, const double *charge, const double *x, const double *y,
+                 const double *z, int ncav, const double *ccav, double *phi, int nylm,
+                 double *psi) {
+  if (!h || n < 1 || ncav < 0 || nylm < 1 || !charge || !x || !y || !z ||
+      (ncav && (!ccav || !phi)) || !psi) {
+    set_error("ddx_mkrhs_at: invalid arguments");
+    return DDX_ERR_ARG;
+  }
+  hipStream_t s = h->stream;
+  std::vector<double4> src(n);
+  for (int i = 0; i < n; ++i) src[i] = make_double4(x[i], y[i], z[i], 0.0);
+  DevArray<double4> dsrc;
+  DevArray<double> dq, dtrg, dphi, dpsi;
+  DevArray<int4> dtile;
+  std::vector<int4> tiles;
+  std::vector<int> grp;
+  coulomb_tiles({0, ncav}, {0, n}, false, tiles, grp);
+  DDX_TRY(upload(dsrc, src.data(), n, s));
+  DDX_TRY(upload(dq, charge, n, s));
+  DDX_TRY(upload(dtrg, ccav, 3 * (size_t)ncav, s));
+  DDX_TRY(upload(dtile, tiles.data(), tiles.size(), s));
+  DDX_TRY(dphi.reserve((size_t)ncav + 1));
+  DDX_TRY(dpsi.reserve((size_t)nylm * n));
+  DDX_TRY(coulomb_potential(h, dtile.p, (int)tiles.size(), dq.p, dsrc.p, dtrg.p, dphi.p, nullptr));
+  DDX_TRY(make_psi_n(h, n, nylm, dq.p, dpsi.p));
+  DDX_TRY(download(phi, dphi.p, ncav, s));
+  DDX_TRY(download(psi, dpsi.p, (size_t)nylm * n, s));
+  return sync(h);
+}
+
 int ddx_cosmo(ddx_handle *h, int star, int cart, const double *phi, const double *glm,
               const double *psi, double *sigma, double *esolv) {
   if (!h || !psi || !sigma) return DDX_ERR_ARG;
@@ -644,7 +676,9 @@ int ddx_jacobi_diis(ddx_handle *h, int star, int diis_max, double tol, const dou
 
 int ddx_get_history(const ddx_handle *h, int star, int cap, double *norms, int *count) {
   if (!h || !count) return DDX_ERR_ARG;
-  const std::vector<double> &v = star ? h->hist_adjoint : h->hist_direct;
+  if (star < 0 || star > 2) return DDX_ERR_ARG;
+  const std::vector<double> &v =
+      star == 2 ? h->hist_ops : star ? h->hist_adjoint : h->hist_direct;
   int c = (int)v.size();
   *count = c;
   if (norms)

@@ -144,6 +144,8 @@ struct Lane {
   bool f32 = false;               // operators in FP32 (inner solves of the refinement)
   DevArray<double> rres, dcorr;   // refinement: FP64 residual and correction
   DevArray<double> dnorm;         // refinement: per-instance norms of the correction
+  std::vector<double> joined;     // refinement: increment norms of all inner solves
+  bool use_joined = false;
 };
 
 }  // namespace ddx
@@ -167,6 +169,7 @@ struct ddx_handle {
   // sizes produced by setup
   int ncav = 0, nnl = 0, ntri = 0, maxdeg = 0;
   bool have_setup = false, have_sigma = false, have_s = false, have_charge = false;
+  bool conv_sigma = false, conv_s = false;   // the last solutions converged
 
   hipStream_t stream = nullptr;
   hipEvent_t ev[10] = {};
@@ -256,7 +259,7 @@ struct ddx_handle {
   ddx::DevArray<double> gprev[2];     // the learned map of the last right-hand side
   bool gprev_ok[2] = {false, false};
 
-  std::vector<double> hist_direct, hist_adjoint;
+  std::vector<double> hist_direct, hist_adjoint, hist_ops;
   std::vector<int> iters_direct, iters_adjoint;   // per instance
   ddx_stats stats{};
   ddx::KernelTimer ktimer[DDX_K_COUNT];

@@ -98,6 +98,8 @@ int coulomb_field(ddx_handle *h, const int4 *tiles, int ntile, const int *grp, i
 int coulomb_field_points(ddx_handle *h, int nsrc, const double *q, const double *src3, int ntrg,
                          const double *trg3, double *ef);
 int make_psi(ddx_handle *h, const double *charge, double *psi);
+// psi (nylm, n) of n charges: sqrt(4 pi) q_i in the l = 0 entry, zeros
+int make_psi_n(ddx_handle *h, int n, int nylm, const double *charge, double *psi);
 // device address of the scratch array of ddx_efld_noalloc on the current
 // device (looked up once per device; out may be null)
 int noalloc_setup(double **out);

@@ -414,13 +414,17 @@ int coulomb_field_points(ddx_handle *h, int nsrc, const double *q, const double 
                        ef);
 }
 
-int make_psi(ddx_handle *h, const double *charge, double *psi) {
-  size_t nn = (size_t)h->n * h->nylm;
+int make_psi_n(ddx_handle *h, int n, int nylm, const double *charge, double *psi) {
+  size_t nn = (size_t)n * nylm;
   double fac = sqrt(4.0 * (4.0 * atan(1.0)));
-  hipLaunchKernelGGL(k_psi, dim3((nn + 255) / 256), dim3(256), 0, h->stream, h->n, h->nylm, fac,
+  hipLaunchKernelGGL(k_psi, dim3((nn + 255) / 256), dim3(256), 0, h->stream, n, nylm, fac,
                      charge, psi);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
+}
+
+int make_psi(ddx_handle *h, const double *charge, double *psi) {
+  return make_psi_n(h, h->n, h->nylm, charge, psi);
 }
 
 // ---------------------------------------------------------------------------

@@ -615,13 +615,13 @@ int ddx_guess_reset(ddx_handle *h, unsigned seed) {
 
 int ddx_guess_train(ddx_handle *h, int steps, double lr, double *loss) {
   if (!h || steps < 1 || !(lr > 0.0)) return DDX_ERR_ARG;
-  if (!h->have_sigma) {
+  if (!h->have_sigma || !h->conv_sigma) {
     set_error("ddx_guess_train needs a converged solve on the handle");
     return DDX_ERR_STATE;
   }
   double l[2] = {0.0, 0.0};
   DDX_TRY(train_guess(h, 0, h->rhs.p, h->sigma.p, steps, lr, &l[0]));
-  if (h->have_s) DDX_TRY(train_guess(h, 1, h->psi.p, h->s.p, steps, lr, &l[1]));
+  if (h->have_s && h->conv_s) DDX_TRY(train_guess(h, 1, h->psi.p, h->s.p, steps, lr, &l[1]));
   if (loss) {
     loss[0] = l[0];
     loss[1] = l[1];

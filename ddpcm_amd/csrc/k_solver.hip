@@ -999,9 +999,11 @@ int solve_end(ddx_handle *h, Lane &ln, int *n_iter, int *ok, std::vector<double>
   *n_iter = hs->ok ? hs->it : ln.max_iter + 1;
   *ok = hs->ok;
   if (hist) {
-    if (h->ninst == 1) hist->assign(hs->hist, hs->hist + (hs->it < kMaxIter ? hs->it : kMaxIter));
-    else hist->clear();
+    if (h->ninst > 1) hist->clear();
+    else if (ln.use_joined) *hist = ln.joined;
+    else hist->assign(hs->hist, hs->hist + (hs->it < kMaxIter ? hs->it : kMaxIter));
   }
+  ln.use_joined = false;
   return DDX_OK;
 }
 
@@ -1073,9 +1075,15 @@ constexpr int kMaxRefine = 6;
 // contraction) stays below the reference's own.
 constexpr double kFinal = 0.2;
 
-// iterations and ok of each instance in the solve that just ended on ln
-void add_iterations(const ddx_handle *h, const Lane &ln, std::vector<int> &its,
+// iterations and ok of each instance in the solve that just ended on ln;
+// for a single system its increment norms join the lane's refinement history
+void add_iterations(const ddx_handle *h, Lane &ln, std::vector<int> &its,
                     std::vector<int> &ok) {
+  if (h->ninst == 1) {
+    const DiisState *hs = ln.h_diis;
+    const int c = std::min(hs->it, kMaxIter);
+    ln.joined.insert(ln.joined.end(), hs->hist, hs->hist + c);
+  }
   for (int q = 0; q < h->ninst; ++q) {
     const bool oq = h->ninst > 1 ? ln.h_res[2 * q + 1] != 0 : ln.h_diis->ok != 0;
     const int iq = h->ninst > 1 ? (oq ? ln.h_res[2 * q] : ln.max_iter) : ln.h_diis->it;
@@ -1134,6 +1142,7 @@ int solve_run_mixed(ddx_handle *h, Lane *const *lanes, int count) {
     DDX_TRY(ln.dnorm.reserve(ninst + 1));
     DDX_TRY(ln.part.reserve((size_t)h->nbj * kPartStride + h->nbe + 1));
     DDX_TRY(inst_norms(h, ln, ln.x, nullptr, kRelTol, ln.diis.p, nullptr));
+    ln.joined.clear();
   }
   DDX_TRY(run_f32(h, lanes, count));
   bool all_ok = true;
@@ -1195,6 +1204,7 @@ int solve_run_mixed(ddx_handle *h, Lane *const *lanes, int count) {
     }
     ln.h_diis->it = itmax;
     ln.h_diis->ok = all_ok ? 1 : 0;
+    ln.use_joined = true;
     if (!ln.star) DDX_TRY(cf_scale(h, ln.stream, ln.x, ln.xs.p));
   }
   return DDX_OK;

@@ -30,6 +30,8 @@ EXPORTS = (
     "ddx_batch_init", "ddx_batch_energy_forces", "ddx_batch_result", "ddx_batch_iterations",
     "ddx_batch_sizes", "ddx_gemm_bf16", "ddx_set_guess", "ddx_guess_train", "ddx_guess_reset",
     "ddx_mlp_grad", "ddx_set_graphs", "ddx_autotune", "ddx_efld_noalloc",
+    "ddx_jacobi_diis_ops", "ddx_op_lx", "ddx_op_lstarx", "ddx_op_ldm1x", "ddx_op_hnorm",
+    "ddx_mkrhs_at",
 )
 DDX_GUESS_REFERENCE, DDX_GUESS_PREVIOUS, DDX_GUESS_LEARNED, DDX_GUESS_LEARNED_PREVIOUS = 0, 1, 2, 3
 KERNELS = ("sweep_l", "sweep_lstar", "project", "expand", "jacobi", "diis", "extrap", "field",
@@ -87,6 +89,12 @@ def load_library(path: str = LIB_PATH):
         getattr(L, f).argtypes = [C.c_void_p, _dp, _dp]
     L.ddx_hnorm.argtypes = [C.c_void_p, _dp, _dp]
     L.ddx_jacobi_diis.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp, _ip, _ip]
+    L.ddx_jacobi_diis_ops.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.c_double, _dp, _dp, _ip, _ip, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]
+    L.ddx_op_hnorm.restype = C.c_double
+    L.ddx_mkrhs_at.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp,
+                               C.c_int, _dp]
     L.ddx_get_history.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _ip]
     L.ddx_energy_forces.argtypes = [C.c_void_p, _dp, _dp, _dp]
     L.ddx_upload.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]

@@ -253,15 +253,16 @@ contains
   end subroutine ddmkzeta
 
   ! ---------------------------------------------------------------- solute
-  ! mkrhs (mkrhs.f90:1) for point charges; the coordinates are those of
-  ! ddinit (x, y, z are accepted for interface compatibility)
+  ! mkrhs (mkrhs.f90:1): the potential at the nc points cc of the n point
+  ! charges at (x, y, z), and psi; any coordinates, not only ddinit's
   subroutine mkrhs(n, charge, x, y, z, nc, cc, phi, ny, psi)
     integer, intent(in) :: n, nc, ny
     real*8, dimension(n), intent(in) :: charge, x, y, z
     real*8, dimension(3,nc), intent(in) :: cc
     real*8, dimension(nc), intent(inout) :: phi
     real*8, dimension(ny,n), intent(inout) :: psi
-    call ddx_check(ddx_mkrhs(handle, charge, phi, psi), 'mkrhs')
+    call ddx_check(ddx_mkrhs_at(handle, int(n, c_int), charge, x, y, z, int(nc, c_int), cc, phi, &
+                                int(ny, c_int), psi), 'mkrhs')
   end subroutine mkrhs
 
   ! efld (efld.f90:1): field of nsrc point sources at ntrg targets
@@ -305,6 +306,51 @@ contains
     call ddx_check(ddx_hnorm(handle, x, v), 'hnorm')
     hnorm = v
   end function hnorm
+
+  ! jacobi_diis (jacobi_diis.f90:43-55): the reference's solver interface
+  ! with caller-supplied matvec, dm1vec and (norm 4) u_norm.  With this
+  ! module's own lx or lstarx, ldm1x and hnorm the whole solve runs on the
+  ! GPU; any other operator is called back on the host while the iterate
+  ! and the DIIS history stay on the GPU.  Residual lines as the
+  ! reference's when lprint > 0.
+  subroutine jacobi_diis(n, lprint, diis_max, norm, tol, rhs, x, n_iter, ok, matvec, dm1vec, &
+                         u_norm)
+    integer, intent(in) :: n, diis_max, norm, lprint
+    real*8, intent(in) :: tol
+    real*8, dimension(n), intent(in) :: rhs
+    real*8, dimension(n), intent(inout) :: x
+    integer, intent(inout) :: n_iter
+    logical, intent(inout) :: ok
+    external :: matvec, dm1vec
+    real*8, optional :: u_norm
+    external :: u_norm
+    type(c_funptr) :: fm, fd, fn
+    integer(c_int) :: it, iok, rc
+    if (norm == 4 .and. .not. present(u_norm)) then
+      write(iout,*) ' must provide a function norm(n,x) to evaluate the norm of the increment'
+      stop
+    end if
+    ! this module's operators become the library's own callbacks, which the
+    ! library recognises and runs on the device
+    fm = c_funloc(matvec)
+    if (c_associated(fm, c_funloc(lx))) fm = c_funloc(ddx_op_lx)
+    if (c_associated(fm, c_funloc(lstarx))) fm = c_funloc(ddx_op_lstarx)
+    fd = c_funloc(dm1vec)
+    if (c_associated(fd, c_funloc(ldm1x))) fd = c_funloc(ddx_op_ldm1x)
+    fn = c_null_funptr
+    if (present(u_norm)) then
+      fn = c_funloc(u_norm)
+      if (c_associated(fn, c_funloc(hnorm))) fn = c_funloc(ddx_op_hnorm)
+    end if
+    flush(iout)
+    it = int(n_iter, c_int)
+    iok = 0
+    rc = ddx_jacobi_diis_ops(handle, int(n, c_int), int(lprint, c_int), int(diis_max, c_int), &
+                             int(norm, c_int), tol, rhs, x, it, iok, fm, fd, fn)
+    if (rc /= DDX_OK .and. rc /= DDX_ERR_NOCONV) call ddx_check(rc, 'jacobi_diis')
+    n_iter = it
+    ok = iok /= 0
+  end subroutine jacobi_diis
 
   ! sprod (ddcosmo.f90:531)
   real*8 function sprod(n, u, v)

@@ -3,7 +3,7 @@ module ddx_cabi
   use, intrinsic :: iso_c_binding
   implicit none
 
-  integer(c_int), parameter :: DDX_OK = 0, DDX_ERR_NOCONV = 3
+  integer(c_int), parameter :: DDX_OK = 0, DDX_ERR_ARG = 1, DDX_ERR_NOCONV = 3
   integer(c_int), parameter :: DDX_PREC_FP64 = 0, DDX_PREC_MIXED = 1
 
   interface
@@ -128,6 +128,54 @@ module ddx_cabi
       real(c_double), intent(inout) :: x(*)
       integer(c_int), intent(inout) :: n_iter
       integer(c_int), intent(out) :: ok
+    end function
+
+    integer(c_int) function ddx_jacobi_diis_ops(h, n, lprint, diis_max, norm, tol, rhs, x, &
+                                                n_iter, ok, matvec, dm1vec, u_norm) &
+        bind(C, name='ddx_jacobi_diis_ops')
+      import :: c_ptr, c_int, c_double, c_funptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: n, lprint, diis_max, norm
+      real(c_double), value :: tol
+      real(c_double), intent(in) :: rhs(*)
+      real(c_double), intent(inout) :: x(*)
+      integer(c_int), intent(inout) :: n_iter
+      integer(c_int), intent(out) :: ok
+      type(c_funptr), value :: matvec, dm1vec, u_norm
+    end function
+
+    ! the handle's operators as callbacks of ddx_jacobi_diis_ops
+    subroutine ddx_op_lx(n, x, y) bind(C, name='ddx_op_lx')
+      import :: c_int, c_double
+      integer(c_int), intent(in) :: n
+      real(c_double), intent(in) :: x(*)
+      real(c_double), intent(inout) :: y(*)
+    end subroutine
+    subroutine ddx_op_lstarx(n, x, y) bind(C, name='ddx_op_lstarx')
+      import :: c_int, c_double
+      integer(c_int), intent(in) :: n
+      real(c_double), intent(in) :: x(*)
+      real(c_double), intent(inout) :: y(*)
+    end subroutine
+    subroutine ddx_op_ldm1x(n, x, y) bind(C, name='ddx_op_ldm1x')
+      import :: c_int, c_double
+      integer(c_int), intent(in) :: n
+      real(c_double), intent(in) :: x(*)
+      real(c_double), intent(inout) :: y(*)
+    end subroutine
+    real(c_double) function ddx_op_hnorm(n, x) bind(C, name='ddx_op_hnorm')
+      import :: c_int, c_double
+      integer(c_int), intent(in) :: n
+      real(c_double), intent(in) :: x(*)
+    end function
+
+    integer(c_int) function ddx_mkrhs_at(h, n, charge, x, y, z, ncav, ccav, phi, nylm, psi) &
+        bind(C, name='ddx_mkrhs_at')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: n, ncav, nylm
+      real(c_double), intent(in) :: charge(*), x(*), y(*), z(*), ccav(*)
+      real(c_double), intent(out) :: phi(*), psi(*)
     end function
 
     integer(c_int) function ddx_get_history(h, star, cap, norms, count) &

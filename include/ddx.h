@@ -77,9 +77,17 @@ int ddx_get_neighbours(const ddx_handle *h, int *inl, int *nl);
  * the psi vector; phi(ncav), psi(nylm,nsph). */
 int ddx_mkrhs(ddx_handle *h, const double *charge, double *phi, double *psi);
 
+/* mkrhs (mkrhs.f90:1) with its whole argument list: the potential at the
+ * ncav points ccav(3, ncav) of the n charges at (x, y, z), and psi(nylm, n)
+ * (sqrt(4 pi) q_i in the l = 0 entries).  Any coordinates, not only the
+ * handle's; the handle gives the stream. */
+int ddx_mkrhs_at(ddx_handle *h, int n, const double *charge, const double *x,
+                 const double *y, const double *z, int ncav, const double *ccav,
+                 double *phi, int nylm, double *psi);
+
 /* cosmo (cosmo.f90:1): star=0 solves L X = g (g from phi when cart=1, else
- * glm) and returns esolv; star=1 solves L* S = psi.  The guess is always
- * D^-1 rhs, as in the reference. */
+ * glm) and returns esolv; star=1 solves L* S = psi.  The guess is D^-1 rhs,
+ * as in the reference, unless ddx_set_guess chose another. */
 int ddx_cosmo(ddx_handle *h, int star, int cart, const double *phi,
               const double *glm, const double *psi, double *sigma,
               double *esolv);
@@ -115,14 +123,49 @@ int ddx_lstarx(ddx_handle *h, const double *x, double *y);
 int ddx_ldm1x(ddx_handle *h, const double *x, double *y);
 int ddx_hnorm(ddx_handle *h, const double *x, double *norm);
 
-/* jacobi_diis (jacobi_diis.f90:43) with norm = 4 (hnorm); the matvec
- * callback of the reference is the device operator selected by star
- * (0: lx, 1: lstarx) and dm1vec is ldm1x.  x holds the guess on entry.
- * n_iter: in = maximum iterations, out = iterations done; ok = converged. */
+/* jacobi_diis (jacobi_diis.f90:43-55), the reference's plugin interface:
+ * jacobi_diis(n, lprint, diis_max, norm, tol, rhs, x, n_iter, ok, matvec,
+ * dm1vec, u_norm).  The callbacks take the reference's Fortran argument
+ * passing (everything by reference): subroutine matvec(n, x, y) and
+ * dm1vec(n, x, y), real*8 function u_norm(n, x).  norm: 1 tests the rms of
+ * the increment, 2 its max, 3 the rms and 10 tol for the max, 4 u_norm
+ * (jacobi_diis.f90:132-156); lprint > 0 prints the reference's residual
+ * lines.  x holds the guess on entry; n_iter: in = maximum iterations, out =
+ * iterations done (n_iter + 1 with DDX_ERR_NOCONV, as the reference leaves
+ * it); ok = converged; diis_max 0 (plain Jacobi) or 11..25.
+ * With the handle's own operators (ddx_op_lx or ddx_op_lstarx, ddx_op_ldm1x,
+ * norm 4 with ddx_op_hnorm) the whole solve runs on the device, in FP64.
+ * Any other callbacks are called on the host with host vectors; the iterate
+ * and the DIIS history stay in device memory, where the Gram rows, the
+ * extrapolation and the norms are formed.  h may be NULL for those.
+ * Where the reference prints and stops (norm 4 without u_norm), this
+ * returns DDX_ERR_ARG. */
+typedef void (*ddx_matvec_fn)(const int *n, const double *x, double *y);
+typedef double (*ddx_norm_fn)(const int *n, const double *x);
+int ddx_jacobi_diis_ops(ddx_handle *h, int n, int lprint, int diis_max, int norm,
+                        double tol, const double *rhs, double *x, int *n_iter,
+                        int *ok, ddx_matvec_fn matvec, ddx_matvec_fn dm1vec,
+                        ddx_norm_fn u_norm);
+/* The handle's operators lx (matvec.f90:94), lstarx (:161), ldm1x (:246)
+ * and hnorm (:281) as such callbacks: they act on the handle of the
+ * ddx_jacobi_diis_ops call running on the calling thread (and record an
+ * error for that call outside one, or when n is not nylm * nsph). */
+void ddx_op_lx(const int *n, const double *x, double *y);
+void ddx_op_lstarx(const int *n, const double *x, double *y);
+void ddx_op_ldm1x(const int *n, const double *x, double *y);
+double ddx_op_hnorm(const int *n, const double *x);
+
+/* The device solve of ddx_jacobi_diis_ops with the handle's operators,
+ * selected by star (0: lx, 1: lstarx), dm1vec = ldm1x and norm 4 (hnorm),
+ * lprint 0.  Same arguments otherwise; returns DDX_OK also when it did not
+ * converge (ok = 0, n_iter = maximum + 1). */
 int ddx_jacobi_diis(ddx_handle *h, int star, int diis_max, double tol,
                     const double *rhs, double *x, int *n_iter, int *ok);
 
-/* residual history of the last solve (star 0 / 1): up to cap norms */
+/* residual history of the last solve: star 0 / 1 the last direct /
+ * adjoint solve (under mixed precision, the increment norms of all its
+ * inner solves in order), 2 the last ddx_jacobi_diis_ops call; count = all
+ * norms, of which up to cap are written */
 int ddx_get_history(const ddx_handle *h, int star, int cap, double *norms,
                     int *count);
 
@@ -176,8 +219,9 @@ int ddx_batch_sizes(const ddx_handle *h, int *count, int *sph_off, int *cav_off)
  * inside FP64 iterative refinement: the residual g - (D + L) x, the
  * Jacobi/DIIS update, Gram row and bordered solve, the energies and the
  * forces stay FP64.  Results agree with FP64 to well inside the
- * convergence threshold; iteration counts are those of the inner solves
- * and differ from the reference's.  ddx_lx / ddx_lstarx apply the FP32
+ * convergence threshold; iteration counts are the sums over the inner
+ * solves (they may exceed the 200 of one solve) and differ from the
+ * reference's.  ddx_lx / ddx_lstarx apply the FP32
  * operators while it is set; ddx_jacobi_diis always runs FP64. */
 int ddx_set_precision(ddx_handle *h, int prec);
 

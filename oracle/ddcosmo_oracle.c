@@ -606,6 +606,83 @@ static int diis(size_t n, int *nmat, int ndiis, double **x, double **e,
 
 /* Jacobi iteration with DIIS, norm = 4 (user norm hnorm), jacobi_diis.f90:43-191.
  * Returns 1 when converged.  x holds the guess on entry. */
+/* jacobi_diis with the reference's full plugin interface
+ * (jacobi_diis.f90:43-191): callbacks with Fortran argument passing
+ * (subroutine matvec(n,x,y), dm1vec(n,x,y), real*8 function u_norm(n,x)),
+ * norm 1..4 with the reference's tests (:132-156: norm 1 tests the rms,
+ * norm 2 the max, norm 3 both with 10 tol for the max, norm 4 u_norm), and
+ * rmsvec (:405-438).  norms[2 (it-1)] = (rms or u_norm, max or -1) per
+ * iteration, up to 256 iterations.  Returns n_iter as the reference leaves
+ * it (n_iter + 1 when it did not converge); *ok = converged. */
+int orc_jacobi_diis_ops(int n, int diis_max, int norm, double tol, const double *rhs,
+                        double *x, int n_iter, orc_fmatvec_fn matvec,
+                        orc_fmatvec_fn dm1vec, orc_fnorm_fn u_norm, int *ok_out,
+                        double *norms) {
+  size_t nn = (size_t)n;
+  int dodiis = diis_max != 0, nmat = 1, ok = 0, it;
+  int lenb = diis_max + 1;
+  double tol_max = 10.0 * tol;
+  double **xd = NULL, **ed = NULL, *b = NULL;
+  if (dodiis) {
+    xd = (double **)xcalloc(diis_max, sizeof(double *));
+    ed = (double **)xcalloc(diis_max, sizeof(double *));
+    for (int k = 0; k < diis_max; ++k) {
+      xd[k] = (double *)xcalloc(nn, sizeof(double));
+      ed[k] = (double *)xcalloc(nn, sizeof(double));
+    }
+    b = (double *)xcalloc((size_t)lenb * lenb, sizeof(double));
+  }
+  double *x_new = (double *)xcalloc(nn, sizeof(double));
+  double *y = (double *)xcalloc(nn, sizeof(double));
+  for (it = 1; it <= n_iter; ++it) {
+    matvec(&n, x, y);
+    for (size_t k = 0; k < nn; ++k) y[k] = rhs[k] - y[k];
+    dm1vec(&n, y, x_new);
+    if (dodiis) {
+      for (size_t k = 0; k < nn; ++k) {
+        xd[nmat - 1][k] = x_new[k];
+        ed[nmat - 1][k] = x_new[k] - x[k];
+      }
+      diis(nn, &nmat, diis_max, xd, ed, b, x_new);
+    }
+    for (size_t k = 0; k < nn; ++k) x[k] = x_new[k] - x[k];
+    double rms_norm = 0.0, max_norm = 0.0;
+    if (norm <= 3) {
+      for (size_t k = 0; k < nn; ++k) {
+        max_norm = fmax(max_norm, fabs(x[k]));
+        rms_norm = rms_norm + x[k] * x[k];
+      }
+      rms_norm = sqrt(rms_norm / (double)n);
+      if (norm == 1) ok = rms_norm < tol;
+      else if (norm == 2) ok = max_norm < tol;
+      else ok = rms_norm < tol && max_norm < tol_max;
+    } else {
+      max_norm = -1.0;
+      rms_norm = u_norm(&n, x);
+      ok = rms_norm < tol;
+    }
+    if (norms && it <= 256) {
+      norms[2 * (it - 1)] = rms_norm;
+      norms[2 * (it - 1) + 1] = max_norm;
+    }
+    memcpy(x, x_new, nn * sizeof(double));
+    if (ok) break;
+  }
+  if (dodiis) {
+    for (int k = 0; k < diis_max; ++k) {
+      free(xd[k]);
+      free(ed[k]);
+    }
+    free(xd);
+    free(ed);
+    free(b);
+  }
+  free(x_new);
+  free(y);
+  *ok_out = ok;
+  return ok ? it : n_iter + 1;
+}
+
 int orc_jacobi_diis(const orc_state *st, int n, int diis_max, double tol,
                     const double *rhs, double *x, int n_iter,
                     orc_matvec_fn matvec, orc_hist *hist) {

@@ -79,6 +79,15 @@ int    orc_jacobi_diis(const orc_state *st, int n, int diis_max, double tol,
                        const double *rhs, double *x, int n_iter,
                        orc_matvec_fn matvec, orc_hist *hist);
 
+/* the reference's plugin interface (jacobi_diis.f90:43-55), Fortran-style
+ * callbacks: arguments by reference */
+typedef void (*orc_fmatvec_fn)(const int *n, const double *x, double *y);
+typedef double (*orc_fnorm_fn)(const int *n, const double *x);
+int    orc_jacobi_diis_ops(int n, int diis_max, int norm, double tol,
+                           const double *rhs, double *x, int n_iter,
+                           orc_fmatvec_fn matvec, orc_fmatvec_fn dm1vec,
+                           orc_fnorm_fn u_norm, int *ok, double *norms);
+
 int    orc_cosmo(const orc_state *st, int star, int cart, const double *phi,
                  const double *glm, const double *psi, double *sigma,
                  double *esolv, orc_hist *hist);

@@ -23,6 +23,11 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 
 
+# callbacks with the reference's Fortran argument passing (jacobi_diis.f90:33-55)
+MATVEC_FN = C.CFUNCTYPE(None, _ip, _dp, _dp)
+NORM_FN = C.CFUNCTYPE(C.c_double, _ip, _dp)
+
+
 def build():
     subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
 
@@ -47,6 +52,8 @@ def _lib():
     lib.orc_intrhs.argtypes = [C.c_void_p, _dp, _dp]
     lib.orc_jacobi_diis.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp, _dp, C.c_int,
                                     C.c_void_p, C.c_void_p]
+    lib.orc_jacobi_diis_ops.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp,
+                                        C.c_int, MATVEC_FN, MATVEC_FN, NORM_FN, _ip, _dp]
     lib.orc_cosmo.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_void_p]
     lib.orc_forces_dd.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
     lib.orc_ddmkzeta.argtypes = [C.c_void_p, _dp, _dp]
@@ -254,3 +261,40 @@ def efld(src, csrc, ctrg):
     ef = np.zeros((ctrg.shape[0], 3))
     lib().orc_efld(src.size, _p(src), _p(csrc), ctrg.shape[0], _p(ctrg), _p(ef))
     return ef
+
+
+def dense_callbacks(a, d):
+    """matvec y = A x, dm1vec y = x / d and u_norm sqrt(sum_i x_i^2 / i / n)
+    for a dense system, summed in index order as a plain loop would (the
+    callbacks of oracle/jd_dump.f90), as ctypes function pointers."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    inv_i = 1.0 / np.arange(1, d.size + 1, dtype=np.float64)
+
+    def mv(n_p, x_p, y_p):
+        n = n_p[0]
+        x = np.ctypeslib.as_array(x_p, shape=(n,))
+        np.ctypeslib.as_array(y_p, shape=(n,))[:] = np.cumsum(a * x[None, :], axis=1)[:, -1]
+
+    def dm1(n_p, x_p, y_p):
+        n = n_p[0]
+        np.ctypeslib.as_array(y_p, shape=(n,))[:] = np.ctypeslib.as_array(x_p, shape=(n,)) / d
+
+    def nrm(n_p, x_p):
+        n = n_p[0]
+        x = np.ctypeslib.as_array(x_p, shape=(n,))
+        return float(np.sqrt(np.cumsum((x * x) * inv_i)[-1] / n))
+
+    return MATVEC_FN(mv), MATVEC_FN(dm1), NORM_FN(nrm)
+
+
+def jacobi_diis_ops(n, diis_max, norm, tol, rhs, x, n_iter, matvec, dm1vec, u_norm):
+    """The reference's jacobi_diis with callbacks; returns (x, n_iter, ok,
+    norms (iterations, 2): rms or u_norm, max or -1)."""
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    x = np.array(x, dtype=np.float64)
+    ok = C.c_int(0)
+    norms = np.zeros((256, 2))
+    it = lib().orc_jacobi_diis_ops(n, diis_max, norm, tol, _p(rhs), _p(x), n_iter, matvec,
+                                   dm1vec, u_norm, C.byref(ok), _p(norms))
+    return x, it, ok.value, norms[:min(it if ok.value else n_iter, 256)]

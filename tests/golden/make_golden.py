@@ -28,6 +28,7 @@ from ddpcm_amd import inputs  # noqa: E402
 
 REF = os.environ.get("DDPCM_REFERENCE", "/root/reference")
 DUMP = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
+JD_DUMP = os.path.join(ROOT, "oracle", "_ref", "jd_dump")
 MOLECULES = ["formaldehyde", "benzene", "caffeine", "taxol", "ala25", "crambin"]
 
 
@@ -66,10 +67,16 @@ def read_dump(prefix):
     return hdr, d
 
 
-def run_case(name, deck, keep):
-    prefix = os.path.join("/tmp", "golden_" + name)
-    out = subprocess.run([DUMP, deck, prefix, "probe"], capture_output=True, text=True,
-                         check=True).stdout
+def run_case(name, deck, keep, prefix=None, every=1, probe=True):
+    """Runs ref_dump on the deck (or reads the dump already at `prefix` with
+    its stdout in prefix.out) and writes <name>.npz.  With every > 1 the
+    (nylm, nsph) solutions keep only every `every`-th sphere (sph_index)."""
+    if prefix and os.path.exists(prefix + ".hdr"):
+        out = open(prefix + ".out").read()
+    else:
+        prefix = prefix or os.path.join("/tmp", "golden_" + name)
+        args = [DUMP, deck, prefix] + (["probe"] if probe else [])
+        out = subprocess.run(args, capture_output=True, text=True, check=True).stdout
     iters = [line for line in out.splitlines() if "iter=" in line]
     hdr, d = read_dump(prefix)
     # iteration counts: the residual lines restart at 1 for the adjoint solve
@@ -84,10 +91,15 @@ def run_case(name, deck, keep):
     rec = {"esolv": np.float64(hdr["esolv"]), "ncav": np.int64(hdr["ncav"]),
            "niter_direct": np.int64(counts[0]),
            "niter_adjoint": np.int64(counts[1] if len(counts) > 1 else 0),
-           "residuals": np.array(norms), "hnorm_probe": np.float64(hdr["hnorm_probe"])}
+           "residuals": np.array(norms), "hnorm_probe": np.float64(hdr.get("hnorm_probe", 0.0))}
+    if "hnorm_probe" not in hdr:
+        rec.pop("hnorm_probe")
+    sub = np.arange(0, hdr["nsph"], every)
     for k in keep:
         if k in d:
-            rec[k] = d[k]
+            rec[k] = d[k][sub] if every > 1 and k in ("sigma", "s") else d[k]
+    if every > 1:
+        rec["sph_index"] = sub
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **rec)
     print("%-14s n=%5d ncav=%7d E=%.12e iters=%s" % (name, hdr["nsph"], hdr["ncav"],
                                                     hdr["esolv"], counts))
@@ -100,12 +112,139 @@ SOLUTION = ["phi", "rhs", "sigma", "s", "fxdd", "zeta", "fx", "probe_v", "probe_
             "probe_lstarx", "inl", "nl"]
 
 
+# Dense systems (D + A) x = rhs for the reference's jacobi_diis with user
+# callbacks (jacobi_diis.f90:43-55): (name, n, diis_max, norm, tol, n_iter,
+# off-diagonal scale).  "slow" runs past 25 iterations (the drop-10 rule),
+# "plain" is Jacobi without DIIS, "fail" stops at n_iter without converging.
+JACOBI_CASES = [
+    ("slow", 160, 25, 4, 1e-12, 200, 0.95),
+    ("norm1", 48, 25, 1, 1e-9, 200, 0.7),
+    ("norm2", 48, 25, 2, 1e-9, 200, 0.7),
+    ("norm3", 48, 25, 3, 1e-9, 200, 0.7),
+    ("plain", 48, 0, 3, 1e-8, 200, 0.5),
+    ("fail", 48, 25, 4, 1e-12, 5, 0.7),
+]
+
+
+def dense_system(n, scale, seed):
+    """D in [1, 2]; A random with zero diagonal, scaled so that the Jacobi
+    iteration matrix D^-1 A has spectral radius `scale`."""
+    rng = np.random.default_rng(seed)
+    d = rng.uniform(1.0, 2.0, size=n)
+    a = rng.normal(size=(n, n))
+    np.fill_diagonal(a, 0.0)
+    rho = np.max(np.abs(np.linalg.eigvals(a / d[:, None])))
+    a *= scale / rho
+    rhs = rng.normal(size=n)
+    x0 = rhs / d
+    return a, d, rhs, x0
+
+
+def jacobi_dense():
+    out = {}
+    for k, (name, n, diis_max, norm, tol, n_iter, scale) in enumerate(JACOBI_CASES):
+        a, d, rhs, x0 = dense_system(n, scale, 7000 + k)
+        fin = os.path.join("/tmp", "jd_%s.in" % name)
+        fout = os.path.join("/tmp", "jd_%s.out" % name)
+        with open(fin, "wb") as f:
+            np.array([n, diis_max, norm, n_iter, 1], dtype="<i4").tofile(f)
+            np.array([tol], dtype="<f8").tofile(f)
+            for v in (a.T, d, rhs, x0):   # column-major A
+                np.ascontiguousarray(v, dtype="<f8").tofile(f)
+        lines = subprocess.run([JD_DUMP, fin, fout], capture_output=True, text=True,
+                               check=True).stdout.splitlines()
+        raw = open(fout, "rb").read()
+        it, ok = np.frombuffer(raw, dtype="<i4", count=2)
+        x = np.frombuffer(raw, dtype="<f8", offset=8, count=n)
+        for key, v in (("a", a), ("d", d), ("rhs", rhs), ("x0", x0), ("x", x),
+                       ("n_iter", np.int64(it)), ("ok", np.int64(ok)),
+                       ("params", np.array([n, diis_max, norm, n_iter], dtype=np.int64)),
+                       ("tol", np.float64(tol)), ("lines", np.array(lines, dtype=str))):
+            out["%s_%s" % (name, key)] = v
+        print("jacobi %-6s n=%3d diis=%2d norm=%d iters=%3d ok=%d" % (name, n, diis_max, norm,
+                                                                     it, ok))
+    np.savez_compressed(os.path.join(HERE, "jacobi_dense.npz"), **out)
+
+
+def _dump_ef(name, d, iconv=None):
+    """E, F (bohr-based, (n, 3)) and iteration counts of the reference on deck d."""
+    if iconv is not None:
+        d = inputs.DDInput(d.iprint, d.nproc, d.lmax, d.ngrid, iconv, d.igrad, d.eps, d.eta,
+                           d.charge, d.xyz, d.rvdw)
+    d.nproc = os.cpu_count() or 1
+    deck = os.path.join("/tmp", "golden_%s.txt" % name)
+    inputs.write_input(deck, d)
+    prefix = os.path.join("/tmp", "golden_" + name)
+    out = subprocess.run([DUMP, deck, prefix], capture_output=True, text=True,
+                         check=True).stdout
+    hdr, dd = read_dump(prefix)
+    its = [int(l.split("iter=")[1].split()[0]) for l in out.splitlines() if "iter=" in l]
+    counts, cur = [], 0
+    for it in its:
+        if it == 1 and cur:
+            counts.append(cur)
+        cur = it
+    counts.append(cur)
+    for ext in (".hdr", ".bin"):
+        os.remove(prefix + ext)
+    return hdr["esolv"], dd["fx"], counts, hdr["ncav"]
+
+
+def batch_sample(every=32):
+    """Every 32nd molecule of the 512-molecule batch (inputs.batch_inputs)."""
+    decks = inputs.batch_inputs()
+    qs = list(range(0, len(decks), every))
+    rec = {"q": np.array(qs), "esolv": [], "niter_direct": [], "niter_adjoint": [],
+           "nsph": [], "ncav": [], "esolv_exact": []}
+    fx, fxx = [], []
+    for q in qs:
+        e, f, c, nc = _dump_ef("c4_%d" % q, decks[q])
+        ex, fe, _, _ = _dump_ef("c4x_%d" % q, decks[q], iconv=13)
+        rec["esolv_exact"].append(ex)
+        fxx.append(fe)
+        rec["esolv"].append(e)
+        rec["niter_direct"].append(c[0])
+        rec["niter_adjoint"].append(c[1])
+        rec["nsph"].append(decks[q].n)
+        rec["ncav"].append(nc)
+        fx.append(f)
+        print("c4 molecule %3d n=%3d E=%.12e iters=%s" % (q, decks[q].n, e, c))
+    rec = {k: np.array(v) for k, v in rec.items()}
+    rec["fx"] = np.concatenate(fx)
+    rec["fx_exact"] = np.concatenate(fxx)
+    np.savez_compressed(os.path.join(HERE, "c4_sample.npz"), **rec)
+
+
+def trajectory_frames(first=8, count=4):
+    """Frames first .. first+count-1 of the 1000-atom trajectory
+    (inputs.trajectory, lmax 8, 590 points): the reference's E and F at
+    conv 1e-8 from its D^-1 rhs guess, and the solution converged to 1e-13."""
+    rec = {"frames": np.arange(first, first + count)}
+    keys = ("esolv", "fx", "niter_direct", "niter_adjoint", "esolv_exact", "fx_exact")
+    out = {k: [] for k in keys}
+    for k, (q, xyz, r) in enumerate(inputs.trajectory()):
+        if k < first:
+            continue
+        if k >= first + count:
+            break
+        d = inputs.DDInput(0, 1, 8, 590, 8, 1, 78.39, 0.1, q, xyz, r)
+        e, f, c, _ = _dump_ef("c5_%d" % k, d)
+        ex, fxx, _, _ = _dump_ef("c5x_%d" % k, d, iconv=13)
+        for key, v in zip(keys, (e, f, c[0], c[1], ex, fxx)):
+            out[key].append(v)
+        print("c5 frame %d E=%.12e exact %.12e iters=%s" % (k, e, ex, c))
+    rec.update({k: np.array(v) for k, v in out.items()})
+    np.savez_compressed(os.path.join(HERE, "c5_frames.npz"), **rec)
+
+
 def main():
     if not os.path.exists(DUMP):
         sys.exit("build the reference first: make -C oracle ref")
     os.makedirs(os.path.join(HERE, "inputs"), exist_ok=True)
     os.makedirs(os.path.join(HERE, "logs"), exist_ok=True)
     only = sys.argv[1:]   # case names to (re)write; all when empty
+    if not only or "jacobi_dense" in only:
+        jacobi_dense()
     for m in MOLECULES:
         if only and m not in only:
             continue
@@ -128,6 +267,36 @@ def main():
         "l16": (inputs.config_input("c1", n=12, seed=4016, lmax=16, ngrid=1202, igrad=1),
                 SOLUTION),
     }
+    # full-size workloads: energies, forces, iteration counts and sampled
+    # solutions only (the 20 000-atom dump is 1.5 GB; about 20 minutes on 8
+    # cores).  DDX_GOLDEN_DUMPS names a folder with dumps made beforehand as
+    # <case>.hdr/.bin/.out.
+    dumps = os.environ.get("DDX_GOLDEN_DUMPS", "/tmp")
+    large = {
+        "c3": (inputs.config_input("c3"), 50),
+    }
+    pz = os.path.join(HERE, "inputs", "protein.npz")
+    if os.path.exists(pz):
+        # the 24 036 atoms of the reference's own Input.txt (its first line
+        # declares n = 10; the records are a protein), as data
+        with np.load(pz, allow_pickle=False) as z:
+            hd = [int(v) for v in z["header"]]
+            large["protein"] = (inputs.DDInput(hd[0], hd[1], hd[2], hd[3], hd[4], hd[5],
+                                               float(z["eps"]), float(z["eta"]), z["charge"],
+                                               z["xyz"], z["rvdw"]), 0)
+    for name, (d, every) in large.items():
+        if not only or name not in only:
+            continue
+        deck = os.path.join(dumps, name + ".txt")
+        if not os.path.exists(deck):
+            d.nproc = os.cpu_count() or 1
+            inputs.write_input(deck, d)
+        run_case(name, deck, ["fx", "sigma", "s"] if every else ["fx"],
+                 prefix=os.path.join(dumps, name), every=max(every, 1), probe=False)
+    if not only or "c4_sample" in only:
+        batch_sample()
+    if not only or "c5_frames" in only:
+        trajectory_frames()
     for name, (d, keep) in synth.items():
         if only and name not in only:
             continue

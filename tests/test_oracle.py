@@ -266,3 +266,27 @@ def test_nonconvergence_reported(oracle_mod):
     ok = o.L.orc_jacobi_diis(o.h, rhs.size, 25, 1e-7, _p(rhs), _p(x), 3,
                              C.cast(o.L.orc_lx, C.c_void_p), C.byref(hist))
     assert ok == 0 and hist.n_iter == 4 and hist.ok == 0
+
+
+@pytest.mark.parametrize("name", ["slow", "norm1", "norm2", "norm3", "plain", "fail"])
+def test_jacobi_diis_plugin_interface_against_reference(oracle_mod, name):
+    """The oracle's jacobi_diis with callbacks (every norm choice, plain
+    Jacobi, the drop-10 rule past 25 iterations, a run out of iterations)
+    against the reference's own jacobi_diis.f90 on dense systems
+    (oracle/jd_dump.f90): bit for bit, and the printed norms."""
+    z = golden("jacobi_dense")
+    g = {k[len(name) + 1:]: z[k] for k in z if k.startswith(name + "_")}
+    n, dmax, nrm, nit = (int(v) for v in g["params"])
+    cb = oracle_mod.dense_callbacks(g["a"], g["d"])
+    x, it, ok, norms = oracle_mod.jacobi_diis_ops(n, dmax, nrm, float(g["tol"]), g["rhs"],
+                                                  g["x0"], nit, *cb)
+    assert it == int(g["n_iter"]) and ok == int(g["ok"])
+    assert np.array_equal(x, g["x"])
+    lines = [l for l in g["lines"] if "iter=" in l]
+    assert len(lines) == len(norms)
+    for line, (rms, mx) in zip(lines, norms):
+        vals = [float(t.replace("D", "E")) for t in line.split()[3 if nrm < 4 else 3:]
+                if "D" in t]
+        want = {1: [mx], 2: [rms], 3: [rms, mx], 4: [rms]}[nrm]
+        for v, w in zip(vals, want):
+            assert abs(v - w) <= 0.5001e-4 * 10.0 ** np.ceil(np.log10(abs(w))), (line, w)
