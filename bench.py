@@ -1,21 +1,33 @@
 """Energy + forces ddCOSMO solves per second on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload c2|c3|c4|c5] [--precision fp64|mixed]
+    python bench.py [--steps K] [--warmup W] [--workload c4|c2|c3|c5]
 
-One step is one complete solve from coordinates already resident in HBM:
-device setup (neighbours, fi/ui/zi, cavity, triple lists), RHS (solute
-potential at the cavity points, projection), direct Jacobi/DIIS solve and
-energy, adjoint solve, analytic forces with zeta and both field terms.  The
-workload is the 2000-atom synthetic system (lmax 8, 590 Lebedev points,
-eps 78.39, eta 0.1, convergence 1e-8); c3 is the 20000-atom one (lmax 10,
-1202 points), c4 a batch of 512 molecules of 20..800 atoms (lmax 6, 302
-points) solved on one handle, one step being the whole batch; c5 a 1000-atom
-trajectory timed under each initial-guess mode (the learned guess trained on
-the first frames).
+The default workload (c4) is the metric's own comparison on the largest
+single-GPU configuration: 512 synthetic molecules of 20..800 atoms (113 185
+spheres; lmax 6, 302 Lebedev points, eps 78.39, eta 0.1, conv 1e-8, energy +
+forces) on one batched handle, one step being the whole batch from
+coordinates already resident in HBM: device setup (neighbours, fi/ui/zi,
+cavity, triple lists), RHS (solute potential at the cavity points,
+projection), direct and adjoint Jacobi/DIIS solves with energies, analytic
+forces with zeta and both field terms.
 
-Beside the GPU figure the reference's own OpenMP Fortran program (compiled
-from its sources into oracle/_ref/main.exe) is timed on the host cores on
-one solve of the same workload.
+  value      solves/s of the mixed precision (FP32 sweeps and products on
+             the increments, the residual carried in FP64 and refreshed by
+             the FP64 operator every eighth round) with the learned initial
+             guess, its
+             network trained beforehand on other molecules of the same sizes
+             (training batches drawn with other seeds, untimed)
+  fp64       the plain FP64 path on the same handle (the reference's
+             arithmetic and iteration counts), and speedup_vs_fp64 = their
+             ratio; per-stage times and iterations of both
+  roofline   the dominant kernel of the timed (mixed) solves
+  cpu_baseline  the reference's own OpenMP Fortran program (oracle/_ref/
+             main.exe, built from its sources) on the first molecules of the
+             batch, one process each, for about 20 s
+
+c2 is the 2000-atom system (lmax 8, 590 points), c3 the 20000-atom one
+(lmax 10, 1202 points): an FP64 line with the mixed precision beside it.
+c5 is the 1000-atom trajectory timed under each initial-guess mode.
 """
 from __future__ import annotations
 
@@ -271,7 +283,7 @@ def trajectory_bench(args):
 
 def mixed_beside(h, args, sec64, es64, fx64, batch):
     """The same workload with FP32 sweeps and harmonic products inside FP64
-    iterative refinement: solves/s, its speed-up over the FP64 solves timed
+    iterations: solves/s, its speed-up over the FP64 solves timed
     just before, iterations, and the differences of the energies and forces
     from the FP64 results."""
     h.set_precision(1)
@@ -301,14 +313,155 @@ def mixed_beside(h, args, sec64, es64, fx64, batch):
     return out
 
 
+def training_batches(decks, count, seed):
+    """`count` batches of molecules with the sizes of `decks` drawn with
+    other seeds (the same generator): data the network may learn from
+    without seeing the timed molecules."""
+    from ddpcm_amd import inputs
+    out = []
+    for b in range(count):
+        out.append([inputs.synthetic_input(d.n, seed + 1000 * b + q, d.lmax, d.ngrid)
+                    for q, d in enumerate(decks)])
+    return out
+
+
+def _upload(h, decks):
+    b = [dk.bohr() for dk in decks]
+    h.upload(*[np.concatenate([v[k] for v in b]) for k in range(4)],
+             np.concatenate([dk.charge for dk in decks]))
+
+
+def _timed_leg(h, args, iters_fn, precision):
+    """Warmup, one run with every kernel class timed, then the timed steps
+    with the dominant class bracketed launch by launch on the library's
+    stream; returns the leg's record and its results."""
+    for _ in range(args.warmup):
+        h.run()
+    table = kernel_table(h, iters_fn)
+    dominant = max(table, key=lambda k: table[k]["ms"])
+    h.kernel_timer(dominant, True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        h.run()
+    t1 = time.perf_counter()
+    launches, kms = h.kernel_timer_read(dominant)
+    h.kernel_timer(dominant, False)
+    es, fx = h.result()
+    st = h.stats()
+    nd, na = h.iterations()
+    sec = (t1 - t0) / args.steps
+    work = algorithmic_work(h, st, iters_fn())
+    roofline = None
+    if dominant in work and launches > 0:
+        unit, w, per_solve = work[dominant]
+        per_launch = w / per_solve
+        avg_s = kms * 1e-3 / launches
+        if unit == "flop":
+            ach = per_launch / avg_s / 1e12
+            peak = FP64_PEAK_TFLOPS if precision == 0 else FP32_PEAK_TFLOPS
+            roofline = {"bound": "valu" if dominant.startswith("sweep") else "mfma",
+                        "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+                        "traffic": None}
+            if precision:
+                # gfx950 issues unpacked FP32 VALU at the FP64 rate; packed
+                # FP32 is kept out of this build, so 78.6 is the ceiling of
+                # the FP32 sweeps as built
+                roofline["peak_unpacked_fp32"] = FP64_PEAK_TFLOPS
+                roofline["frac_of_unpacked"] = ach / FP64_PEAK_TFLOPS
+        else:
+            ach = per_launch / avg_s / 1e9
+            roofline = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK_GBS, "traffic": None}
+        tag = "c4batch" if precision == 0 else "c4batch_mixed"
+        roofline["traffic"] = traffic_from_profiles(dominant, tag)
+        roofline.update({"kernel": dominant, "launches": launches,
+                         "avg_launch_us": avg_s * 1e6, "work_per_launch": per_launch,
+                         "work_unit": unit})
+    rec = {"value": h.count / sec, "unit": "solves/s", "ms_per_step": sec * 1e3,
+           "stages_ms": {"setup": st.ms_setup, "rhs": st.ms_rhs,
+                         "solves": st.ms_direct + st.ms_adjoint, "forces": st.ms_forces},
+           "iterations": {"direct": [int(nd.min()), int(nd.max()), float(nd.mean())],
+                          "adjoint": [int(na.min()), int(na.max()), float(na.mean())]},
+           "kernels": table,
+           "hbm_gbs": {k: table[k].get("gbs") for k in ("jacobi", "extrap")},
+           "roofline": roofline}
+    return rec, es, fx
+
+
+def batch_bench(args):
+    """c4: the metric's comparison on the 512-molecule batch (module
+    docstring)."""
+    from ddpcm_amd import inputs
+    from ddpcm_amd.ddx import DDCosmoBatch, DDX_GUESS_LEARNED, DDX_GUESS_REFERENCE
+    decks = inputs.batch_inputs()
+    d = decks[0]
+    h = DDCosmoBatch.from_inputs(decks)
+    _upload(h, decks)
+
+    def iters_fn():
+        nd, na = h.iterations()
+        return list(zip(np.diff(h.sph_off).tolist(), np.diff(h.cav_off).tolist(), nd.tolist(),
+                        na.tolist()))
+
+    # plain FP64, the reference's guess
+    h.set_precision(0)
+    h.set_guess(DDX_GUESS_REFERENCE)
+    fp64, es64, fx64 = _timed_leg(h, args, iters_fn, 0)
+    # the learned guess, trained on other molecules (untimed), then the
+    # mixed precision with it on the timed molecules
+    train_loss = []
+    h.set_guess(DDX_GUESS_LEARNED)
+    for tb in training_batches(decks, args.train_batches, 90001):
+        _upload(h, tb)
+        for _ in range(2):
+            h.run()
+            train_loss.append(h.guess_train(steps=8, lr=2e-3))
+    _upload(h, decks)
+    h.set_precision(1)
+    mixed, es, fx = _timed_leg(h, args, iters_fn, 1)
+    h.set_precision(0)
+    config = {"workload": "c4: %d synthetic molecules of %d..%d atoms (%d in all), lmax %d, "
+                          "%d Lebedev points, eps %.2f, eta %.2f, conv 1e-%d, energy + forces, "
+                          "one batched handle" % (h.count, int(np.diff(h.sph_off).min()),
+                                                  int(np.diff(h.sph_off).max()), h.nsph,
+                                                  d.lmax, h.ngrid, d.eps, d.eta, d.iconv),
+              "precision": "mixed (FP32 sweeps and harmonic products on the increments, "
+                           "FP64 residual, Jacobi/DIIS and reductions)",
+              "guess": "learned (trained on %d batches of other molecules)" % args.train_batches,
+              "nsph": h.nsph, "ncav": h.ncav, "ntriples": h.stats().ntriples,
+              "molecules": h.count}
+    out = {
+        "metric": METRIC, "value": mixed["value"], "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": mixed["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 sweeps and products, f64 residual and DIIS", "data": "synthetic",
+        "config": config,
+        "speedup_vs_fp64": mixed["value"] / fp64["value"],
+        "stages_ms": mixed["stages_ms"], "iterations": mixed["iterations"],
+        "roofline": mixed["roofline"],
+        "hbm_gbs": mixed["hbm_gbs"],
+        "errors_vs_fp64": {
+            "energy_max_rel": float(np.max(np.abs(es - es64) / np.abs(es64))),
+            "force_max_abs": float(np.max(np.abs(fx - fx64)))},
+        "guess_training_loss": train_loss[-1] if train_loss else None,
+        "kernels": mixed["kernels"],
+        "fp64": fp64,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_batch(decks, args.cpu_cores)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="c4")
     ap.add_argument("--train", type=int, default=200,
                     help="c5: frames whose solutions train the guess network")
+    ap.add_argument("--train-batches", type=int, default=6,
+                    help="c4: batches of other molecules that train the guess network")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
                     help="precision of the timed solves; an fp64 run also times the mixed "
                          "precision beside it (\"mixed_precision\" in the JSON line)")
@@ -331,6 +484,9 @@ def main():
 
     if args.workload == "c5":
         trajectory_bench(args)
+        return
+    if args.workload == "c4":
+        batch_bench(args)
         return
     from ddpcm_amd import inputs
     from ddpcm_amd.ddx import DDCosmo, DDCosmoBatch
@@ -440,7 +596,7 @@ def main():
         "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32 sweeps in f64 refinement",
+        "dtype": "f64" if args.precision == "fp64" else "f32 sweeps, f64 residual and DIIS",
         "data": "synthetic",
         "config": config,
         "stages_ms": {"setup": st.ms_setup, "rhs": st.ms_rhs,

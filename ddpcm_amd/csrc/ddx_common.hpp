@@ -141,11 +141,11 @@ struct Lane {
   const double *rhs = nullptr;
   double *x = nullptr;
   bool active = false;
-  bool f32 = false;               // operators in FP32 (inner solves of the refinement)
-  DevArray<double> rres, dcorr;   // refinement: FP64 residual and correction
-  DevArray<double> dnorm;         // refinement: per-instance norms of the correction
-  std::vector<double> joined;     // refinement: increment norms of all inner solves
-  bool use_joined = false;
+  bool f32 = false;               // the operators of this apply run in FP32
+  // mixed precision (solve_run_mixed): L x carried in FP64 in yacc, FP32
+  // operator rounds on the increment inc
+  bool incr = false;
+  DevArray<double> yacc, inc;
 };
 
 }  // namespace ddx

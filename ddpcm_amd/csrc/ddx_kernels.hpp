@@ -47,9 +47,6 @@ int apply_pair(ddx_handle *h, Lane &ld, double *yd, Lane &la, const double *xa, 
 int cf_scale(ddx_handle *h, hipStream_t s, const double *x, double *xs);
 // y = sum of the per-segment partial sums of L* S
 int seg_sum(ddx_handle *h, hipStream_t s, const double *yp, double *y, const DiisState *gate);
-// ungated FP64 operators for residuals: L xd into ld->y and L* xa into
-// la->y (either lane may be null)
-int apply_fp64(ddx_handle *h, Lane *ld, const double *xd, Lane *la, const double *xa);
 
 // k_solver.hip — Jacobi/DIIS (jacobi_diis.f90:43-438), RHS, norms
 // Solves on lanes: solve_begin sets one up (x holds the guess); solve_run
@@ -68,8 +65,9 @@ void drop_round_graphs(ddx_handle *h);
 int solve_begin(ddx_handle *h, Lane &ln, int star, int diis_max, double tol,
                 const double *d_rhs, double *d_x, int max_iter);
 int solve_run(ddx_handle *h, Lane *const *lanes, int count);
-// solve_run in the handle's precision: FP64, or the refinement of the mixed
-// precision (FP32 operators inside FP64 iterative refinement)
+// solve_run in the handle's precision: FP64, or the mixed precision (FP32
+// operator rounds on the increments, the operator image carried in FP64
+// and refreshed by the FP64 operator)
 int solve_lanes(ddx_handle *h, Lane *const *lanes, int count);
 int solve_end(ddx_handle *h, Lane &ln, int *n_iter, int *ok, std::vector<double> *hist);
 int solve_jacobi_diis(ddx_handle *h, Lane &ln, int star, int diis_max, double tol,

@@ -32,6 +32,9 @@ namespace {
 
 constexpr int kJThreads = 256;
 constexpr int kPartStride = kDiisMax + 1;   // 25 dots + self dot per block
+// mixed precision: every kRefresh-th round applies the FP64 operator to the
+// iterate itself (the FP64 image of the iterate restarts exactly there)
+constexpr int kRefresh = 8;
 
 // Pointers of one system's Jacobi/DIIS iteration; the kernels below take
 // one or two systems (the paired direct and adjoint solves), selected by
@@ -46,6 +49,12 @@ struct SysArgs {
   double *part, *part_e;     // Gram-row (26 rows of nbj) and increment-norm partial sums
   int *newslot;              // per instance
   DiisState *st;             // per instance, then the summary of a batch
+  // mixed precision: the operator's image of the iterate carried in FP64
+  double *yacc;              // L x (L* x): y adds to it (acc = 1) or replaces it
+  int acc;                   //   (acc = 0, a round with the FP64 operator on x)
+  double *inc;               // the last increment x_new - x (null: not kept)
+  int xs_inc;                // xs = cf inc instead of cf x (the next round applies
+                             //   the FP32 operator to the increment)
 };
 struct Systems {
   SysArgs s[2];
@@ -92,6 +101,10 @@ __global__ __launch_bounds__(kJThreads) void k_jacobi(size_t nn, int nylm, int d
     } else {
       yk = y[k];
       for (int j = 1; j < ny; ++j) yk += y[(size_t)j * nn + k];
+    }
+    if (q0.yacc) {
+      if (q0.acc) yk += q0.yacc[k];
+      q0.yacc[k] = yk;
     }
     double xn = facl[k % nylm] * (rhs[k] - yk);
     double e = xn - x[k];
@@ -476,7 +489,8 @@ __global__ __launch_bounds__(kJThreads) void k_extrap(int nylm, size_t nn, int d
       }
       double inc = xn - x[k];
       x[k] = xn;
-      if (xsc) xsc[k] = cf[lm] * xn;
+      if (xsc) xsc[k] = cf[lm] * (q0.xs_inc ? inc : xn);
+      if (q0.inc) q0.inc[k] = inc;
       s += invl1[lm] * inc * inc;
     }
     s = wave_sum(s);
@@ -533,53 +547,6 @@ __global__ void k_diis_init(int ninst, double tol, DiisState *__restrict__ st) {
     d->norm = 0.0;
     d->tol = tol;
   }
-}
-
-// Norms of the refinement, per instance: sqrt(sum over its spheres of
-// sum_lm x^2 / (l + 1), divided by the sphere count), the norm of the
-// convergence test (rmsvec, jacobi_diis.f90:405-438; the per-sphere square
-// root and square cancel).  Partial sums over the Jacobi block map (every
-// block inside one instance), then one wave per instance.
-__global__ __launch_bounds__(kJThreads) void k_sq_part(const int4 *__restrict__ jblk, int nylm,
-                                                       const double *__restrict__ invl1,
-                                                       const double *__restrict__ x,
-                                                       double *__restrict__ xadd,
-                                                       double *__restrict__ part) {
-  __shared__ double lds[kJThreads / kWave];
-  const int4 blk = jblk[blockIdx.x];
-  double s = 0.0;
-  for (int k = blk.y + threadIdx.x; k < blk.z; k += kJThreads) {
-    const double v = x[k];
-    s += invl1[k % nylm] * v * v;
-    if (xadd) xadd[k] += v;   // refinement: x += d
-  }
-  const double tot = block_sum_256(s, lds);
-  if (threadIdx.x == 0) part[blockIdx.x] = tot;
-}
-
-// mode 0: st[q].tol = max(st[q].tol, rel norm_q); mode 1: out[q] = norm_q
-__global__ __launch_bounds__(kWave) void k_inst_norm(const int *__restrict__ jboff,
-                                                     const int *__restrict__ soff,
-                                                     const double *__restrict__ part, double rel,
-                                                     DiisState *__restrict__ st,
-                                                     double *__restrict__ out) {
-  const int q = blockIdx.x;
-  double s = 0.0;
-  for (int b = jboff[q] + threadIdx.x; b < jboff[q + 1]; b += kWave) s += part[b];
-  s = wave_sum(s);
-  if (threadIdx.x == 0) {
-    const double nrm = sqrt(s / (double)max(1, soff[q + 1] - soff[q]));
-    if (st) st[q].tol = fmax(st[q].tol, rel * nrm);
-    else out[q] = nrm;
-  }
-}
-
-// refinement residual r = g - D x - y, y = L x (or L* x); D = 1 / facl
-__global__ void k_residual(size_t nn, int nylm, const double *__restrict__ facl,
-                           const double *__restrict__ g, const double *__restrict__ x,
-                           const double *__restrict__ y, double *__restrict__ r) {
-  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < nn) r[k] = (g[k] - x[k] / facl[k % nylm]) - y[k];
 }
 
 // per instance: iterations as the reference reports them and the ok flag
@@ -781,6 +748,10 @@ SysArgs sys_args(const ddx_handle *h, Lane &ln) {
   a.part_e = ln.part.p + (size_t)h->nbj * kPartStride;
   a.newslot = ln.misc.p;
   a.st = ln.diis.p;
+  a.yacc = ln.incr ? ln.yacc.p : nullptr;
+  a.acc = 0;
+  a.inc = ln.incr ? ln.inc.p : nullptr;
+  a.xs_inc = 0;
   return a;
 }
 
@@ -862,13 +833,29 @@ int enqueue_rounds(ddx_handle *h, Lane *const *lanes, int count, int nr) {
   for (int k = 0; k < count; ++k) sys.s[k] = sys_args(h, *lanes[k]);
   for (int g = 0; g < nr; ++g, ++l0.enqueued) {
     h->tag_round = l0.enqueued;
-    if (pair) {
-      DDX_TRY(apply_pair(h, l0, l0.y.p, *lanes[1], lanes[1]->x, h->ysplit));
-    } else if (l0.star) {
-      DDX_TRY(apply_lstarx(h, l0, l0.x, l0.y.p, solve_gate(h, l0), false));
-    } else {
-      DDX_TRY(apply_lx(h, l0, l0.x, l0.y.p, solve_gate(h, l0), true, h->ysplit));
+    // mixed precision: rounds 0, kRefresh, 2 kRefresh, ... apply the FP64
+    // operator to the iterate itself; the others the FP32 operator to the
+    // last increment, added to the FP64 image
+    const bool incr = l0.incr;
+    const bool exact = !incr || l0.enqueued % kRefresh == 0;
+    const bool next_exact = !incr || (l0.enqueued + 1) % kRefresh == 0;
+    for (int k = 0; k < count; ++k) {
+      lanes[k]->f32 = !exact;
+      sys.s[k].acc = exact ? 0 : 1;
+      sys.s[k].xs_inc = next_exact ? 0 : 1;
     }
+    const double *xa = pair ? (exact ? lanes[1]->x : lanes[1]->inc.p) : nullptr;
+    const double *x1 = exact ? l0.x : l0.inc.p;
+    int rc;
+    if (pair) {
+      rc = apply_pair(h, l0, l0.y.p, *lanes[1], xa, h->ysplit);
+    } else if (l0.star) {
+      rc = apply_lstarx(h, l0, x1, l0.y.p, solve_gate(h, l0), false);
+    } else {
+      rc = apply_lx(h, l0, x1, l0.y.p, solve_gate(h, l0), true, h->ysplit);
+    }
+    for (int k = 0; k < count; ++k) lanes[k]->f32 = false;
+    DDX_TRY(rc);
     {
       TimedLaunch tl(h, DDX_K_JACOBI, s);
       hipExtLaunchKernelGGL(k_jacobi, dim3(nbj, count), dim3(kJThreads), 0, s, tl.b, tl.e, 0, nn,
@@ -904,7 +891,9 @@ std::vector<uintptr_t> round_key(const ddx_handle *h, Lane *const *lanes, int co
                               (uintptr_t)h->nbj, (uintptr_t)h->nbe, (uintptr_t)h->nworker};
   for (int c = 0; c < count; ++c) {
     const Lane &l = *lanes[c];
-    const uintptr_t v[] = {(uintptr_t)l.star, (uintptr_t)l.f32, (uintptr_t)l.diis_max,
+    const uintptr_t v[] = {(uintptr_t)l.star, (uintptr_t)l.incr,
+                           (uintptr_t)(l.incr ? lanes[0]->enqueued % kRefresh : 0),
+                           (uintptr_t)l.yacc.p, (uintptr_t)l.inc.p, (uintptr_t)l.diis_max,
                            (uintptr_t)l.max_iter,
                            (uintptr_t)l.rhs, (uintptr_t)l.x, (uintptr_t)l.y.p, (uintptr_t)l.xnew.p,
                            (uintptr_t)l.xs.p, (uintptr_t)l.P.p, (uintptr_t)l.xiw.p,
@@ -999,11 +988,9 @@ int solve_end(ddx_handle *h, Lane &ln, int *n_iter, int *ok, std::vector<double>
   *n_iter = hs->ok ? hs->it : ln.max_iter + 1;
   *ok = hs->ok;
   if (hist) {
-    if (h->ninst > 1) hist->clear();
-    else if (ln.use_joined) *hist = ln.joined;
-    else hist->assign(hs->hist, hs->hist + (hs->it < kMaxIter ? hs->it : kMaxIter));
+    if (h->ninst == 1) hist->assign(hs->hist, hs->hist + (hs->it < kMaxIter ? hs->it : kMaxIter));
+    else hist->clear();
   }
-  ln.use_joined = false;
   return DDX_OK;
 }
 
@@ -1060,154 +1047,28 @@ int solve_run(ddx_handle *h, Lane *const *lanes, int count) {
   return DDX_OK;
 }
 
-namespace {
-// Inner solves of the refinement stop at max(tol, kRelTol |start|): the
-// increments of FP32 operators cannot fall much below 1e-7 of the iterate.
-constexpr double kRelTol = 1e-5;
-// The refinement stops once every instance's correction is below tol, the
-// reference's own test on the increment (jacobi_diis.f90:153-156); the
-// solve of that last correction starts below its threshold and takes one
-// or two iterations.
-constexpr double kStop = 1.0;
-constexpr int kMaxRefine = 6;
-// The corrections are solved to kFinal tol, so that the error left in x
-// (about the last threshold times rho / (1 - rho), rho the Jacobi/DIIS
-// contraction) stays below the reference's own.
-constexpr double kFinal = 0.2;
-
-// iterations and ok of each instance in the solve that just ended on ln;
-// for a single system its increment norms join the lane's refinement history
-void add_iterations(const ddx_handle *h, Lane &ln, std::vector<int> &its,
-                    std::vector<int> &ok) {
-  if (h->ninst == 1) {
-    const DiisState *hs = ln.h_diis;
-    const int c = std::min(hs->it, kMaxIter);
-    ln.joined.insert(ln.joined.end(), hs->hist, hs->hist + c);
-  }
-  for (int q = 0; q < h->ninst; ++q) {
-    const bool oq = h->ninst > 1 ? ln.h_res[2 * q + 1] != 0 : ln.h_diis->ok != 0;
-    const int iq = h->ninst > 1 ? (oq ? ln.h_res[2 * q] : ln.max_iter) : ln.h_diis->it;
-    its[q] += iq;
-    ok[q] = ok[q] && oq;
-  }
-}
-
-// per-instance norms of x into st[q].tol (raised to rel |x|_q) or out[q];
-// with xadd, xadd += x on the way
-int inst_norms(ddx_handle *h, Lane &ln, const double *x, double *xadd, double rel,
-               DiisState *st, double *out) {
-  hipLaunchKernelGGL(k_sq_part, dim3(h->nbj), dim3(kJThreads), 0, ln.stream, h->jblk.p, h->nylm,
-                     h->cf.p + h->nylm, x, xadd, ln.part.p);
-  DDX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_inst_norm, dim3(h->ninst), dim3(kWave), 0, ln.stream, h->jboff.p,
-                     h->soff.p, ln.part.p, rel, st, out);
-  DDX_HIP(hipGetLastError());
-  return DDX_OK;
-}
-
-// runs the inner solve on the lanes with their FP32 operators
-int run_f32(ddx_handle *h, Lane *const *lanes, int count) {
-  for (int k = 0; k < count; ++k) lanes[k]->f32 = true;
-  const int rc = solve_run(h, lanes, count);
-  for (int k = 0; k < count; ++k) lanes[k]->f32 = false;
-  return rc;
-}
-}  // namespace
-
-// Mixed precision: iterative refinement in FP64 around Jacobi/DIIS solves
-// whose sweeps and harmonic products run in FP32.  The lanes have been
-// begun (solve_begin) with the guess in x.  First the FP32 operator is
-// solved for x itself; then, until the corrections are small enough, the
-// FP64 residual r = g - (D + L) x is formed and (D + L) d = r is solved with
-// the FP32 operator, from D^-1 r, and x += d.  The Jacobi/DIIS kernels
-// (update, Gram row, bordered solve, extrapolation, norms) stay FP64.
-// Reported iterations are the inner iterations of all stages.
+// Mixed precision: one Jacobi/DIIS iteration whose operator image L x
+// (L* x) is carried in FP64 and refreshed exactly.  Round 0 and every
+// kRefresh-th round apply the FP64 operator to the iterate; the rounds in
+// between apply the FP32 sweeps and products to the last increment
+// x_new - x and add the result to the FP64 image.  The Jacobi update, the
+// residual rhs - L x, the DIIS Gram row and bordered solve, the norms and
+// the convergence test are the FP64 path's (jacobi_diis.f90:102-183): the
+// FP32 rounding enters only through increments, which shrink with the
+// iteration, so the error carried in the image stays near 1e-7 of the
+// increments since the last refresh and the iteration counts stay those of
+// FP64 to within one or two.
 int solve_run_mixed(ddx_handle *h, Lane *const *lanes, int count) {
   DDX_TRY(make_tables32(h));
-  const int ninst = h->ninst;
   const size_t nn = (size_t)h->nylm * h->n;
-  const double *keep_rhs[2];
-  double *keep_x[2];
-  double keep_tol[2];
-  std::vector<int> its[2], ok[2];
   for (int k = 0; k < count; ++k) {
-    Lane &ln = *lanes[k];
-    keep_rhs[k] = ln.rhs;
-    keep_x[k] = ln.x;
-    keep_tol[k] = ln.tol;
-    its[k].assign(ninst, 0);
-    ok[k].assign(ninst, 1);
-    DDX_TRY(ln.rres.reserve(nn + 1));
-    DDX_TRY(ln.dcorr.reserve(nn + 1));
-    DDX_TRY(ln.dnorm.reserve(ninst + 1));
-    DDX_TRY(ln.part.reserve((size_t)h->nbj * kPartStride + h->nbe + 1));
-    DDX_TRY(inst_norms(h, ln, ln.x, nullptr, kRelTol, ln.diis.p, nullptr));
-    ln.joined.clear();
+    DDX_TRY(lanes[k]->yacc.reserve(nn + 1));
+    DDX_TRY(lanes[k]->inc.reserve(nn + 1));
+    lanes[k]->incr = true;
   }
-  DDX_TRY(run_f32(h, lanes, count));
-  bool all_ok = true;
-  for (int k = 0; k < count; ++k) {
-    add_iterations(h, *lanes[k], its[k], ok[k]);
-    for (int q = 0; q < ninst; ++q) all_ok = all_ok && ok[k][q];
-  }
-  std::vector<double> dn(ninst);
-  for (int pass = 0; all_ok && pass < kMaxRefine; ++pass) {
-    // FP64 residuals of the current x
-    Lane *ld = nullptr, *la = nullptr;
-    for (int k = 0; k < count; ++k) (lanes[k]->star ? la : ld) = lanes[k];
-    h->mute_timers = true;
-    const int rc = apply_fp64(h, ld, ld ? keep_x[ld == lanes[0] ? 0 : 1] : nullptr, la,
-                              la ? keep_x[la == lanes[0] ? 0 : 1] : nullptr);
-    h->mute_timers = false;
-    DDX_TRY(rc);
-    for (int k = 0; k < count; ++k) {
-      Lane &ln = *lanes[k];
-      hipLaunchKernelGGL(k_residual, dim3((nn + 255) / 256), dim3(256), 0, ln.stream, nn,
-                         h->nylm, h->facl.p, keep_rhs[k], keep_x[k], ln.y.p, ln.rres.p);
-      DDX_HIP(hipGetLastError());
-      DDX_TRY(device_ldm1x(h, ln.stream, ln.rres.p, ln.dcorr.p));
-      DDX_TRY(solve_begin(h, ln, ln.star, ln.diis_max, kFinal * keep_tol[k], ln.rres.p,
-                          ln.dcorr.p, ln.max_iter));
-      DDX_TRY(inst_norms(h, ln, ln.dcorr.p, nullptr, kRelTol, ln.diis.p, nullptr));
-    }
-    DDX_TRY(run_f32(h, lanes, count));
-    bool small = true;
-    for (int k = 0; k < count; ++k) {
-      Lane &ln = *lanes[k];
-      add_iterations(h, ln, its[k], ok[k]);
-      DDX_TRY(inst_norms(h, ln, ln.dcorr.p, keep_x[k], 0.0, nullptr, ln.dnorm.p));
-      DDX_HIP(hipMemcpyAsync(dn.data(), ln.dnorm.p, sizeof(double) * ninst,
-                             hipMemcpyDeviceToHost, ln.stream));
-      DDX_HIP(hipStreamSynchronize(ln.stream));
-      for (int q = 0; q < ninst; ++q) {
-        all_ok = all_ok && ok[k][q];
-        small = small && dn[q] <= kStop * keep_tol[k];
-      }
-    }
-    if (small) break;
-    if (pass + 1 == kMaxRefine) all_ok = false;
-  }
-  // the lanes report the whole refinement
-  for (int k = 0; k < count; ++k) {
-    Lane &ln = *lanes[k];
-    ln.rhs = keep_rhs[k];
-    ln.x = keep_x[k];
-    ln.tol = keep_tol[k];
-    int itmax = 0;
-    for (int q = 0; q < ninst; ++q) {
-      const bool oq = all_ok && ok[k][q];
-      itmax = std::max(itmax, its[k][q]);
-      if (ninst > 1) {
-        ln.h_res[2 * q] = oq ? its[k][q] : ln.max_iter + 1;
-        ln.h_res[2 * q + 1] = oq;
-      }
-    }
-    ln.h_diis->it = itmax;
-    ln.h_diis->ok = all_ok ? 1 : 0;
-    ln.use_joined = true;
-    if (!ln.star) DDX_TRY(cf_scale(h, ln.stream, ln.x, ln.xs.p));
-  }
-  return DDX_OK;
+  const int rc = solve_run(h, lanes, count);
+  for (int k = 0; k < count; ++k) lanes[k]->incr = false;
+  return rc;
 }
 
 int solve_lanes(ddx_handle *h, Lane *const *lanes, int count) {

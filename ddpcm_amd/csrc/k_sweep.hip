@@ -484,7 +484,7 @@ int sweep_lmax(ddx_handle *h, hipStream_t s, int mode, const double *Xs, R *rw, 
 }
 
 // mode: 1 = L X terms, 2 = L* S shares per segment, 3 = both.  R: the
-// arithmetic of the sweep (double, or float inside the refinement of the
+// arithmetic of the sweep (double, or float in the increment rounds of the
 // mixed precision); X' and the L* S shares stay double.
 template <typename R>
 int sweep(ddx_handle *h, hipStream_t s, int mode, const double *Xs, R *rw, const R *xiw,
@@ -615,26 +615,6 @@ int apply_pair(ddx_handle *h, Lane &ld, double *yd, Lane &la, const double *xa, 
   const DiisState *gd = solve_gate(h, ld), *ga = solve_gate(h, la);
   return ld.f32 ? pair_r<float>(h, ld, yd, gd, la, xa, ga, ysplit)
                 : pair_r<double>(h, ld, yd, gd, la, xa, ga, ysplit);
-}
-
-// FP64 operators of the lanes' current iterates for the residuals of the
-// refinement, ungated: L x into ld.y (x' = cf x rebuilt in ld.xs) and
-// L* s into la.y, in one sweep when both are asked for.
-int apply_fp64(ddx_handle *h, Lane *ld, const double *xd, Lane *la, const double *xa) {
-  if (ld) {
-    DDX_TRY(reserve_lx(h, *ld));
-    DDX_TRY(cf_scale(h, ld->stream, xd, ld->xs.p));
-  }
-  if (la) DDX_TRY(reserve_lstarx(h, *la));
-  if (ld && la) {
-    DDX_TRY(pair_r<double>(h, *ld, ld->y.p, nullptr, *la, xa, nullptr, 1));
-  } else if (ld) {
-    DDX_TRY(lx_r<double>(h, *ld, xd, ld->y.p, nullptr, true, 1));
-  } else if (la) {
-    DDX_TRY(lstarx_r<double>(h, *la, xa, nullptr, nullptr, false));
-  }
-  if (la) DDX_TRY(seg_sum(h, la->stream, la->yp.p, la->y.p, nullptr));
-  return DDX_OK;
 }
 
 }  // namespace ddx

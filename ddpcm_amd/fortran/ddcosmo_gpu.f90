@@ -32,6 +32,7 @@ module ddcosmo
   logical :: grad
 
   type(c_ptr), save, private :: handle = c_null_ptr
+  logical, save, private :: mixed = .false.
 
 contains
 
@@ -61,10 +62,10 @@ contains
                   int(ngrid, c_int), int(iconv, c_int), int(igrad, c_int), eps, eta)
     call ddx_check(rc, 'ddinit')
     ! DDX_PRECISION=mixed in the environment selects the mixed precision
-    ! (FP32 sweeps inside FP64 refinement); the default is the reference's FP64
+    ! (FP32 operator rounds, FP64 residual); the default is the reference's FP64
     call get_environment_variable('DDX_PRECISION', prec_env)
-    if (trim(prec_env) == 'mixed') &
-      call ddx_check(ddx_set_precision(handle, DDX_PREC_MIXED), 'ddinit')
+    mixed = trim(prec_env) == 'mixed'
+    if (mixed) call ddx_check(ddx_set_precision(handle, DDX_PREC_MIXED), 'ddinit')
     rc = ddx_get_sizes(handle, ns, nc, ny, ng, lm)
     nsph = ns ; ncav = nc ; nylm = ny ; ngrid = ng
     grad = igrad /= 0
@@ -374,6 +375,8 @@ contains
       write(iout, '(a,8x,d8.1)') '   convergence threshold:                  ', 10.0d0**(-iconv)
       write(iout, '(a,8x,f8.3)') '   regularization parameter (eta):         ', eta
       write(iout, '(a,8x,f8.4,/)') '   dielectric constant:                    ', eps
+      if (mixed) write(iout, '(a,/)') &
+        ' Mixed precision (DDX_PRECISION=mixed): FP32 operator rounds, FP64 residual.'
       if (igrad == 1) write(iout, '(a,//)') ' Compute forces.'
     end if
   end subroutine header

@@ -31,7 +31,7 @@ extern "C" {
 
 /* precision of the solve (ddx_set_precision) */
 #define DDX_PREC_FP64 0      /* FP64 throughout: the reference's arithmetic      */
-#define DDX_PREC_MIXED 1     /* FP32 sweeps inside FP64 iterative refinement     */
+#define DDX_PREC_MIXED 1     /* FP32 operator rounds, FP64-carried residual      */
 
 typedef struct ddx_handle ddx_handle;
 
@@ -163,9 +163,8 @@ int ddx_jacobi_diis(ddx_handle *h, int star, int diis_max, double tol,
                     const double *rhs, double *x, int *n_iter, int *ok);
 
 /* residual history of the last solve: star 0 / 1 the last direct /
- * adjoint solve (under mixed precision, the increment norms of all its
- * inner solves in order), 2 the last ddx_jacobi_diis_ops call; count = all
- * norms, of which up to cap are written */
+ * adjoint solve, 2 the last ddx_jacobi_diis_ops call; count = all norms,
+ * of which up to cap are written */
 int ddx_get_history(const ddx_handle *h, int star, int cap, double *norms,
                     int *count);
 
@@ -215,14 +214,16 @@ int ddx_batch_sizes(const ddx_handle *h, int *count, int *sph_off, int *cav_off)
 
 /* Precision of the solves (no counterpart in the reference, which is FP64
  * throughout).  DDX_PREC_MIXED runs the L X and L* S sweeps and the
- * harmonic products (projection, expansion) in FP32 with FP32 accumulation,
- * inside FP64 iterative refinement: the residual g - (D + L) x, the
- * Jacobi/DIIS update, Gram row and bordered solve, the energies and the
- * forces stay FP64.  Results agree with FP64 to well inside the
- * convergence threshold; iteration counts are the sums over the inner
- * solves (they may exceed the 200 of one solve) and differ from the
- * reference's.  ddx_lx / ddx_lstarx apply the FP32
- * operators while it is set; ddx_jacobi_diis always runs FP64. */
+ * harmonic products (projection, expansion) in FP32 with FP32 accumulation
+ * inside an FP64 iteration: the operator's image L x of the iterate is
+ * carried in FP64, the FP32 operator is applied to each increment x_new - x
+ * and added to it, and every eighth round (the first included) applies the
+ * FP64 operator to the iterate itself.  The residual g - (D + L) x, the
+ * Jacobi/DIIS update, Gram row and bordered solve, the convergence test,
+ * the energies and the forces stay FP64.  Results are as close to the exact
+ * solution as the reference's own; iteration counts follow the FP64 path's
+ * to within one or two.  ddx_lx / ddx_lstarx apply the FP32 operators while
+ * it is set; ddx_jacobi_diis always runs FP64. */
 int ddx_set_precision(ddx_handle *h, int prec);
 
 /* Initial guess of the solves (no counterpart in the reference beyond

@@ -187,6 +187,9 @@ def _dump_ef(name, d, iconv=None):
     counts.append(cur)
     for ext in (".hdr", ".bin"):
         os.remove(prefix + ext)
+    # the scale of the energy's terms: (eps-1)/(2 eps) sum |sigma psi|
+    scale = 0.5 * (d.eps - 1.0) / d.eps * float(np.sum(np.abs(dd["sigma"] * dd["psi"])))
+    _dump_ef.scale = scale
     return hdr["esolv"], dd["fx"], counts, hdr["ncav"]
 
 
@@ -195,12 +198,13 @@ def batch_sample(every=32):
     decks = inputs.batch_inputs()
     qs = list(range(0, len(decks), every))
     rec = {"q": np.array(qs), "esolv": [], "niter_direct": [], "niter_adjoint": [],
-           "nsph": [], "ncav": [], "esolv_exact": []}
+           "nsph": [], "ncav": [], "esolv_exact": [], "escale": []}
     fx, fxx = [], []
     for q in qs:
         e, f, c, nc = _dump_ef("c4_%d" % q, decks[q])
         ex, fe, _, _ = _dump_ef("c4x_%d" % q, decks[q], iconv=13)
         rec["esolv_exact"].append(ex)
+        rec["escale"].append(_dump_ef.scale)
         fxx.append(fe)
         rec["esolv"].append(e)
         rec["niter_direct"].append(c[0])
