@@ -106,6 +106,7 @@ struct GuessWork {
   DevArray<unsigned short> F, H1, H2, dYb, dZb;
   DevArray<float> Z1, Z2, Y, T, dY, dH, dZ;
   DevArray<double> scale, part;
+  DevArray<float> ksplit;   // split-K partials of the weight gradients
 };
 
 // Event pairs around the launches of one kernel class (ddx_kernel_timer).
@@ -206,6 +207,9 @@ struct ddx_handle {
   int ncu = 0;                      // compute units of the device
   ddx::DevArray<double4> tgeo;    // (cos theta, sin theta, cos phi, sin phi)
   ddx::DevArray<double2> ttw;     // (t, W)
+  ddx::DevArray<float4> tgeo32;   // float copies for the FP32 sweeps
+  ddx::DevArray<float2> ttw32;
+  bool have_geo32 = false;        // the copies match tgeo / ttw
   ddx::DevArray<int> tp;          // owner point index i*ngrid + n
   ddx::DevArray<int> pttri;       // per point: its triples in neighbour order
   ddx::DevArray<int> tslot;       // per triple: its slot in pttri

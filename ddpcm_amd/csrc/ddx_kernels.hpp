@@ -115,8 +115,10 @@ int field_forces(ddx_handle *h, const double *d_charge, const double *d_zeta,
 size_t field_buffer_size(const ddx_handle *h);
 
 // k_guess.hip — the learned initial guess
+// (ws: room for split-K partials of products with few tiles and a long K)
 int gemm_bf16(hipStream_t s, int M, int N, int K, const unsigned short *a, long ai, long ak,
-              const unsigned short *b, long bk, long bj, float *c, long ldc, bool accumulate);
+              const unsigned short *b, long bk, long bj, float *c, long ldc, bool accumulate,
+              float *ws = nullptr, size_t ws_floats = 0);
 int convert_bf16(hipStream_t s, long n, const float *x, unsigned short *y);
 int learned_guess(ddx_handle *h, int star, const double *v, double *x, bool delta);
 int train_guess(ddx_handle *h, int star, const double *v, const double *x, int steps, double lr,

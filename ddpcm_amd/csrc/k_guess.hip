@@ -47,21 +47,25 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K,
                                                    const unsigned short *__restrict__ a, long ai,
                                                    long ak, const unsigned short *__restrict__ b,
                                                    long bk, long bj, float *__restrict__ c,
-                                                   long ldc, int accumulate) {
+                                                   long ldc, int accumulate, int kchunk) {
   const int wave = threadIdx.x / kWave, lane = lane_id();
   const int tiles_n = (N + 15) / 16;
   const long tile = (long)blockIdx.x * 4 + wave;
   const int ti = (int)(tile / tiles_n), tj = (int)(tile % tiles_n);
   if ((long)ti * 16 >= M) return;
   const int i = ti * 16 + (lane & 15), j = tj * 16 + (lane & 15), kq = (lane >> 4) * 4;
+  // split K: this workgroup's range (kchunk a multiple of 16), partials of
+  // split y at c + y M ldc
+  const int kb = (int)blockIdx.y * kchunk, ke = min(K, kb + kchunk);
+  if (gridDim.y > 1) c += (long)blockIdx.y * M * ldc;
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 16) {
+  for (int k0 = kb; k0 < ke; k0 += 16) {
     bf16x4 av, bv;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int k = k0 + kq + e;
-      av[e] = (i < M && k < K) ? (short)a[(long)i * ai + (long)k * ak] : (short)0;
-      bv[e] = (j < N && k < K) ? (short)b[(long)k * bk + (long)j * bj] : (short)0;
+      av[e] = (i < M && k < ke) ? (short)a[(long)i * ai + (long)k * ak] : (short)0;
+      bv[e] = (j < N && k < ke) ? (short)b[(long)k * bk + (long)j * bj] : (short)0;
     }
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
   }
@@ -70,7 +74,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K,
     const int r = ti * 16 + (lane >> 4) * 4 + e;
     if (r < M && j < N) {
       float *o = c + (long)r * ldc + j;
-      *o = accumulate ? *o + acc[e] : acc[e];
+      *o = (accumulate && gridDim.y == 1) ? *o + acc[e] : acc[e];
     }
   }
 }
@@ -119,13 +123,34 @@ __global__ __launch_bounds__(256) void k_mse(long M, int nout, int ld, const flo
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// db(j) = sum_i dZ(i, j), one thread per column, fixed order
-__global__ void k_col_sum(long M, int N, const float *__restrict__ dz, float *__restrict__ db) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= N) return;
+// db(j) = sum_i dZ(i, j): one workgroup per column, thread t summing rows
+// t, t + 256, ..., then a fixed-order tree over the threads
+__global__ __launch_bounds__(256) void k_col_sum(long M, int N, const float *__restrict__ dz,
+                                                 float *__restrict__ db) {
+  __shared__ float red[256];
+  const int j = blockIdx.x;
   float s = 0.f;
-  for (long i = 0; i < M; ++i) s += dz[i * N + j];
-  db[j] = s;
+  for (long i = threadIdx.x; i < M; i += 256) s += dz[i * N + j];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[j] = red[0];
+}
+
+// C = sum of the nsplit partial products of a split-K product, in split
+// order (part: nsplit blocks of M x ldc)
+__global__ void k_splitk_sum(long M, int N, int nsplit, const float *__restrict__ part,
+                             float *__restrict__ c, long ldc, int accumulate) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= M * N) return;
+  const long r = k / N, j = k % N;
+  float s = 0.f;
+  for (int q = 0; q < nsplit; ++q) s += part[(long)q * M * ldc + r * ldc + j];
+  float *o = c + r * ldc + j;
+  *o = accumulate ? *o + s : s;
 }
 
 // Adam on FP32 master weights, BF16 copy refreshed
@@ -294,12 +319,29 @@ int grid_of(long n, int t = 256) { return (int)((n + t - 1) / t); }
 }  // namespace
 
 int gemm_bf16(hipStream_t s, int M, int N, int K, const unsigned short *a, long ai, long ak,
-              const unsigned short *b, long bk, long bj, float *c, long ldc, bool accumulate) {
+              const unsigned short *b, long bk, long bj, float *c, long ldc, bool accumulate,
+              float *ws, size_t ws_floats) {
   if (M <= 0 || N <= 0) return DDX_OK;
   const long tiles = (long)((M + 15) / 16) * ((N + 15) / 16);
-  hipLaunchKernelGGL(k_gemm_bf16, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s, M, N, K, a,
-                     ai, ak, b, bk, bj, c, ldc, accumulate ? 1 : 0);
+  // few output tiles over a long K (the weight gradients, K = the spheres):
+  // split K so that about 2048 waves share it, partials summed in order
+  int nsplit = 1, kchunk = ((K + 15) / 16) * 16;
+  if (ws && tiles < 2048 && K >= 4096) {
+    nsplit = (int)std::min<long>((2048 + tiles - 1) / tiles, (K + 1023) / 1024);
+    nsplit = (int)std::min<long>(nsplit, (long)(ws_floats / ((size_t)M * ldc)));
+    if (nsplit > 1) kchunk = ((K + nsplit - 1) / nsplit + 15) / 16 * 16;
+    nsplit = std::max(1, (K + kchunk - 1) / kchunk);
+  }
+  hipLaunchKernelGGL(k_gemm_bf16, dim3((unsigned)((tiles + 3) / 4), nsplit), dim3(256), 0, s, M,
+                     N, K, a, ai, ak, b, bk, bj, nsplit > 1 ? ws : c, ldc, accumulate ? 1 : 0,
+                     kchunk);
   DDX_HIP(hipGetLastError());
+  if (nsplit > 1) {
+    const long mn = (long)M * N;
+    hipLaunchKernelGGL(k_splitk_sum, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, s, (long)M,
+                       N, nsplit, ws, c, ldc, accumulate ? 1 : 0);
+    DDX_HIP(hipGetLastError());
+  }
   return DDX_OK;
 }
 
@@ -370,6 +412,7 @@ int work_reserve(GuessWork &w, const GuessNet &g, long n) {
   DDX_TRY(w.dYb.reserve((size_t)n * g.outp + 1));
   DDX_TRY(w.scale.reserve(2));
   DDX_TRY(w.part.reserve((size_t)grid_of(n * g.outp) + 257));
+  DDX_TRY(w.ksplit.reserve((size_t)64 * kH * kH));   // up to 64 split-K partials
   return DDX_OK;
 }
 
@@ -398,22 +441,23 @@ int net_backward(hipStream_t s, GuessNet &g, GuessWork &w, long n, int *nblocks)
                      w.dYb.p, w.part.p);
   // layer 3: dW3 = H2^T dY, db3, dH2 = dY W3^T
   DDX_TRY(gemm_bf16(s, kH, g.outp, (int)n, w.H2.p, 1, kH, w.dYb.p, g.outp, 1, g.gw[2].p, g.outp,
-                    false));
-  hipLaunchKernelGGL(k_col_sum, dim3(grid_of(g.outp)), dim3(256), 0, s, n, g.outp, w.dY.p,
-                     g.gb[2].p);
+                    false, w.ksplit.p, w.ksplit.cap));
+  hipLaunchKernelGGL(k_col_sum, dim3(g.outp), dim3(256), 0, s, n, g.outp, w.dY.p, g.gb[2].p);
   DDX_TRY(gemm_bf16(s, (int)n, kH, g.outp, w.dYb.p, g.outp, 1, g.wb[2].p, 1, g.outp, w.dH.p, kH,
                     false));
   // layer 2
   hipLaunchKernelGGL(k_act_grad, dim3(grid_of(n * kH)), dim3(256), 0, s, n, kH, w.dH.p, w.Z2.p,
                      w.dZ.p, w.dZb.p);
-  DDX_TRY(gemm_bf16(s, kH, kH, (int)n, w.H1.p, 1, kH, w.dZb.p, kH, 1, g.gw[1].p, kH, false));
-  hipLaunchKernelGGL(k_col_sum, dim3(1), dim3(256), 0, s, n, kH, w.dZ.p, g.gb[1].p);
+  DDX_TRY(gemm_bf16(s, kH, kH, (int)n, w.H1.p, 1, kH, w.dZb.p, kH, 1, g.gw[1].p, kH, false,
+                    w.ksplit.p, w.ksplit.cap));
+  hipLaunchKernelGGL(k_col_sum, dim3(kH), dim3(256), 0, s, n, kH, w.dZ.p, g.gb[1].p);
   DDX_TRY(gemm_bf16(s, (int)n, kH, kH, w.dZb.p, kH, 1, g.wb[1].p, 1, kH, w.dH.p, kH, false));
   // layer 1
   hipLaunchKernelGGL(k_act_grad, dim3(grid_of(n * kH)), dim3(256), 0, s, n, kH, w.dH.p, w.Z1.p,
                      w.dZ.p, w.dZb.p);
-  DDX_TRY(gemm_bf16(s, g.inp, kH, (int)n, w.F.p, 1, g.inp, w.dZb.p, kH, 1, g.gw[0].p, kH, false));
-  hipLaunchKernelGGL(k_col_sum, dim3(1), dim3(256), 0, s, n, kH, w.dZ.p, g.gb[0].p);
+  DDX_TRY(gemm_bf16(s, g.inp, kH, (int)n, w.F.p, 1, g.inp, w.dZb.p, kH, 1, g.gw[0].p, kH, false,
+                    w.ksplit.p, w.ksplit.cap));
+  hipLaunchKernelGGL(k_col_sum, dim3(kH), dim3(256), 0, s, n, kH, w.dZ.p, g.gb[0].p);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }

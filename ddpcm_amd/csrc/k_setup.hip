@@ -448,6 +448,7 @@ int run_setup(ddx_handle *h) {
   DDX_TRY(scan_exclusive(h->segcnt.p, h->segoff.p, n, h->scan_tmp, s));
   h->nseg_cap = (int)G + n;   // segments: at most one per container plus one per worker
 
+  h->have_geo32 = false;   // the float copies are remade on their next use
   DDX_TRY(h->tgeo.reserve(ntri + 1));
   DDX_TRY(h->ttw.reserve(ntri + 1));
   DDX_TRY(h->tp.reserve(ntri + 1));

@@ -230,13 +230,28 @@ __device__ __forceinline__ void wave_reduce_split(R *v, int &base, int &len) {
   if constexpr (S < 5) wave_reduce_split<S + 1, H, R>(v, base, len);
 }
 
+// The per-triple records in the sweep's arithmetic: (cos theta, sin theta,
+// cos phi, sin phi) and (t, W) as double4 / double2 for the FP64 sweep, as
+// float4 / float2 copies (half the bytes) for the FP32 sweep.
+template <typename R>
+struct Geo {
+  typedef double4 G4;
+  typedef double2 G2;
+};
+template <>
+struct Geo<float> {
+  typedef float4 G4;
+  typedef float2 G2;
+};
+
 // One pass over triples beg..end-1 of a container: rw[tslot] = W (R . X_c)
 // when DIR (X' in xsh), and when AM0 < AM1 the (l, +-m), AM0 <= m < AM1,
 // entries of the chunk's share of (L*S)_c, -cf * sum_t w_n xi(n, o) W p~
 // trig, into the segment row yp.
 template <typename R, int LMAX, bool DIR, int AM0, int AM1>
 __device__ __forceinline__ void sweep_pass(
-    int beg, int end, const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
+    int beg, int end, const typename Geo<R>::G4 *__restrict__ tgeo,
+    const typename Geo<R>::G2 *__restrict__ ttw,
     const int *__restrict__ tslot, const int *__restrict__ tp, const R *xsh,
     R *__restrict__ rw, const R *__restrict__ xiw, const double *__restrict__ cf,
     double *__restrict__ yp) {
@@ -247,11 +262,15 @@ __device__ __forceinline__ void sweep_pass(
   for (int k = 0; k < NA; ++k) acc[k] = R(0);
   // software pipeline, two triples deep: geometry, slot and owner point of
   // triple k + 128 and xi of triple k + 64 load while triple k is evaluated
+  typedef typename Geo<R>::G4 G4;
+  typedef typename Geo<R>::G2 G2;
   int k = beg + (int)threadIdx.x;
-  const double4 z4 = make_double4(0.0, 0.0, 0.0, 0.0);
-  const double2 z2 = make_double2(0.0, 0.0);
-  double4 g0 = z4, g1 = z4;
-  double2 w0 = z2, w1 = z2;
+  G4 z4;
+  z4.x = z4.y = z4.z = z4.w = R(0);
+  G2 z2;
+  z2.x = z2.y = R(0);
+  G4 g0 = z4, g1 = z4;
+  G2 w0 = z2, w1 = z2;
   int s0 = 0, s1 = 0, p1 = 0;
   R x0 = R(0);
   if (k < end) {
@@ -268,8 +287,8 @@ __device__ __forceinline__ void sweep_pass(
   }
   for (; k < end; k += kWave) {
     const int k2 = k + 2 * kWave;
-    double4 g2 = z4;
-    double2 w2 = z2;
+    G4 g2 = z4;
+    G2 w2 = z2;
     int s2 = 0, p2 = 0;
     if (k2 < end) {
       g2 = tgeo[k2];
@@ -317,7 +336,8 @@ __device__ __forceinline__ void sweep_pass(
 // The adjoint passes of blocks B.. (the first also evaluates L X when DIR)
 template <typename R, int LMAX, bool DIR, int B>
 __device__ __forceinline__ void sweep_blocks(
-    int beg, int end, const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
+    int beg, int end, const typename Geo<R>::G4 *__restrict__ tgeo,
+    const typename Geo<R>::G2 *__restrict__ ttw,
     const int *__restrict__ tslot, const int *__restrict__ tp, const R *xsh,
     R *__restrict__ rw, const R *__restrict__ xiw, const double *__restrict__ cf,
     double *__restrict__ yp) {
@@ -333,7 +353,8 @@ __device__ __forceinline__ void sweep_blocks(
 // when ADJ the chunk's share of (L*S)_c into the segment row yp.
 template <typename R, int LMAX, bool DIR, bool ADJ>
 __device__ __forceinline__ void sweep_chunk(
-    int c, int beg, int end, const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
+    int c, int beg, int end, const typename Geo<R>::G4 *__restrict__ tgeo,
+    const typename Geo<R>::G2 *__restrict__ ttw,
     const int *__restrict__ tslot, const int *__restrict__ tp, const double *__restrict__ Xs,
     R *__restrict__ rw, const R *__restrict__ xiw, const double *__restrict__ cf,
     double *__restrict__ yp) {
@@ -363,7 +384,8 @@ template <typename R, int LMAX, bool DIR, bool ADJ>
 __global__ __launch_bounds__(kWave, 2) void k_sweep(
     long long U, long long G, const int *__restrict__ uoff, const int *__restrict__ segoff,
     const int *__restrict__ inl, const int *__restrict__ pairoff, int n,
-    const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
+    const typename Geo<R>::G4 *__restrict__ tgeo,
+    const typename Geo<R>::G2 *__restrict__ ttw,
     const int *__restrict__ tslot, const int *__restrict__ tp, const double *__restrict__ Xs,
     R *__restrict__ rw, const R *__restrict__ xiw, const double *__restrict__ cf,
     double *__restrict__ yp, const DiisState *__restrict__ gdir,
@@ -461,14 +483,51 @@ __global__ void k_cf_scale(size_t nn, int nylm, const double *__restrict__ cf,
   if (k < nn) xs[k] = cf[k % nylm] * x[k];
 }
 
+// float copies of the per-triple records for the FP32 sweeps, made once
+// per setup (on the first FP32 sweep after it)
+__global__ void k_geo_to_f32(long long n, const double4 *__restrict__ g,
+                             const double2 *__restrict__ w, float4 *__restrict__ g32,
+                             float2 *__restrict__ w32) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double4 a = g[k];
+  const double2 b = w[k];
+  g32[k] = make_float4((float)a.x, (float)a.y, (float)a.z, (float)a.w);
+  w32[k] = make_float2((float)b.x, (float)b.y);
+}
+
+int geometry32(ddx_handle *h, hipStream_t s) {
+  if (h->have_geo32) return DDX_OK;
+  DDX_TRY(h->tgeo32.reserve((size_t)h->ntri + 1));
+  DDX_TRY(h->ttw32.reserve((size_t)h->ntri + 1));
+  if (h->ntri > 0) {
+    hipLaunchKernelGGL(k_geo_to_f32, dim3((unsigned)(((long long)h->ntri + 255) / 256)),
+                       dim3(256), 0, s, (long long)h->ntri, h->tgeo.p, h->ttw.p, h->tgeo32.p,
+                       h->ttw32.p);
+    DDX_HIP(hipGetLastError());
+  }
+  h->have_geo32 = true;
+  return DDX_OK;
+}
+
 template <typename R, int LMAX, bool DIR, bool ADJ>
 int launch_sweep(ddx_handle *h, hipStream_t s, const double *Xs, R *rw, const R *xiw, double *yp,
                  const DiisState *gdir, const DiisState *gadj) {
   if (h->nworker == 0) return DDX_OK;
+  const typename Geo<R>::G4 *tgeo;
+  const typename Geo<R>::G2 *ttw;
+  if constexpr (sizeof(R) == 4) {
+    DDX_TRY(geometry32(h, s));
+    tgeo = h->tgeo32.p;
+    ttw = h->ttw32.p;
+  } else {
+    tgeo = h->tgeo.p;
+    ttw = h->ttw.p;
+  }
   TimedLaunch tl(h, (DIR && ADJ) ? DDX_K_SWEEP_PAIR : (DIR ? DDX_K_SWEEP_L : DDX_K_SWEEP_LSTAR), s);
   hipExtLaunchKernelGGL((k_sweep<R, LMAX, DIR, ADJ>), dim3(h->nworker), dim3(kWave), 0, s, tl.b,
                         tl.e, 0, (long long)h->nunit, (long long)h->nworker, h->uoff.p,
-                        h->segoff.p, h->inl.p, h->pairoff.p, h->n, h->tgeo.p, h->ttw.p,
+                        h->segoff.p, h->inl.p, h->pairoff.p, h->n, tgeo, ttw,
                         h->tslot.p, h->tp.p, Xs, rw, xiw, h->cf.p, yp, gdir, gadj,
                         h->ninst > 1 ? h->inst.p : nullptr, h->ninst);
   DDX_HIP(hipGetLastError());
