@@ -32,7 +32,8 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["ddx_api.hip", "k_scan.hip", "k_setup.hip", "k_gemm.hip", "k_sweep.hip",
-           "k_solver.hip", "k_coulomb.hip", "k_forces.hip", "k_guess.hip", "k_ops.hip", "tables.cpp"]
+           "k_solver.hip", "k_coulomb.hip", "k_forces.hip", "k_guess.hip", "k_ops.hip", "k_fmm.hip",
+           "tables.cpp"]
 
 
 def _compile(src: str) -> str:

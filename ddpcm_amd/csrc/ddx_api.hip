@@ -175,8 +175,11 @@ int pipeline(ddx_handle *h) {
   if (h->igrad) DDX_TRY(h->ef.reserve(field_buffer_size(h)));
   DDX_HIP(hipEventRecord(h->ev[1], s));
   // potential at the cavity points, with forces also the solute field there
-  DDX_TRY(coulomb_potential(h, h->ctile.p, h->nctile, h->charge.p, h->sph.p, h->ccav.p,
-                            h->phi.p, h->igrad ? h->ef.p : nullptr));
+  if (use_fmm(h))
+    DDX_TRY(fmm_potential(h, h->charge.p, h->phi.p, h->igrad ? h->ef.p : nullptr));
+  else
+    DDX_TRY(coulomb_potential(h, h->ctile.p, h->nctile, h->charge.p, h->sph.p, h->ccav.p,
+                              h->phi.p, h->igrad ? h->ef.p : nullptr));
   DDX_TRY(make_psi(h, h->charge.p, h->psi.p));
   DDX_TRY(build_rhs(h, h->phi.p, h->rhs.p));
   DDX_HIP(hipEventRecord(h->ev[2], s));
@@ -514,8 +517,11 @@ int ddx_mkrhs(ddx_handle *h, const double *charge, double *phi, double *psi) {
   DDX_TRY(reserve_vectors(h));
   DDX_TRY(upload(h->charge, charge, h->n, h->stream));
   DDX_TRY(h->phi.reserve((size_t)h->ncav + 1));
-  DDX_TRY(coulomb_potential(h, h->ctile.p, h->nctile, h->charge.p, h->sph.p, h->ccav.p,
-                            h->phi.p, nullptr));
+  if (use_fmm(h))
+    DDX_TRY(fmm_potential(h, h->charge.p, h->phi.p, nullptr));
+  else
+    DDX_TRY(coulomb_potential(h, h->ctile.p, h->nctile, h->charge.p, h->sph.p, h->ccav.p,
+                              h->phi.p, nullptr));
   DDX_TRY(make_psi(h, h->charge.p, h->psi.p));
   DDX_TRY(download(phi, h->phi.p, h->ncav, h->stream));
   DDX_TRY(download(psi, h->psi.p, (size_t)h->nylm * h->n, h->stream));
@@ -806,6 +812,20 @@ int ddx_autotune(ddx_handle *h, int *workers_per_cu, int *proj_split) {
   DDX_TRY(sync(h));
   if (workers_per_cu) *workers_per_cu = bw;
   if (proj_split) *proj_split = bs;
+  return DDX_OK;
+}
+
+int ddx_set_coulomb(ddx_handle *h, int mode) {
+  if (!h || mode < DDX_COULOMB_AUTO || mode > DDX_COULOMB_FMM) {
+    set_error("ddx_set_coulomb: mode must be one of DDX_COULOMB_*");
+    return DDX_ERR_ARG;
+  }
+  if (mode == DDX_COULOMB_FMM && h->ninst > 1) {
+    set_error("ddx_set_coulomb: batch handles sum the Coulomb terms directly");
+    return DDX_ERR_STATE;
+  }
+  DDX_TRY(sync(h));
+  h->coulomb_mode = mode;
   return DDX_OK;
 }
 

@@ -109,6 +109,23 @@ struct GuessWork {
   DevArray<float> ksplit;   // split-K partials of the weight gradients
 };
 
+// Far-field Coulomb sums (k_fmm.hip): expansion order, far criterion
+// d >= R / rho, and the systems that take them by default
+constexpr int kFmmOrder = 16;        // the potential at the cavity points (enters E)
+constexpr int kFmmFieldOrder = 10;   // the field of zeta at the nuclei (forces only)
+constexpr double kFmmRho = 0.2;
+constexpr int kFmmMinSpheres = 8000;
+constexpr double kFmmMinPoints = 100.0;   // exposed points per sphere
+struct FmmState {
+  int ngrid = 0;                        // grid the order-P tables were made for
+  bool lists_ok = false;                // near lists of the current setup
+  DevArray<double> Yp, facsP, cfP;      // Y (ngrid, (P+1)^2), facs and cf to order P
+  DevArray<double> Yq, cfQ;             // Y and cf to the field's order
+  DevArray<int> cnt, noff, nidx;        // near charges of each sphere (CSR)
+  DevArray<int> toff, tidx;             // near spheres of each charge (CSR)
+  DevArray<double> A, Xm, part;         // local expansions, cf-scaled multipoles, partials
+};
+
 // Event pairs around the launches of one kernel class (ddx_kernel_timer).
 // A launch made for a Jacobi/DIIS solve carries the solve's serial number
 // and round, so that rounds enqueued after convergence can be dropped.
@@ -242,6 +259,8 @@ struct ddx_handle {
   };
   std::vector<RoundGraph> round_graphs;
   int ysplit = 1;                 // partial projections of L X in the solver
+  int coulomb_mode = DDX_COULOMB_AUTO;   // ddx_set_coulomb
+  ddx::FmmState fmm;
 
   // solver workspace: lane 0 runs on `stream`, lane 1 on a stream of its own
   ddx::Lane lane[2];

@@ -43,6 +43,10 @@ int apply_lstarx(ddx_handle *h, Lane &ln, const double *x, double *y, const Diis
 // yd = L xd (ld.xs holds cf xd) and L* xa as per-segment partial sums in
 // la.yp, in one pass over the triples, gated by the DIIS states of the lanes
 int apply_pair(ddx_handle *h, Lane &ld, double *yd, Lane &la, const double *xa, int ysplit);
+// local expansions A (n, (P+1)^2) of the far point charges q at every
+// sphere (charges j with d_ij >= R_i / rho), P in {8, 12, 16}; cfP the cf
+// factors facs 4pi/(2l+1) to order P
+int fmm_p2l(ddx_handle *h, int P, double rho, const double *q, const double *cfP, double *A);
 // xs = cf x
 int cf_scale(ddx_handle *h, hipStream_t s, const double *x, double *xs);
 // y = sum of the per-segment partial sums of L* S
@@ -101,6 +105,13 @@ int make_psi_n(ddx_handle *h, int n, int nylm, const double *charge, double *psi
 // device address of the scratch array of ddx_efld_noalloc on the current
 // device (looked up once per device; out may be null)
 int noalloc_setup(double **out);
+
+// k_fmm.hip — far-field expansions of the Coulomb sums (ddx_set_coulomb)
+bool use_fmm(const ddx_handle *h);
+// phi at the cavity points and, with ef, the solute field there
+int fmm_potential(ddx_handle *h, const double *q, double *phi, double *ef);
+// ez(3, n): the field of the cavity charges zeta at the nuclei
+int fmm_zeta_field(ddx_handle *h, const double *zeta, double *ez);
 
 // k_forces.hip — forces_dd, ddmkzeta and the field contractions
 int forces_dd(ddx_handle *h, const double *d_phi, const double *d_sigma,

@@ -411,12 +411,20 @@ int field_forces(ddx_handle *h, const double *d_charge, const double *d_zeta, do
   DDX_HIP(hipGetLastError());
   // solute field at the cavity points (unless the potential kernel made
   // it), and the field of zeta at the nuclei
+  const bool fmm = use_fmm(h);
   if (!solute_field) {
     double *phi_scratch = c3 + 3 * (size_t)n;
-    DDX_TRY(coulomb_potential(h, h->ctile.p, h->nctile, d_charge, h->sph.p, h->ccav.p,
-                              phi_scratch, ef));
+    if (fmm)
+      DDX_TRY(fmm_potential(h, d_charge, phi_scratch, ef));
+    else
+      DDX_TRY(coulomb_potential(h, h->ctile.p, h->nctile, d_charge, h->sph.p, h->ccav.p,
+                                phi_scratch, ef));
   }
-  DDX_TRY(coulomb_field(h, h->ftile.p, h->nftile, h->fgrp.p, h->nfgrp, d_zeta, h->ccav.p, c3, ez));
+  if (fmm)
+    DDX_TRY(fmm_zeta_field(h, d_zeta, ez));
+  else
+    DDX_TRY(coulomb_field(h, h->ftile.p, h->nftile, h->fgrp.p, h->nfgrp, d_zeta, h->ccav.p, c3,
+                          ez));
   hipLaunchKernelGGL(k_field_contract, dim3(n), dim3(256), 0, s, h->ngrid, h->cavidx.p, d_zeta, ef,
                      d_charge, ez, d_fx);
   DDX_HIP(hipGetLastError());

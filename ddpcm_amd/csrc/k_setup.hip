@@ -449,6 +449,7 @@ int run_setup(ddx_handle *h) {
   h->nseg_cap = (int)G + n;   // segments: at most one per container plus one per worker
 
   h->have_geo32 = false;   // the float copies are remade on their next use
+  h->fmm.lists_ok = false;  // as are the near lists of the far-field sums
   DDX_TRY(h->tgeo.reserve(ntri + 1));
   DDX_TRY(h->ttw.reserve(ntri + 1));
   DDX_TRY(h->tp.reserve(ntri + 1));

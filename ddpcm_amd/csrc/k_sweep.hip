@@ -483,6 +483,105 @@ __global__ void k_cf_scale(size_t nn, int nylm, const double *__restrict__ cf,
   if (k < nn) xs[k] = cf[k % nylm] * x[k];
 }
 
+// ---------------------------------------------------------------------------
+// Far-field expansions of the Coulomb sums (used by k_fmm.hip).  The local
+// expansion at target sphere i of the point charges j far from it:
+//   A_i,lm = cf_lm sum_{j: d_ij^2 >= rc_i^2} (q_j / d) t^l p~_l^m(cos) trig_m,
+//   t = R_i / d,  rc_i = R_i / rho
+// (addition theorem: 1/|R s - v| = sum_l 4pi/(2l+1) R^l / d^(l+1)
+// Y_lm(s) Y_lm(v / d) for R < d), so that the far charges' potential at the
+// point R_i s of sphere i is sum_lm A_lm Y_lm(s), with a truncation error
+// below (q / (d - R)) rho^(P+1) per charge.  One wave per target sphere,
+// lanes over the charges of its instance, one pass per block of
+// accumulators (the sweep's m-blocks), each lane's sums reduce-scattered.
+template <int P, int AM0, int AM1>
+__device__ __forceinline__ void p2l_pass(int i, double4 si, double rc2, int sb, int se,
+                                         const double4 *__restrict__ sph,
+                                         const double *__restrict__ q,
+                                         const double *__restrict__ cf, double *__restrict__ A) {
+  constexpr int NA = mspan(P, AM0, AM1);
+  constexpr int NY = (P + 1) * (P + 1);
+  double acc[NA];
+#pragma unroll
+  for (int k = 0; k < NA; ++k) acc[k] = 0.0;
+  for (int j = sb + (int)threadIdx.x; j < se; j += kWave) {
+    const double4 sj = sph[j];
+    const double vx = sj.x - si.x, vy = sj.y - si.y, vz = sj.z - si.z;
+    double d2;
+    {
+#pragma clang fp contract(off)
+      const double ax = si.x - sj.x, ay = si.y - sj.y, az = si.z - sj.z;
+      d2 = (ax * ax + ay * ay) + az * az;   // as the near lists classify the pair
+    }
+    double coef = 0.0, cthe = 1.0, sthe = 0.0, cphi = 1.0, sphi = 0.0, t = 0.0;
+    if (d2 >= rc2 && j != i) {
+      const double id = 1.0 / sqrt(d2);
+      const double rxy = sqrt(vx * vx + vy * vy);
+      coef = q[j] * id;
+      t = si.w * id;
+      cthe = vz * id;
+      sthe = rxy * id;
+      if (rxy > 0.0) {
+        cphi = vx / rxy;
+        sphi = vy / rxy;
+      }
+    }
+    sweep_triple<double, P, false, AM0, AM1>(cthe, sthe, cphi, sphi, t, nullptr, coef, acc);
+  }
+  int base = 0, len = NA;
+  wave_reduce_split<0, NA, double>(acc, base, len);
+  constexpr int NF = reduced_len(NA, 6);
+#pragma unroll
+  for (int j = 0; j < NF; ++j) {
+    if (j < len) {
+      const int qi = base + j;
+      int lm = qi;
+#pragma unroll
+      for (int m = AM0; m < ((AM0 == 0 && AM1 == P + 1) ? AM0 : AM1); ++m) {
+        const int o = mspan(P, AM0, m);
+        if (qi >= o && qi < o + mcount(P, m)) {
+          const int r = qi - o;
+          const int l = m + (m == 0 ? r : (r >> 1));
+          lm = lmi(l, (m > 0 && (r & 1)) ? -m : m);
+        }
+      }
+      A[(size_t)i * NY + lm] = cf[lm] * acc[j];
+    }
+  }
+}
+
+template <int P, int B>
+__device__ __forceinline__ void p2l_blocks(int i, double4 si, double rc2, int sb, int se,
+                                           const double4 *__restrict__ sph,
+                                           const double *__restrict__ q,
+                                           const double *__restrict__ cf,
+                                           double *__restrict__ A) {
+  constexpr int M0 = mblock_begin(P, false, B, false), M1 = mblock_begin(P, false, B + 1, false);
+  p2l_pass<P, M0, M1>(i, si, rc2, sb, se, sph, q, cf, A);
+  if constexpr (M1 <= P) p2l_blocks<P, B + 1>(i, si, rc2, sb, se, sph, q, cf, A);
+}
+
+// sources of target sphere i: the spheres of its instance (soff, inst) or
+// all n spheres
+template <int P>
+__global__ __launch_bounds__(kWave, 2) void k_p2l(int n, const double4 *__restrict__ sph,
+                                                  const double *__restrict__ q,
+                                                  const int *__restrict__ inst,
+                                                  const int *__restrict__ soff, double rho,
+                                                  const double *__restrict__ cf,
+                                                  double *__restrict__ A) {
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const double4 si = sph[i];
+  const double rc = si.w / rho;
+  int sb = 0, se = n;
+  if (inst) {
+    sb = soff[inst[i]];
+    se = soff[inst[i] + 1];
+  }
+  p2l_blocks<P, 0>(i, si, rc * rc, sb, se, sph, q, cf, A);
+}
+
 // float copies of the per-triple records for the FP32 sweeps, made once
 // per setup (on the first FP32 sweep after it)
 __global__ void k_geo_to_f32(long long n, const double4 *__restrict__ g,
@@ -674,6 +773,34 @@ int apply_pair(ddx_handle *h, Lane &ld, double *yd, Lane &la, const double *xa, 
   const DiisState *gd = solve_gate(h, ld), *ga = solve_gate(h, la);
   return ld.f32 ? pair_r<float>(h, ld, yd, gd, la, xa, ga, ysplit)
                 : pair_r<double>(h, ld, yd, gd, la, xa, ga, ysplit);
+}
+
+// A (n, (P+1)^2) of the far charges of every sphere (k_p2l); cfP: the cf
+// factors to order P
+int fmm_p2l(ddx_handle *h, int P, double rho, const double *q, const double *cfP, double *A) {
+  if (h->n == 0) return DDX_OK;
+  const int *inst = h->ninst > 1 ? h->inst.p : nullptr;
+  const int *soff = h->ninst > 1 ? h->soff.p : nullptr;
+  hipStream_t s = h->stream;
+  switch (P) {
+    case 8:
+      hipLaunchKernelGGL(k_p2l<8>, dim3(h->n), dim3(kWave), 0, s, h->n, h->sph.p, q, inst, soff,
+                         rho, cfP, A);
+      break;
+    case 12:
+      hipLaunchKernelGGL(k_p2l<12>, dim3(h->n), dim3(kWave), 0, s, h->n, h->sph.p, q, inst,
+                         soff, rho, cfP, A);
+      break;
+    case 16:
+      hipLaunchKernelGGL(k_p2l<16>, dim3(h->n), dim3(kWave), 0, s, h->n, h->sph.p, q, inst,
+                         soff, rho, cfP, A);
+      break;
+    default:
+      set_error("far-field order %d not built (8, 12, 16)", P);
+      return DDX_ERR_ARG;
+  }
+  DDX_HIP(hipGetLastError());
+  return DDX_OK;
 }
 
 }  // namespace ddx

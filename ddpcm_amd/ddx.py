@@ -31,7 +31,7 @@ EXPORTS = (
     "ddx_batch_sizes", "ddx_gemm_bf16", "ddx_set_guess", "ddx_guess_train", "ddx_guess_reset",
     "ddx_mlp_grad", "ddx_set_graphs", "ddx_autotune", "ddx_efld_noalloc",
     "ddx_jacobi_diis_ops", "ddx_op_lx", "ddx_op_lstarx", "ddx_op_ldm1x", "ddx_op_hnorm",
-    "ddx_mkrhs_at",
+    "ddx_mkrhs_at", "ddx_set_coulomb",
 )
 DDX_GUESS_REFERENCE, DDX_GUESS_PREVIOUS, DDX_GUESS_LEARNED, DDX_GUESS_LEARNED_PREVIOUS = 0, 1, 2, 3
 KERNELS = ("sweep_l", "sweep_lstar", "project", "expand", "jacobi", "diis", "extrap", "field",
@@ -114,6 +114,7 @@ def load_library(path: str = LIB_PATH):
     L.ddx_gemm_bf16.argtypes = [C.c_int, C.c_int, C.c_int, _u16, _u16, C.POINTER(C.c_float)]
     L.ddx_set_guess.argtypes = [C.c_void_p, C.c_int]
     L.ddx_set_graphs.argtypes = [C.c_void_p, C.c_int]
+    L.ddx_set_coulomb.argtypes = [C.c_void_p, C.c_int]
     L.ddx_autotune.argtypes = [C.c_void_p, _ip, _ip]
     L.ddx_guess_train.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp]
     L.ddx_guess_reset.argtypes = [C.c_void_p, C.c_uint]
@@ -346,6 +347,10 @@ class DDCosmo:
         w, sp = C.c_int(), C.c_int()
         self._check(self.L.ddx_autotune(self.h, C.byref(w), C.byref(sp)))
         return w.value, sp.value
+
+    def set_coulomb(self, mode):
+        """Coulomb sums: 0 automatic, 1 direct, 2 far-field expansions."""
+        self._check(self.L.ddx_set_coulomb(self.h, int(mode)))
 
     def set_graphs(self, enable=True):
         """Replay the solver's iteration groups as captured HIP graphs."""

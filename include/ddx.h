@@ -226,6 +226,22 @@ int ddx_batch_sizes(const ddx_handle *h, int *count, int *sph_off, int *cav_off)
  * it is set; ddx_jacobi_diis always runs FP64. */
 int ddx_set_precision(ddx_handle *h, int prec);
 
+/* Coulomb sums of the point-charge path (mkrhs.f90:102-114 and the two
+ * field terms of main.f90:235-265, efld.f90:98-114; no counterpart in the
+ * reference, which sums all pairs, as a choice).  DDX_COULOMB_DIRECT sums
+ * all pairs.  DDX_COULOMB_FMM splits each sum into near pairs, summed
+ * directly, and far pairs (sphere centres at least 5 radii apart) taken
+ * through spherical-harmonic expansions of order 16 about the sphere
+ * centres, which leave a relative error near 1e-12 per far term.
+ * DDX_COULOMB_AUTO (the default) takes the expansions for single systems of
+ * at least 8000 spheres with at least 100 exposed points per sphere, and
+ * the direct sums otherwise.  Batch handles always sum directly
+ * (DDX_ERR_STATE for DDX_COULOMB_FMM). */
+#define DDX_COULOMB_AUTO 0
+#define DDX_COULOMB_DIRECT 1
+#define DDX_COULOMB_FMM 2
+int ddx_set_coulomb(ddx_handle *h, int mode);
+
 /* Initial guess of the solves (no counterpart in the reference beyond
  * DDX_GUESS_REFERENCE, which it always uses: cosmo.f90:136-138,174-176).
  *   DDX_GUESS_REFERENCE  D^-1 rhs
