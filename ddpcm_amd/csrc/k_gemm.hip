@@ -10,9 +10,10 @@
 // Projection: one workgroup per 16 spheres holds all nylm rows (up to 8
 // MFMA row tiles per pass); its four waves split the ngrid reduction and
 // are summed in LDS in a fixed order; each wave loads the operands of four
-// k-steps before issuing their MFMAs, to keep loads in flight.  Expansion: one wave per 16 x 16
-// output tile.  Operands go straight from L2 to registers: Y (<= 1.2 MB at
-// lmax 10 / 1202 points) stays cache-resident across workgroups.
+// k-steps before issuing their MFMAs, to keep loads in flight.  Expansion:
+// one workgroup per 16 J spheres with their S rows staged in LDS
+// (k_expand_t).  The Y tables (<= 1.2 MB at lmax 10 / 1202 points) stay
+// cache-resident across workgroups.
 //
 // Operand layout of v_mfma_f64_16x16x4_f64 (one double per lane):
 //   A(i, k) in lane i + 16 k, B(k, j) in lane j + 16 k,
@@ -27,8 +28,6 @@ namespace {
 constexpr int kMaxTiles = 8;   // row tiles of 16 per projection pass
 
 constexpr int kUnroll = 4;   // k-steps whose operands are loaded before their MFMAs
-constexpr int kExpandUnroll = 8;
-constexpr size_t kExpandWide = 2048;   // blocks from which a wave takes 4 sphere tiles
 constexpr int kUnroll32 = 2;   // k-steps in flight in the FP32 projection
 
 __global__ __launch_bounds__(256) void k_project(int nylm, int ngrid, int nsph, double alpha,
@@ -100,64 +99,6 @@ __global__ __launch_bounds__(256) void k_project(int nylm, int ngrid, int nsph, 
       }
     }
     __syncthreads();
-  }
-}
-
-// xi(n, i) = scale_n * sum_lm Y(lm, n) S(lm, i); one wave per 16 points x
-// J tiles of 16 spheres (J independent accumulator chains that share the Y
-// operand), blockIdx.x over point tiles of 64, blockIdx.y over sphere tiles
-// of 16 J.  J = 4 pays where the grid has blocks to spare (large systems,
-// batches), J = 1 keeps small systems' grids full.
-template <int J>
-__global__ __launch_bounds__(256) void k_expand(int nylm, int ngrid, int nsph,
-                                                const double *__restrict__ Y,
-                                                const double *__restrict__ S,
-                                                const double4 *__restrict__ grid, int wscale,
-                                                double *__restrict__ xi,
-                                                const DiisState *__restrict__ gate) {
-  if (gate && gate->done) return;
-  const int lane = lane_id(), wave = threadIdx.x / kWave;
-  const int n0 = blockIdx.x * 64 + 16 * wave;
-  const int i0 = blockIdx.y * 16 * J;
-  if (n0 >= ngrid) return;
-  const int row = lane & 15, kq = lane >> 4;
-  const int na = n0 + row;          // A row: grid point
-  v4d acc[J];
-#pragma unroll
-  for (int j = 0; j < J; ++j) acc[j] = v4d{0.0, 0.0, 0.0, 0.0};
-  // the operands of kExpandUnroll k-steps are loaded before their MFMAs
-  for (int k0 = 0; k0 < nylm; k0 += 4 * kExpandUnroll) {
-    double a[kExpandUnroll], b[kExpandUnroll][J];
-#pragma unroll
-    for (int u = 0; u < kExpandUnroll; ++u) {
-      const int k = k0 + 4 * u + kq;
-      a[u] = (na < ngrid && k < nylm) ? Y[k + (size_t)nylm * na] : 0.0;
-#pragma unroll
-      for (int j = 0; j < J; ++j) {
-        const int sb = i0 + 16 * j + row;   // B column: sphere
-        b[u][j] = (sb < nsph && k < nylm) ? S[k + (size_t)nylm * sb] : 0.0;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kExpandUnroll; ++u)
-#pragma unroll
-      for (int j = 0; j < J; ++j)
-        acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u][j], acc[j], 0, 0, 0);
-  }
-#pragma unroll
-  for (int j = 0; j < J; ++j) {
-    const int sph = i0 + 16 * j + row;
-    if (sph < nsph) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int n = n0 + kq + 4 * q;
-        if (n < ngrid) {
-          double v = acc[j][q];
-          if (wscale) v *= grid[n].w;
-          xi[n + (size_t)ngrid * sph] = v;
-        }
-      }
-    }
   }
 }
 
@@ -237,58 +178,87 @@ __global__ __launch_bounds__(256) void k_project32(int nylm, int ngrid, int nsph
   }
 }
 
-// xi(n, i) = scale_n * sum_lm Y(lm, n) S(lm, i) in FP32 (S rounded on load),
-// with the launch shapes of k_expand (J sphere tiles per wave)
-template <int J>
-__global__ __launch_bounds__(256) void k_expand32(int nylm, int ngrid, int nsph,
-                                                  const float *__restrict__ Y,
+// The expansion, both precisions: xi(n, i) = [w_n] sum_lm Y(lm, n) S(lm, i)
+// with T = double (v_mfma_f64_16x16x4f64) or float (v_mfma_f32_16x16x4f32,
+// S rounded as it is staged).  One workgroup per 16 J spheres: their rows
+// of S, contiguous in memory, are staged in LDS once; the four waves take
+// the point tiles of 16 (w, w + 4, ...), each an MFMA chain over lm with A
+// = the Y rows of the tile's points (a table that stays in the caches) and
+// B from LDS.  Each finished tile goes back through LDS so that 16 lanes
+// store one sphere's 16 points as one contiguous run.
+template <typename T, int J>
+__global__ __launch_bounds__(256) void k_expand_t(int nylm, int ngrid, int nsph,
+                                                  const T *__restrict__ Y,
                                                   const double *__restrict__ S,
                                                   const double4 *__restrict__ grid, int wscale,
-                                                  float *__restrict__ xi,
+                                                  T *__restrict__ xi,
                                                   const DiisState *__restrict__ gate) {
+  typedef T vt __attribute__((ext_vector_type(4)));
   if (gate && gate->done) return;
+  extern __shared__ double smem_d[];
+  T *sS = reinterpret_cast<T *>(smem_d);                 // 16 J spheres x nylm
+  T *tr = sS + ((16 * J * nylm + 1) & ~1);               // 4 waves x J x 16 x 17
+  const int i0 = blockIdx.x * 16 * J;
+  const int nsp = min(16 * J, nsph - i0);
+  for (int k = threadIdx.x; k < 16 * J * nylm; k += 256)
+    sS[k] = k < nsp * nylm ? (T)S[(size_t)nylm * i0 + k] : T(0);
+  __syncthreads();
   const int lane = lane_id(), wave = threadIdx.x / kWave;
-  const int n0 = blockIdx.x * 64 + 16 * wave;
-  const int i0 = blockIdx.y * 16 * J;
-  if (n0 >= ngrid) return;
   const int row = lane & 15, kq = lane >> 4;
-  const int na = n0 + row;
-  v4f acc[J];
+  T *trw = tr + wave * J * 16 * 17;
+  const int ntile = (ngrid + 15) / 16;
+  for (int pt = wave; pt < ntile; pt += 4) {
+    const int n0 = pt * 16;
+    const int na = n0 + row;
+    vt acc[J];
 #pragma unroll
-  for (int j = 0; j < J; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < nylm; k0 += 4 * kExpandUnroll) {
-    float a[kExpandUnroll], b[kExpandUnroll][J];
-#pragma unroll
-    for (int u = 0; u < kExpandUnroll; ++u) {
-      const int k = k0 + 4 * u + kq;
-      a[u] = (na < ngrid && k < nylm) ? Y[k + (size_t)nylm * na] : 0.f;
+    for (int j = 0; j < J; ++j) acc[j] = vt{T(0), T(0), T(0), T(0)};
+    for (int k0 = 0; k0 < nylm; k0 += 4) {
+      const int k = k0 + kq;
+      const T a = (na < ngrid && k < nylm) ? Y[k + (size_t)nylm * na] : T(0);
 #pragma unroll
       for (int j = 0; j < J; ++j) {
-        const int sb = i0 + 16 * j + row;
-        b[u][j] = (sb < nsph && k < nylm) ? (float)S[k + (size_t)nylm * sb] : 0.f;
+        const T b = k < nylm ? sS[(16 * j + row) * nylm + k] : T(0);
+        if constexpr (sizeof(T) == 8)
+          acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
+        else
+          acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[j], 0, 0, 0);
       }
     }
+    // D(i = point, j = sphere) sits in lane j + 16 (i % 4), element i / 4
+    // (f64) or lane j + 16 (i / 4), element i % 4 (f32)
 #pragma unroll
-    for (int u = 0; u < kExpandUnroll; ++u)
-#pragma unroll
-      for (int j = 0; j < J; ++j)
-        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b[u][j], acc[j], 0, 0, 0);
-  }
-#pragma unroll
-  for (int j = 0; j < J; ++j) {
-    const int sph = i0 + 16 * j + row;
-    if (sph < nsph) {
+    for (int j = 0; j < J; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int n = n0 + 4 * kq + q;
-        if (n < ngrid) {
-          float v = acc[j][q];
-          if (wscale) v *= (float)grid[n].w;
-          xi[n + (size_t)ngrid * sph] = v;
-        }
+        const int i = sizeof(T) == 8 ? kq + 4 * q : 4 * kq + q;
+        trw[(j * 16 + row) * 17 + i] = acc[j][q];
       }
-    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    const int pl = lane & 15, sl = lane >> 4;
+    const int n = n0 + pl;
+    const T ws = (wscale && n < ngrid) ? (T)grid[n].w : T(1);
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int sp = 16 * j + sl + 4 * e;
+        if (n < ngrid && sp < nsp) xi[n + (size_t)ngrid * (i0 + sp)] = ws * trw[sp * 17 + pl];
+      }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   }
+}
+
+// sphere tiles per expansion workgroup: LDS for S and the transposes
+template <typename T>
+int expand_tiles(int nylm) {
+  return nylm * 16 * 2 * (int)sizeof(T) <= 48 * 1024 ? 2 : 1;
+}
+template <typename T>
+size_t expand_lds(int nylm, int J) {
+  return (size_t)(((16 * J * nylm + 1) & ~1) + 4 * J * 16 * 17) * sizeof(T);
 }
 
 __global__ void k_to_f32(size_t n, const double *__restrict__ a, float *__restrict__ b) {
@@ -325,12 +295,13 @@ int project32(ddx_handle *h, hipStream_t s, double alpha, const float *P, double
 int expand32(ddx_handle *h, hipStream_t s, const double *S, float *xi, bool wscale,
              const DiisState *gate) {
   if (h->n <= 0) return DDX_OK;
-  const int pt = (h->ngrid + 63) / 64;
-  const bool wide = (size_t)pt * ((h->n + 63) / 64) >= kExpandWide;
-  dim3 g(pt, wide ? (h->n + 63) / 64 : (h->n + 15) / 16);
+  const int J = expand_tiles<float>(h->nylm);
+  const size_t lds = expand_lds<float>(h->nylm, J);
   TimedLaunch tl(h, DDX_K_EXPAND, s);
-  hipExtLaunchKernelGGL(wide ? k_expand32<4> : k_expand32<1>, g, dim3(256), 0, s, tl.b, tl.e, 0, h->nylm, h->ngrid, h->n,
-                        h->Y32.p, S, h->grid.p, wscale ? 1 : 0, xi, gate);
+  hipExtLaunchKernelGGL(J == 2 ? k_expand_t<float, 2> : k_expand_t<float, 1>,
+                        dim3((h->n + 16 * J - 1) / (16 * J)), dim3(256), lds, s, tl.b, tl.e, 0,
+                        h->nylm, h->ngrid, h->n, h->Y32.p, S, h->grid.p, wscale ? 1 : 0, xi,
+                        gate);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
@@ -348,12 +319,13 @@ int project(ddx_handle *h, hipStream_t s, double alpha, const double *P, double 
 int expand(ddx_handle *h, hipStream_t s, const double *S, double *xi, bool wscale,
            const DiisState *gate) {
   if (h->n <= 0) return DDX_OK;
-  const int pt = (h->ngrid + 63) / 64;
-  const bool wide = (size_t)pt * ((h->n + 63) / 64) >= kExpandWide;
-  dim3 g(pt, wide ? (h->n + 63) / 64 : (h->n + 15) / 16);
+  const int J = expand_tiles<double>(h->nylm);
+  const size_t lds = expand_lds<double>(h->nylm, J);
   TimedLaunch tl(h, DDX_K_EXPAND, s);
-  hipExtLaunchKernelGGL(wide ? k_expand<4> : k_expand<1>, g, dim3(256), 0, s, tl.b, tl.e, 0, h->nylm, h->ngrid, h->n,
-                        h->Y.p, S, h->grid.p, wscale ? 1 : 0, xi, gate);
+  hipExtLaunchKernelGGL(J == 2 ? k_expand_t<double, 2> : k_expand_t<double, 1>,
+                        dim3((h->n + 16 * J - 1) / (16 * J)), dim3(256), lds, s, tl.b, tl.e, 0,
+                        h->nylm, h->ngrid, h->n, h->Y.p, S, h->grid.p, wscale ? 1 : 0, xi,
+                        gate);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
