@@ -12,13 +12,16 @@
 // are summed in LDS in a fixed order; each wave loads the operands of four
 // k-steps before issuing their MFMAs, to keep loads in flight.  Expansion:
 // one workgroup per 16 J spheres with their S rows staged in LDS
-// (k_expand_t).  The Y tables (<= 1.2 MB at lmax 10 / 1202 points) stay
+// (k_expand_t), its point tiles split over blockIdx.y when the sphere
+// tiles alone give too few workgroups.  The Y tables (<= 1.2 MB at lmax 10 / 1202 points) stay
 // cache-resident across workgroups.
 //
 // Operand layout of v_mfma_f64_16x16x4_f64 (one double per lane):
 //   A(i, k) in lane i + 16 k, B(k, j) in lane j + 16 k,
 //   D(i, j) in lane j + 16 (i % 4), element i / 4 of the 4-double result
 //   (measured on gfx950).
+#include <algorithm>
+
 #include "ddx_device.hpp"
 #include "ddx_kernels.hpp"
 
@@ -207,7 +210,7 @@ __global__ __launch_bounds__(256) void k_expand_t(int nylm, int ngrid, int nsph,
   const int row = lane & 15, kq = lane >> 4;
   T *trw = tr + wave * J * 16 * 17;
   const int ntile = (ngrid + 15) / 16;
-  for (int pt = wave; pt < ntile; pt += 4) {
+  for (int pt = wave + 4 * blockIdx.y; pt < ntile; pt += 4 * gridDim.y) {
     const int n0 = pt * 16;
     const int na = n0 + row;
     vt acc[J];
@@ -260,6 +263,13 @@ template <typename T>
 size_t expand_lds(int nylm, int J) {
   return (size_t)(((16 * J * nylm + 1) & ~1) + 4 * J * 16 * 17) * sizeof(T);
 }
+// point-tile split: about 2048 workgroups (8 per CU) where the sphere
+// tiles give fewer, at least one point tile per wave
+inline int expand_split(int nx, int ngrid) {
+  const int ntile = (ngrid + 15) / 16;
+  const int want = (2048 + nx - 1) / nx;
+  return std::max(1, std::min(want, (ntile + 3) / 4));
+}
 
 __global__ void k_to_f32(size_t n, const double *__restrict__ a, float *__restrict__ b) {
   size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -298,8 +308,9 @@ int expand32(ddx_handle *h, hipStream_t s, const double *S, float *xi, bool wsca
   const int J = expand_tiles<float>(h->nylm);
   const size_t lds = expand_lds<float>(h->nylm, J);
   TimedLaunch tl(h, DDX_K_EXPAND, s);
+  const int nx = (h->n + 16 * J - 1) / (16 * J);
   hipExtLaunchKernelGGL(J == 2 ? k_expand_t<float, 2> : k_expand_t<float, 1>,
-                        dim3((h->n + 16 * J - 1) / (16 * J)), dim3(256), lds, s, tl.b, tl.e, 0,
+                        dim3(nx, expand_split(nx, h->ngrid)), dim3(256), lds, s, tl.b, tl.e, 0,
                         h->nylm, h->ngrid, h->n, h->Y32.p, S, h->grid.p, wscale ? 1 : 0, xi,
                         gate);
   DDX_HIP(hipGetLastError());
@@ -322,8 +333,9 @@ int expand(ddx_handle *h, hipStream_t s, const double *S, double *xi, bool wscal
   const int J = expand_tiles<double>(h->nylm);
   const size_t lds = expand_lds<double>(h->nylm, J);
   TimedLaunch tl(h, DDX_K_EXPAND, s);
+  const int nx = (h->n + 16 * J - 1) / (16 * J);
   hipExtLaunchKernelGGL(J == 2 ? k_expand_t<double, 2> : k_expand_t<double, 1>,
-                        dim3((h->n + 16 * J - 1) / (16 * J)), dim3(256), lds, s, tl.b, tl.e, 0,
+                        dim3(nx, expand_split(nx, h->ngrid)), dim3(256), lds, s, tl.b, tl.e, 0,
                         h->nylm, h->ngrid, h->n, h->Y.p, S, h->grid.p, wscale ? 1 : 0, xi,
                         gate);
   DDX_HIP(hipGetLastError());
