@@ -444,8 +444,14 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(
 }
 
 // x_new = sum_i c_i x_i (or the plain Jacobi step), increment norm partials
-// per sphere (hsnorm, ddcosmo.f90:1233-1271), x <- x_new.  One wave per sphere.
-// ... and xs = cf x_new where the system keeps it.
+// per sphere (hsnorm, ddcosmo.f90:1233-1271), x <- x_new, and xs = cf x_new
+// where the system keeps it.  The workgroup streams its spheres' elements
+// with all 256 threads (consecutive threads, consecutive elements) and
+// leaves each element's norm term in LDS; then one thread per sphere adds
+// its terms in lm order, as hsnorm does.  Spheres beyond the LDS stage are
+// taken in chunks.
+constexpr int kExtrapTerms = 2048;   // staged norm terms per workgroup (nylm <= 289)
+
 __global__ __launch_bounds__(kJThreads) void k_extrap(int nylm, size_t nn, int diis_max,
                                                       const int4 *__restrict__ eblk,
                                                       const double *__restrict__ invl1,
@@ -460,8 +466,11 @@ __global__ __launch_bounds__(kJThreads) void k_extrap(int nylm, size_t nn, int d
   const int *__restrict__ newslot = q0.newslot + blk.x;
   double *__restrict__ x = q0.x;
   double *__restrict__ xsc = q0.xs;
+  double *__restrict__ incw = q0.inc;
+  const bool xs_inc = q0.xs_inc != 0;
   double *__restrict__ part = q0.part_e;
   __shared__ double lds[kJThreads / kWave];
+  __shared__ double terms[kExtrapTerms];
   const int ncoef = diis_max > 0 ? st->ncoef : 0;
   const double *xs[kDiisMax];
   double cs[kDiisMax];
@@ -471,13 +480,16 @@ __global__ __launch_bounds__(kJThreads) void k_extrap(int nylm, size_t nn, int d
     cs[q] = (q < ncoef) ? st->coef[q] : 0.0;
   }
   const double *xj = diis_max > 0 ? xd + (size_t)(*newslot) * nn : xnew;
-  const int wave = threadIdx.x / kWave, lane = lane_id();
-  const int wpb = kJThreads / kWave;
+  const int chunk = kExtrapTerms / nylm;   // spheres per pass, at least 7
+  const int step = kJThreads % nylm;       // lm advance per thread stride
   double usq = 0.0;
-  for (int i = blk.y + wave; i < blk.z; i += wpb) {
-    double s = 0.0;
-    for (int lm = lane; lm < nylm; lm += kWave) {
-      size_t k = (size_t)i * nylm + lm;
+  for (int c0 = blk.y; c0 < blk.z; c0 += chunk) {
+    const int c1 = min(blk.z, c0 + chunk);
+    const unsigned ne = (unsigned)(c1 - c0) * (unsigned)nylm;
+    const size_t e0 = (size_t)c0 * nylm;
+    int lm = (int)(threadIdx.x % (unsigned)nylm);
+    for (unsigned j = threadIdx.x; j < ne; j += kJThreads) {
+      const size_t k = e0 + j;
       double xn;
       if (ncoef > 0) {
         xn = 0.0;
@@ -487,19 +499,25 @@ __global__ __launch_bounds__(kJThreads) void k_extrap(int nylm, size_t nn, int d
       } else {
         xn = xj[k];
       }
-      double inc = xn - x[k];
+      const double inc = xn - x[k];
       x[k] = xn;
-      if (xsc) xsc[k] = cf[lm] * (q0.xs_inc ? inc : xn);
-      if (q0.inc) q0.inc[k] = inc;
-      s += invl1[lm] * inc * inc;
+      if (xsc) xsc[k] = cf[lm] * (xs_inc ? inc : xn);
+      if (incw) incw[k] = inc;
+      terms[j] = invl1[lm] * inc * inc;
+      lm += step;
+      if (lm >= nylm) lm -= nylm;
     }
-    s = wave_sum(s);
-    double u = sqrt(s);
-    usq += u * u;
+    __syncthreads();
+    for (int i = threadIdx.x; i < c1 - c0; i += kJThreads) {
+      const double *t = terms + (size_t)i * nylm;
+      double sq = 0.0;
+      for (int l = 0; l < nylm; ++l) sq += t[l];
+      const double u = sqrt(sq);
+      usq += u * u;
+    }
+    __syncthreads();
   }
-  // usq is wave-uniform: take lane 0 of each wave
-  if (lane_id() != 0) usq = 0.0;
-  double tot = block_sum_256(usq, lds);
+  const double tot = block_sum_256(usq, lds);
   if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
@@ -687,7 +705,7 @@ int energy_dot(ddx_handle *h, const double *a, const double *b, double scale, do
 // Block maps of the solver kernels, from the instances' sphere ranges: each
 // instance's elements (spheres) are split evenly over
 // ceil(elements / 512) (ceil(spheres / sp)) workgroups, at most 1024 each.
-// sp, the spheres per k_extrap workgroup, is 4 (a wave per sphere) up to
+// sp, the spheres per k_extrap workgroup, is 4 up to
 // about 8000 spheres in all and grows to 32 for larger systems and batches,
 // where the per-workgroup reads of the DIIS state would dominate.
 int build_solver_maps(ddx_handle *h) {
