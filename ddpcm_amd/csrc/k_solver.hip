@@ -93,9 +93,16 @@ __global__ __launch_bounds__(kJThreads) void k_jacobi(size_t nn, int nylm, int d
   for (int q = 0; q < kDiisMax; ++q) acc[q] = 0.0;
   double self = 0.0;
   for (size_t k = (size_t)blk.y + threadIdx.x; k < (size_t)blk.z; k += kJThreads) {
+    // element indices fit in 32 bits (build_solver_maps checks nylm nsph)
+    const unsigned c = (unsigned)k / (unsigned)nylm;
+    const unsigned lm = (unsigned)k - c * (unsigned)nylm;
+    // the history residuals are read first, with the element's other
+    // operands, so that all of them are in flight together
+    double ev[kDiisMax];
+#pragma unroll
+    for (int q = 0; q < kDiisMax; ++q) ev[q] = (q < ndot) ? eh[q][k] : 0.0;
     double yk;
     if (yp) {
-      const size_t c = k / nylm, lm = k - c * nylm;
       yk = 0.0;
       for (int j = segoff[c]; j < segoff[c + 1]; ++j) yk += yp[(size_t)j * nylm + lm];
     } else {
@@ -106,14 +113,14 @@ __global__ __launch_bounds__(kJThreads) void k_jacobi(size_t nn, int nylm, int d
       if (q0.acc) yk += q0.yacc[k];
       q0.yacc[k] = yk;
     }
-    double xn = facl[k % nylm] * (rhs[k] - yk);
+    double xn = facl[lm] * (rhs[k] - yk);
     double e = xn - x[k];
     xo[k] = xn;
     if (eo) {
       eo[k] = e;
 #pragma unroll
       for (int q = 0; q < kDiisMax; ++q)
-        if (q < ndot) acc[q] += eh[q][k] * e;
+        if (q < ndot) acc[q] += ev[q] * e;
       self += e * e;
     }
   }
