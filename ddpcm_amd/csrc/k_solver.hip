@@ -495,23 +495,45 @@ __global__ __launch_bounds__(kJThreads) void k_extrap(int nylm, size_t nn, int d
     const unsigned ne = (unsigned)(c1 - c0) * (unsigned)nylm;
     const size_t e0 = (size_t)c0 * nylm;
     int lm = (int)(threadIdx.x % (unsigned)nylm);
-    for (unsigned j = threadIdx.x; j < ne; j += kJThreads) {
-      const size_t k = e0 + j;
-      double xn;
+    // two elements per step (j and j + 256), all their loads issued before
+    // either one's stores
+    for (unsigned j = threadIdx.x; j < ne; j += 2 * kJThreads) {
+      const bool two = j + kJThreads < ne;
+      int lm2 = lm + step;
+      if (lm2 >= nylm) lm2 -= nylm;
+      const size_t k = e0 + j, k2 = two ? k + kJThreads : k;
+      double xn = 0.0, xn2 = 0.0;
       if (ncoef > 0) {
-        xn = 0.0;
+        double a[kDiisMax], b[kDiisMax];
 #pragma unroll
-        for (int q = 0; q < kDiisMax; ++q)
-          if (q < ncoef) xn = xn + cs[q] * xs[q][k];
+        for (int q = 0; q < kDiisMax; ++q) {
+          a[q] = (q < ncoef) ? xs[q][k] : 0.0;
+          b[q] = (q < ncoef) ? xs[q][k2] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < kDiisMax; ++q) {
+          if (q < ncoef) {
+            xn = xn + cs[q] * a[q];
+            xn2 = xn2 + cs[q] * b[q];
+          }
+        }
       } else {
         xn = xj[k];
+        xn2 = xj[k2];
       }
-      const double inc = xn - x[k];
+      const double x1 = x[k], x2 = x[k2];
+      const double inc = xn - x1, inc2 = xn2 - x2;
       x[k] = xn;
       if (xsc) xsc[k] = cf[lm] * (xs_inc ? inc : xn);
       if (incw) incw[k] = inc;
       terms[j] = invl1[lm] * inc * inc;
-      lm += step;
+      if (two) {
+        x[k2] = xn2;
+        if (xsc) xsc[k2] = cf[lm2] * (xs_inc ? inc2 : xn2);
+        if (incw) incw[k2] = inc2;
+        terms[j + kJThreads] = invl1[lm2] * inc2 * inc2;
+      }
+      lm = lm2 + step;
       if (lm >= nylm) lm -= nylm;
     }
     __syncthreads();
