@@ -124,6 +124,7 @@ struct FmmState {
   DevArray<int> cnt, noff, nidx;        // near charges of each sphere (CSR)
   DevArray<int> toff, tidx;             // near spheres of each charge (CSR)
   DevArray<double> A, Xm, part;         // local expansions, cf-scaled multipoles, partials
+  DevArray<int2> cmap;                  // per cavity point: (sphere, grid point)
 };
 
 // Event pairs around the launches of one kernel class (ddx_kernel_timer).

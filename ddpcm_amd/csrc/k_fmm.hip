@@ -91,9 +91,10 @@ __device__ __forceinline__ void legendre_rows(double cthe, double sthe, double c
       c += (pole ? -dp * vs : (double)m * pc[m]) * us;
     }
     f(l, a, b, c);
-    // keep the next row's LDS reads of xs after this row's use (hoisting
-    // them all would exceed the register file)
-    asm volatile("" ::: "memory");
+    // keep the next row's LDS reads of xs after this row's use, and its
+    // recurrence after this row's sums (scheduled ahead, the rows of all l
+    // would be live at once and exceed the register file)
+    asm volatile("" : "+v"(cthe), "+v"(sthe) : "v"(a), "v"(b), "v"(c) : "memory");
   }
 }
 
@@ -195,90 +196,103 @@ __global__ __launch_bounds__(kFT) void k_near_fill(int n, int mode, double rho,
   }
 }
 
-// Potential (and with ef the solute field) at the exposed points of sphere
-// i = blockIdx.x: the far part from the local expansion A_i, the near part
-// directly from the charges of its near list.
+// (sphere, grid point) of every cavity point
+__global__ void k_cav_map(size_t np, int ngrid, const int *__restrict__ cavidx,
+                          int2 *__restrict__ cmap) {
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const int c = cavidx[p];
+  if (cavidx[p + 1] > c) cmap[c] = make_int2((int)(p / ngrid), (int)(p % ngrid));
+}
+
+// Far part of the potential (and with ef of the solute field) at the cavity
+// points of sphere i = blockIdx.x: its local expansion (facs A in LDS) with
+// its gradient, one wave taking the sphere's exposed points in turn.
 template <int P>
-__global__ __launch_bounds__(kFT) void k_l2p_near(
-    int ngrid, const double4 *__restrict__ sph, const double4 *__restrict__ grid,
-    const int *__restrict__ cavidx, const double *__restrict__ A,
-    const double *__restrict__ facsP, const int *__restrict__ noff,
-    const int *__restrict__ nidx, const double *__restrict__ q,
-    const double *__restrict__ ccav, double *__restrict__ phi, double *__restrict__ ef) {
+__global__ __launch_bounds__(kWave) void k_l2p(int ngrid, const int *__restrict__ cavidx,
+                                               const int2 *__restrict__ cmap,
+                                               const double4 *__restrict__ sph,
+                                               const double4 *__restrict__ grid,
+                                               const double *__restrict__ A,
+                                               const double *__restrict__ facsP,
+                                               double *__restrict__ phi,
+                                               double *__restrict__ ef) {
   constexpr int NY = (P + 1) * (P + 1);
   __shared__ double xs[NY];
-  __shared__ double4 src[kFT];
   const int i = blockIdx.x;
-  const double4 si = sph[i];
-  for (int k = threadIdx.x; k < NY; k += kFT) xs[k] = facsP[k] * A[(size_t)i * NY + k];
+  for (int k = threadIdx.x; k < NY; k += kWave) xs[k] = facsP[k] * A[(size_t)i * NY + k];
+  __syncthreads();
   const int c0 = cavidx[(size_t)i * ngrid], c1 = cavidx[(size_t)(i + 1) * ngrid];
-  const int nb = noff[i], ne = noff[i + 1];
-  for (int p0 = 0; p0 < ngrid; p0 += kFT) {
-    const int pn = p0 + (int)threadIdx.x;
-    const size_t pt = (size_t)i * ngrid + pn;
-    const bool on = pn < ngrid && cavidx[pt + 1] > cavidx[pt];
-    const int ic = on ? cavidx[pt] : 0;
-    double v = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0;
-    double cx = 0.0, cy = 0.0, cz = 0.0;
-    __syncthreads();   // xs written; the previous chunk's src reads are done
-    if (on) {
-      const double4 g = grid[pn];
-      double cthe, sthe, cphi, sphi;
-      angles(g.x, g.y, g.z, 1.0, cthe, sthe, cphi, sphi);
-      double gr = 0.0, gt = 0.0, gp = 0.0;
-      legendre_rows<P>(cthe, sthe, cphi, sphi, xs, [&](int l, double a, double b, double c) {
-        v += a;
-        gr += (double)l * a;
-        gt += b;
-        gp += c;
-      });
+  const double ir = 1.0 / sph[i].w;
+  for (int base = c0; base < c1; base += kWave) {
+    // the barrier keeps the LDS reads of xs inside each point's Legendre
+    // rows (hoisted out of the loop, all (P+1)^2 values would need registers)
+    __syncthreads();
+    const int ic = base + (int)threadIdx.x;
+    if (ic >= c1) continue;
+    const double4 g = grid[cmap[ic].y];
+    double cthe, sthe, cphi, sphi;
+    angles(g.x, g.y, g.z, 1.0, cthe, sthe, cphi, sphi);
+    double v = 0.0, gr = 0.0, gt = 0.0, gp = 0.0;
+    legendre_rows<P>(cthe, sthe, cphi, sphi, xs, [&](int l, double a, double b, double c) {
+      v += a;
+      gr += (double)l * a;
+      gt += b;
+      gp += c;
+    });
+    phi[ic] = v;
+    if (ef) {
       double s[3], et[3], ep[3];
       frame(cthe, sthe, cphi, sphi, s, et, ep);
-      const double ir = 1.0 / si.w;
       // field = - grad phi
-      e0 = -ir * (gr * s[0] + gt * et[0] + gp * ep[0]);
-      e1 = -ir * (gr * s[1] + gt * et[1] + gp * ep[1]);
-      e2 = -ir * (gr * s[2] + gt * et[2] + gp * ep[2]);
-      cx = ccav[3 * (size_t)ic];
-      cy = ccav[3 * (size_t)ic + 1];
-      cz = ccav[3 * (size_t)ic + 2];
-    }
-    // the near charges, kFT at a time through LDS
-    for (int b0 = nb; b0 < ne; b0 += kFT) {
-      __syncthreads();
-      if (b0 + (int)threadIdx.x < ne) {
-        const int j = nidx[b0 + threadIdx.x];
-        const double4 sj = sph[j];
-        src[threadIdx.x] = make_double4(sj.x, sj.y, sj.z, q[j]);
-      }
-      __syncthreads();
-      const int cnt = min(kFT, ne - b0);
-      if (on) {
-        for (int k = 0; k < cnt; ++k) {
-          const double4 sj = src[k];
-          const double dx = cx - sj.x, dy = cy - sj.y, dz = cz - sj.z;
-          const double r2 = dx * dx + dy * dy + dz * dz;
-          const double ir = 1.0 / sqrt(r2);
-          const double qr = sj.w * ir;
-          v += qr;
-          const double f = qr * ir * ir;
-          e0 += f * dx;
-          e1 += f * dy;
-          e2 += f * dz;
-        }
-      }
-    }
-    if (on) {
-      phi[ic] = v;
-      if (ef) {
-        ef[3 * (size_t)ic] = e0;
-        ef[3 * (size_t)ic + 1] = e1;
-        ef[3 * (size_t)ic + 2] = e2;
-      }
+      ef[3 * (size_t)ic] = -ir * (gr * s[0] + gt * et[0] + gp * ep[0]);
+      ef[3 * (size_t)ic + 1] = -ir * (gr * s[1] + gt * et[1] + gp * ep[1]);
+      ef[3 * (size_t)ic + 2] = -ir * (gr * s[2] + gt * et[2] + gp * ep[2]);
     }
   }
-  (void)c0;
-  (void)c1;
+}
+
+// Near part: the charges of the sphere's near list added, in list order, to
+// the far part k_l2p left at each cavity point.
+__global__ __launch_bounds__(kFT) void k_near_pot(int ncav, const int2 *__restrict__ cmap,
+                                                  const double4 *__restrict__ sph,
+                                                  const int *__restrict__ noff,
+                                                  const int *__restrict__ nidx,
+                                                  const double *__restrict__ q,
+                                                  const double *__restrict__ ccav,
+                                                  double *__restrict__ phi,
+                                                  double *__restrict__ ef) {
+  const int ic = blockIdx.x * kFT + threadIdx.x;
+  if (ic >= ncav) return;
+  const int i = cmap[ic].x;
+  const double cx = ccav[3 * (size_t)ic], cy = ccav[3 * (size_t)ic + 1],
+               cz = ccav[3 * (size_t)ic + 2];
+  double v = phi[ic], e0 = 0.0, e1 = 0.0, e2 = 0.0;
+  if (ef) {
+    e0 = ef[3 * (size_t)ic];
+    e1 = ef[3 * (size_t)ic + 1];
+    e2 = ef[3 * (size_t)ic + 2];
+  }
+  const int ne = noff[i + 1];
+  for (int k = noff[i]; k < ne; ++k) {
+    const int j = nidx[k];
+    const double4 sj = sph[j];
+    const double dx = cx - sj.x, dy = cy - sj.y, dz = cz - sj.z;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double ir = 1.0 / sqrt(r2);
+    const double qr = q[j] * ir;
+    v += qr;
+    const double f = qr * ir * ir;
+    e0 += f * dx;
+    e1 += f * dy;
+    e2 += f * dz;
+  }
+  phi[ic] = v;
+  if (ef) {
+    ef[3 * (size_t)ic] = e0;
+    ef[3 * (size_t)ic + 1] = e1;
+    ef[3 * (size_t)ic + 2] = e2;
+  }
 }
 
 // X'_i = cf M_i, M_i,lm = sum over the exposed points n of sphere i of
@@ -470,6 +484,10 @@ int fmm_prepare(ddx_handle *h) {
                        soff, off.p, idx.p);
     DDX_HIP(hipGetLastError());
   }
+  DDX_TRY(f.cmap.reserve((size_t)h->ncav + 1));
+  hipLaunchKernelGGL(k_cav_map, dim3((unsigned)((h->np + 255) / 256)), dim3(256), 0, s, h->np,
+                     h->ngrid, h->cavidx.p, f.cmap.p);
+  DDX_HIP(hipGetLastError());
   f.lists_ok = true;
   return DDX_OK;
 }
@@ -485,10 +503,15 @@ int fmm_potential(ddx_handle *h, const double *q, double *phi, double *ef) {
   TimedLaunch tl(h, DDX_K_POTENTIAL, h->stream);
   if (tl.b) DDX_HIP(hipEventRecord(tl.b, h->stream));
   DDX_TRY(fmm_p2l(h, P, kFmmRho, q, f.cfP.p, f.A.p));
-  hipLaunchKernelGGL(k_l2p_near<P>, dim3(h->n), dim3(kFT), 0, h->stream, h->ngrid, h->sph.p,
-                     h->grid.p, h->cavidx.p, f.A.p, f.facsP.p, f.noff.p, f.nidx.p, q, h->ccav.p,
-                     phi, ef);
-  DDX_HIP(hipGetLastError());
+  if (h->ncav > 0) {
+    hipLaunchKernelGGL(k_l2p<P>, dim3(h->n), dim3(kWave), 0, h->stream, h->ngrid, h->cavidx.p,
+                       f.cmap.p, h->sph.p, h->grid.p, f.A.p, f.facsP.p, phi, ef);
+    DDX_HIP(hipGetLastError());
+    const unsigned nb = (unsigned)((h->ncav + kFT - 1) / kFT);
+    hipLaunchKernelGGL(k_near_pot, dim3(nb), dim3(kFT), 0, h->stream, h->ncav, f.cmap.p,
+                       h->sph.p, f.noff.p, f.nidx.p, q, h->ccav.p, phi, ef);
+    DDX_HIP(hipGetLastError());
+  }
   if (tl.e) DDX_HIP(hipEventRecord(tl.e, h->stream));
   return DDX_OK;
 }
