@@ -457,9 +457,17 @@ __global__ __launch_bounds__(256) void k_gather_points(size_t np, const int *__r
   if (gate && gate->done) return;
   size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  R s = R(0);
   const int b = ptoff[p], e = ptoff[p + 1];
-  for (int k = b; k < e; ++k) s += rw[k];
+  // the first four terms are loaded together (most points have at most
+  // four), then added in order like the rest
+  R v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = (b + k < e) ? rw[b + k] : R(0);
+  R s = R(0);
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (b + k < e) s += v[k];
+  for (int k = b + 4; k < e; ++k) s += rw[k];
   P[p] = s;
 }
 
