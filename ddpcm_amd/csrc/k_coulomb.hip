@@ -1,10 +1,11 @@
 // All-pairs Coulomb sums for point-charge solutes.
 //   potential  phi(j) = sum_i q_i / |r_j - r_i|            (mkrhs.f90:102-114)
 //   field      E(j)   = sum_i q_i (r_j - r_i) / |r_j - r_i|^3  (efld.f90:98-114)
-// Each thread owns one target; sources stream through LDS in tiles of 256.
-// Work comes as a table of tiles (targets tb..te-1, at most 256, and sources
-// sb..se-1): for a batch of molecules every tile stays inside one
-// molecule, so each sums only its own molecule's charges.
+// Each thread owns one or two targets; sources stream through LDS.  Work
+// comes as a table of tiles (targets tb..te-1, at most 512 for the potential
+// and 64 for the tiled field, and sources sb..se-1): for a batch of molecules
+// every tile stays inside one molecule, so each sums only its own molecule's
+// charges.
 // The potential kernel can return the field of the same sources as well
 // (the solute field at the cavity points that the forces contract with
 // zeta), which costs one all-pairs pass less than a separate field kernel.
@@ -26,6 +27,7 @@ namespace {
 
 constexpr int kCT = 256;          // threads per workgroup, sources per LDS tile
 constexpr int kTW = 2 * kCT;      // targets per tile: two per thread
+constexpr int kFW = kWave;        // targets per tile of the tiled field: one per lane
 
 // 1/sqrt(x) for x > 0: v_rsq_f64 (relative error below 2^-24 on gfx950)
 // and one third-order correction: with e = 1 - x y^2, y (1 + e/2 + 3e^2/8)
@@ -175,27 +177,8 @@ __device__ __forceinline__ void field_tile(const int4 tl, const double *__restri
   e[5] = eb2;
 }
 
-// partial fields of tile (tb, te, sb, se) at its targets (two per thread):
-// out[(b kTW + t) 3]
-__global__ __launch_bounds__(kCT) void k_field(const int4 *__restrict__ tiles,
-                                               const double *__restrict__ q,
-                                               const double *__restrict__ src,
-                                               const double *__restrict__ trg,
-                                               double *__restrict__ out) {
-  __shared__ double4 tile[kCT];
-  double e[6];
-  field_tile(tiles[blockIdx.x], q, src, trg, tile, e);
-  const size_t o = ((size_t)blockIdx.x * kTW + threadIdx.x) * 3, o1 = o + 3 * (size_t)kCT;
-  out[o] = e[0];
-  out[o + 1] = e[1];
-  out[o + 2] = e[2];
-  out[o1] = e[3];
-  out[o1 + 1] = e[4];
-  out[o1 + 2] = e[5];
-}
-
-// ef of the targets of group g = sum of the partials of its tiles
-// grp[g] .. grp[g+1]-1 (chunks of one target tile), in a fixed order: each
+// ef of the targets of a target tile = sum of the partials of its chunk
+// tiles b0 .. b1-1 (kTW targets each), in a fixed order: each
 // target's chunks are split over kRq threads (chunk c to thread c mod kRq),
 // whose sums meet in LDS in thread order.  blockIdx.y picks the slice of
 // kCT / kRq targets.
@@ -230,14 +213,85 @@ __device__ __forceinline__ void field_reduce(int b0, int b1, int tb, int te,
   }
 }
 
-__global__ __launch_bounds__(kCT) void k_field_reduce(const int *__restrict__ grp,
-                                                      const int4 *__restrict__ tiles,
-                                                      const double *__restrict__ part,
-                                                      double *__restrict__ ef) {
-  __shared__ double red[kCT][3];
+// The tiled field (few targets per molecule: the nuclei): one wave per tile
+// of at most kFW targets, one per lane, the tile's sources through LDS kFW at
+// a time; part[(b kFW + t) 3] holds tile b's partial field at target t.
+__global__ __launch_bounds__(kFW) void k_field_w(const int4 *__restrict__ tiles,
+                                                 const double *__restrict__ q,
+                                                 const double *__restrict__ src,
+                                                 const double *__restrict__ trg,
+                                                 double *__restrict__ part) {
+  __shared__ double4 tile[kFW];
+  const int4 tl = tiles[blockIdx.x];
+  const int j = tl.x + (int)threadIdx.x;
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  if (j < tl.y) {
+    ax = trg[3 * (size_t)j];
+    ay = trg[3 * (size_t)j + 1];
+    az = trg[3 * (size_t)j + 2];
+  }
+  double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+  for (int i0 = tl.z; i0 < tl.w; i0 += kFW) {
+    const int i = i0 + (int)threadIdx.x;
+    __syncthreads();   // the previous sources are read
+    if (i < tl.w)
+      tile[threadIdx.x] = make_double4(src[3 * (size_t)i], src[3 * (size_t)i + 1],
+                                       src[3 * (size_t)i + 2], q[i]);
+    __syncthreads();
+    const int cnt = min(kFW, tl.w - i0);
+    for (int k = 0; k < cnt; ++k) {
+      const double4 sk = tile[k];
+      const double dx = ax - sk.x, dy = ay - sk.y, dz = az - sk.z;
+      const double y = rsqrt_nr(dx * dx + dy * dy + dz * dz);
+      const double f = sk.w * (y * y * y);
+      e0 = e0 + f * dx;
+      e1 = e1 + f * dy;
+      e2 = e2 + f * dz;
+    }
+  }
+  if (j >= tl.y) e0 = e1 = e2 = 0.0;
+  const size_t o = ((size_t)blockIdx.x * kFW + threadIdx.x) * 3;
+  part[o] = e0;
+  part[o + 1] = e1;
+  part[o + 2] = e2;
+}
+
+// ef of the targets of group g = the partials of its tiles grp[g] ..
+// grp[g+1]-1 (the source chunks of one target tile) in a fixed order: each
+// target's chunks are split over kRw threads (chunk c to thread c mod kRw),
+// whose sums meet in LDS in thread order.
+constexpr int kRw = 4;
+__global__ __launch_bounds__(kFW * kRw) void k_field_reduce_w(const int *__restrict__ grp,
+                                                              const int4 *__restrict__ tiles,
+                                                              const double *__restrict__ part,
+                                                              double *__restrict__ ef) {
+  __shared__ double red[kFW * kRw][3];
   const int b0 = grp[blockIdx.x], b1 = grp[blockIdx.x + 1];
   const int4 tl = tiles[b0];
-  field_reduce(b0, b1, tl.x, tl.y, part, ef, red);
+  const int r = threadIdx.x % kRw, t = threadIdx.x / kRw;
+  double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+  for (int b = b0 + r; b < b1; b += kRw) {
+    const size_t o = ((size_t)b * kFW + t) * 3;
+    e0 += part[o];
+    e1 += part[o + 1];
+    e2 += part[o + 2];
+  }
+  red[threadIdx.x][0] = e0;
+  red[threadIdx.x][1] = e1;
+  red[threadIdx.x][2] = e2;
+  __syncthreads();
+  const int j = tl.x + t;
+  if (r == 0 && j < tl.y) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int k = 0; k < kRw; ++k) {
+      s0 += red[threadIdx.x + k][0];
+      s1 += red[threadIdx.x + k][1];
+      s2 += red[threadIdx.x + k][2];
+    }
+    ef[3 * (size_t)j] = s0;
+    ef[3 * (size_t)j + 1] = s1;
+    ef[3 * (size_t)j + 2] = s2;
+  }
 }
 
 // The same field on a uniform tiling that needs no table: tile b covers
@@ -308,24 +362,27 @@ __global__ void k_psi(int n, int nylm, double fac, const double *__restrict__ q,
 // tile's chunks.
 void coulomb_tiles(const std::vector<int> &toff, const std::vector<int> &soff, bool split,
                    std::vector<int4> &tiles, std::vector<int> &grp) {
+  // split: the tiled field, kFW targets per tile (one wave) and the sources
+  // cut into chunks; otherwise kTW targets and all of the molecule's sources
+  const int tw = split ? kFW : kTW;
   const size_t ninst = toff.size() - 1;
   long long work = 0;   // target tiles x sources
   for (size_t q = 0; q < ninst; ++q)
-    work += (long long)((toff[q + 1] - toff[q] + kTW - 1) / kTW) * (soff[q + 1] - soff[q]);
+    work += (long long)((toff[q + 1] - toff[q] + tw - 1) / tw) * (soff[q + 1] - soff[q]);
   long long chunk = 0x7fffffff;
   if (split) {
-    // about 8192 tiles: several rounds of the resident workgroups, so that
-    // the uneven last chunks of each target tile do not leave CUs idle
-    chunk = (work + 8191) / 8192;
-    chunk = ((chunk + kCT - 1) / kCT) * kCT;
-    if (chunk < 4 * kCT) chunk = 4 * kCT;
+    // about 32768 one-wave tiles: several rounds of the resident waves, so
+    // that the uneven last chunks of each target tile do not leave CUs idle
+    chunk = (work + 32767) / 32768;
+    chunk = ((chunk + kFW - 1) / kFW) * kFW;
+    if (chunk < 4 * kFW) chunk = 4 * kFW;
   }
   tiles.clear();
   grp.clear();
   for (size_t q = 0; q < ninst; ++q) {
     const int sb = soff[q], se = soff[q + 1];
-    for (int tb = toff[q]; tb < toff[q + 1]; tb += kTW) {
-      const int te = std::min(tb + kTW, toff[q + 1]);
+    for (int tb = toff[q]; tb < toff[q + 1]; tb += tw) {
+      const int te = std::min(tb + tw, toff[q + 1]);
       grp.push_back((int)tiles.size());
       if (se <= sb) {
         tiles.push_back(make_int4(tb, te, sb, sb));
@@ -384,16 +441,15 @@ int coulomb_potential(ddx_handle *h, const int4 *tiles, int ntile, const double 
 int coulomb_field(ddx_handle *h, const int4 *tiles, int ntile, const int *grp, int ngrp,
                   const double *q, const double *src3, const double *trg3, double *ef) {
   if (ntile <= 0 || ngrp <= 0) return DDX_OK;
-  DDX_TRY(h->fpart.reserve((size_t)ntile * kTW * 3));
+  DDX_TRY(h->fpart.reserve((size_t)ntile * kFW * 3));
   {
     TimedLaunch tl(h, DDX_K_FIELD, h->stream);
-    hipExtLaunchKernelGGL(k_field, dim3(ntile), dim3(kCT), 0, h->stream, tl.b, tl.e, 0, tiles, q,
-                          src3, trg3, h->fpart.p);
+    hipExtLaunchKernelGGL(k_field_w, dim3(ntile), dim3(kFW), 0, h->stream, tl.b, tl.e, 0, tiles,
+                          q, src3, trg3, h->fpart.p);
   }
   DDX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_field_reduce, dim3(ngrp, kTW / (kCT / kRq)), dim3(kCT), 0, h->stream, grp,
-                     tiles, h->fpart.p,
-                     ef);
+  hipLaunchKernelGGL(k_field_reduce_w, dim3(ngrp), dim3(kFW * kRw), 0, h->stream, grp, tiles,
+                     h->fpart.p, ef);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
