@@ -35,6 +35,10 @@ constexpr int kPartStride = kDiisMax + 1;   // 25 dots + self dot per block
 // mixed precision: every kRefresh-th round applies the FP64 operator to the
 // iterate itself (the FP64 image of the iterate restarts exactly there)
 constexpr int kRefresh = 8;
+// rounds per captured graph group (a multiple of kRefresh, so that every
+// group of a mixed-precision solve starts at the same phase and one graph
+// serves them all)
+constexpr int kGraphRounds = 32;
 
 // Pointers of one system's Jacobi/DIIS iteration; the kernels below take
 // one or two systems (the paired direct and adjoint solves), selected by
@@ -1001,8 +1005,11 @@ void drop_round_graphs(ddx_handle *h) {
 
 // Enqueues the next group of rounds and the copy of each lane's state header
 // (nmat, it, done, ok) into slot `slot` of its pinned h_head, marked by
-// h->gev[slot].  The first group of a solve runs eagerly; with graphs on
-// (and no kernel timer) the following groups are one graph launch each.
+// h->gev[slot].  The first group of a solve (3 rounds) runs eagerly and
+// makes every buffer the rounds use; eager groups after it have 2 rounds.
+// With graphs on (and no kernel timer) each later group is one launch of a
+// captured graph of kGraphRounds rounds, which covers the rest of a typical
+// solve; the rounds after convergence return at once.
 int solve_enqueue(ddx_handle *h, Lane *const *lanes, int count, int slot) {
   Lane &l0 = *lanes[0];
   const bool pair = count == 2;
@@ -1013,8 +1020,9 @@ int solve_enqueue(ddx_handle *h, Lane *const *lanes, int count, int slot) {
   }
   hipStream_t s = l0.stream;
   h->tag_solve = l0.serial;
-  const int nr = std::min(l0.group, l0.rounds - l0.enqueued);
-  if (h->use_graphs && l0.group == 2 && nr == 2 && !timers_on(h))
+  const bool graph = h->use_graphs && l0.enqueued > 0 && !timers_on(h);
+  const int nr = std::min(graph ? kGraphRounds : l0.group, l0.rounds - l0.enqueued);
+  if (graph)
     DDX_TRY(launch_round_graph(h, lanes, count, nr));
   else
     DDX_TRY(enqueue_rounds(h, lanes, count, nr));
@@ -1044,19 +1052,31 @@ int solve_end(ddx_handle *h, Lane &ln, int *n_iter, int *ok, std::vector<double>
 int solve_run(ddx_handle *h, Lane *const *lanes, int count) {
   const int serial = h->solve_serial++;
   for (int k = 0; k < count; ++k) lanes[k]->serial = serial;
-  // one group always runs ahead: while the host waits for the state of
-  // group k, group k + 1 is already queued behind it, so the device does not
-  // wait for the host between groups (a group queued after convergence
-  // returns at once)
-  int cur = 0;
+  // one eager group always runs ahead: while the host waits for the state
+  // of group k, group k + 1 is already queued behind it, so the device does
+  // not wait for the host between groups (a group queued after convergence
+  // returns at once).  Of the graph groups only the first runs ahead (behind
+  // the eager first group): a later one would mostly repeat finished rounds.
+  int cur = 0, queued = 1, groups = 1;
   DDX_TRY(solve_enqueue(h, lanes, count, cur));
   while (true) {
-    const bool more = lanes[0]->enqueued < lanes[0]->rounds;
-    if (more) DDX_TRY(solve_enqueue(h, lanes, count, cur ^ 1));
+    if (lanes[0]->enqueued < lanes[0]->rounds && queued == 1 &&
+        (!h->use_graphs || groups == 1)) {
+      DDX_TRY(solve_enqueue(h, lanes, count, cur ^ 1));
+      ++queued;
+      ++groups;
+    }
     DDX_HIP(hipEventSynchronize(h->gev[cur]));
+    --queued;
     bool all = true;
     for (int k = 0; k < count; ++k) all = all && lanes[k]->h_head[4 * cur + 2] != 0;
-    if (all || !more) break;
+    if (all) break;
+    if (queued == 0) {
+      if (lanes[0]->enqueued >= lanes[0]->rounds) break;
+      DDX_TRY(solve_enqueue(h, lanes, count, cur ^ 1));
+      ++queued;
+      ++groups;
+    }
     cur ^= 1;
   }
   for (int k = 0; k < count; ++k) {

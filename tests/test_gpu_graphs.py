@@ -1,11 +1,11 @@
 """Graph replay of the solver's iteration groups (ddx_set_graphs).
 
-With graphs on, every group of solver rounds after the first is one launch of
-a HIP graph captured on first use.  The launches inside are the eager ones,
+With graphs on, the solver's rounds after the first eager group are launches
+of a HIP graph of 32 rounds captured on first use (one launch covers the rest
+of a solve of up to 35 iterations).  The launches inside are the eager ones,
 so every result must be bitwise equal to the eager run.  Each test replays
-the captured graphs many times (one solve is a dozen or more groups), with
-eager solves, new charges and new geometries in between, and checks every
-replay against the eager result.
+the captured graphs many times, with eager solves, new charges and new
+geometries in between, and checks every replay against the eager result.
 """
 import numpy as np
 import pytest
