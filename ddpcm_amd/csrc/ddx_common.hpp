@@ -114,6 +114,10 @@ struct GuessWork {
 constexpr int kFmmOrder = 16;        // the potential at the cavity points (enters E)
 constexpr int kFmmFieldOrder = 10;   // the field of zeta at the nuclei (forces only)
 constexpr double kFmmRho = 0.2;
+// per-term truncation bound of the field's adaptive orders: a far pair
+// (t = R / d) is expanded to the smallest order p with t^(p+1) below it (at
+// most kFmmFieldOrder, 2e-8 = 0.2^11), a wave to the largest its pairs need
+constexpr double kFmmFieldEps = 2e-10;
 constexpr int kFmmMinSpheres = 8000;
 constexpr double kFmmMinPoints = 100.0;   // exposed points per sphere
 struct FmmState {
@@ -125,6 +129,7 @@ struct FmmState {
   DevArray<int> toff, tidx;             // near spheres of each charge (CSR)
   DevArray<double> A, Xm, part;         // local expansions, cf-scaled multipoles, partials
   DevArray<int2> cmap;                  // per cavity point: (sphere, grid point)
+  DevArray<int> perm;                   // the spheres in Morton order of their centres
 };
 
 // Event pairs around the launches of one kernel class (ddx_kernel_timer).

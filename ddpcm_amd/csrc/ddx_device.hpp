@@ -115,6 +115,17 @@ __device__ __forceinline__ int wave_sum_int(int v) {
   return v;
 }
 
+// largest value over the wave (every lane receives it)
+__device__ __forceinline__ int wave_max_int(int v) {
+  v = max(v, partner<0>(v));
+  v = max(v, partner<1>(v));
+  v = max(v, partner<2>(v));
+  v = max(v, partner<3>(v));
+  v = max(v, partner<4>(v));
+  v = max(v, partner<5>(v));
+  return v;
+}
+
 // number of set ballot bits below this lane
 __device__ __forceinline__ int ballot_prefix(unsigned long long b) {
   unsigned long long below = (lane_id() == 0) ? 0ull : (~0ull >> (64 - lane_id()));

@@ -20,17 +20,24 @@
 //           with cf M, whose gradient is (1/|u|^2) sum_l t^l (-(l+1) A_l u^
 //           + grad_s A_l)
 //     near  direct sums over the cavity points of the near spheres
-// Every term of the truncated expansions is below (charge / (d - R))
-// rho^(P+1).  Each pass takes the order its error budget asks for: the
-// potential enters the energy (parity 1e-10 relative) and takes P = 16,
-// 1.3e-12 of each far term at rho = 0.2; the field of zeta at the nuclei
-// enters only the forces (tolerance 1e-7 Eh/bohr) and takes P = 10, 2e-8 of
-// each far term, about 1e-9 Eh/bohr in the forces.  All sums run in a fixed
-// order: results are deterministic.
+// Truncated at order p, the expansion of a far pair errs by less than
+// (charge / (d - R)) t^(p+1), t = R / d <= rho.  Each pass takes the order
+// its error budget asks for.  The potential enters the energy (parity 1e-10
+// relative) and takes P = 16: 1.3e-12 of each far term at t = 0.2.  The
+// field of zeta at the nuclei enters only the forces (tolerance 1e-7
+// Eh/bohr): its cap is P = 10 (2e-8 of a term at t = 0.2), and a pair needs
+// only the smallest order with t^(p+1) below 2e-10; the nuclei are taken in
+// Morton order of their centres, so that the 64 targets of a wave need
+// similar orders, and the wave expands to the largest of them (most pairs
+// need 5..7).  All sums run in a fixed order: results are deterministic.
 #include "ddx_device.hpp"
 #include "ddx_kernels.hpp"
 #include "tables.hpp"
 
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <utility>
 #include <vector>
 
 namespace ddx {
@@ -46,10 +53,10 @@ __host__ __device__ constexpr int lmi(int l, int m) { return l * l + l + m; }
 // gradient of A_l (e_phi carrying 1/sin th), u_c = cos(m ph) xs_l,m +
 // sin(m ph) xs_l,-m, u_s = cos(m ph) xs_l,-m - sin(m ph) xs_l,m.  The poles
 // take the limits of dbasis (ddcosmo.f90:970-971).  Three rows of P_l^m in
-// registers, l outer.
+// registers, l outer; rows past lcut (wave-uniform) are left out.
 template <int P, typename F>
 __device__ __forceinline__ void legendre_rows(double cthe, double sthe, double cphi, double sphi,
-                                              const double *xs, F &&f) {
+                                              const double *xs, F &&f, int lcut = P) {
   double cs[P + 1], sn[P + 1];
   cs[0] = 1.0;
   sn[0] = 0.0;
@@ -66,6 +73,7 @@ __device__ __forceinline__ void legendre_rows(double cthe, double sthe, double c
   double rows[2][P + 1];
 #pragma unroll
   for (int l = 0; l <= P; ++l) {
+    if (l > lcut) continue;   // (not a break: the loop stays unrolled)
     double *pc = rows[l & 1];
     const double *p1 = rows[(l + 1) & 1];
 #pragma unroll
@@ -318,18 +326,22 @@ __global__ __launch_bounds__(kFT) void k_multipole(int ngrid, const int *__restr
   }
 }
 
-// The far field at the nuclei: targets blockIdx.x * kFT + t, source spheres
-// of chunk blockIdx.y (spheres sb .. se - 1 of the chunk, clipped to the
-// target block's instance when inst is given); partial fields per chunk.
+// The far field at the nuclei: targets perm[blockIdx.x * kFT + t] (the
+// nuclei in Morton order, so that a wave's targets lie close together),
+// source spheres of chunk blockIdx.y (spheres sb .. se - 1); partial fields
+// per chunk.  A far pair (t = R_i / |u|) needs the order p with
+// t^(p+1) <= exp(leps); each wave expands to the largest its pairs need.
 template <int P>
 __global__ __launch_bounds__(kFT) void k_m2p(int n, int nchunk, double rho,
                                              const double4 *__restrict__ sph,
+                                             const int *__restrict__ perm, float leps,
                                              const double *__restrict__ Xm,
                                              double *__restrict__ part) {
   constexpr int NY = (P + 1) * (P + 1);
   __shared__ double xs[NY];
   __shared__ double4 si_sh;
-  const int j = blockIdx.x * kFT + threadIdx.x;
+  const int jj = blockIdx.x * kFT + threadIdx.x;
+  const int j = jj < n ? perm[jj] : n;
   const int per = (n + nchunk - 1) / nchunk;
   const int sb = blockIdx.y * per, se = min(n, sb + per);
   double4 tj = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -344,18 +356,30 @@ __global__ __launch_bounds__(kFT) void k_m2p(int n, int nchunk, double rho,
     const double ux = tj.x - si.x, uy = tj.y - si.y, uz = tj.z - si.z;
     const double u2 = dist2(tj.x, tj.y, tj.z, si.x, si.y, si.z);
     const double rc = si.w / rho;
-    if (j < n && u2 >= rc * rc) {
-      const double ru = sqrt(u2), iu = 1.0 / ru;
+    const bool far = j < n && u2 >= rc * rc;
+    double ru = 0.0, iu = 0.0, t = 0.0;
+    int need = -1;
+    if (far) {
+      ru = sqrt(u2);
+      iu = 1.0 / ru;
+      t = si.w * iu;
+      const int p = (int)ceilf(leps / __logf((float)t)) - 1;
+      need = p < 0 ? 0 : (p > P ? P : p);
+    }
+    const int pw = wave_max_int(need);
+    if (far) {
       double cthe, sthe, cphi, sphi;
       angles(ux, uy, uz, ru, cthe, sthe, cphi, sphi);
-      const double t = si.w * iu;
       double gr = 0.0, gt = 0.0, gp = 0.0, tl = 1.0;
-      legendre_rows<P>(cthe, sthe, cphi, sphi, xs, [&](int l, double a, double b, double c) {
-        gr += tl * (-(double)(l + 1) * a);
-        gt += tl * b;
-        gp += tl * c;
-        tl *= t;
-      });
+      legendre_rows<P>(
+          cthe, sthe, cphi, sphi, xs,
+          [&](int l, double a, double b, double c) {
+            gr += tl * (-(double)(l + 1) * a);
+            gt += tl * b;
+            gp += tl * c;
+            tl *= t;
+          },
+          pw);
       double s[3], et[3], ep[3];
       frame(cthe, sthe, cphi, sphi, s, et, ep);
       const double w = iu * iu;   // field = - grad
@@ -484,6 +508,39 @@ int fmm_prepare(ddx_handle *h) {
                        soff, off.p, idx.p);
     DDX_HIP(hipGetLastError());
   }
+  // the spheres in Morton order of their centres (30-bit codes over the
+  // bounding box), for the adaptive orders of the field pass
+  {
+    std::vector<double4> c(n);
+    DDX_HIP(hipMemcpyAsync(c.data(), h->sph.p, n * sizeof(double4), hipMemcpyDeviceToHost, s));
+    DDX_HIP(hipStreamSynchronize(s));
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (const double4 &v : c) {
+      const double x[3] = {v.x, v.y, v.z};
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = std::min(lo[d], x[d]);
+        hi[d] = std::max(hi[d], x[d]);
+      }
+    }
+    const double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
+    std::vector<std::pair<uint32_t, int>> key(n);
+    for (int i = 0; i < n; ++i) {
+      const double x[3] = {c[i].x, c[i].y, c[i].z};
+      uint32_t code = 0;
+      uint32_t b[3];
+      for (int d = 0; d < 3; ++d)
+        b[d] = (uint32_t)std::min(1023.0, std::max(0.0, (x[d] - lo[d]) / ext * 1024.0));
+      for (int bit = 9; bit >= 0; --bit)
+        for (int d = 0; d < 3; ++d) code = (code << 1) | ((b[d] >> bit) & 1u);
+      key[i] = {code, i};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int> perm(n);
+    for (int i = 0; i < n; ++i) perm[i] = key[i].second;
+    DDX_TRY(f.perm.reserve((size_t)n));
+    DDX_HIP(hipMemcpyAsync(f.perm.p, perm.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+    DDX_HIP(hipStreamSynchronize(s));
+  }
   DDX_TRY(f.cmap.reserve((size_t)h->ncav + 1));
   hipLaunchKernelGGL(k_cav_map, dim3((unsigned)((h->np + 255) / 256)), dim3(256), 0, s, h->np,
                      h->ngrid, h->cavidx.p, f.cmap.p);
@@ -535,7 +592,7 @@ int fmm_zeta_field(ddx_handle *h, const double *zeta, double *ez) {
                      h->cavidx.p, zeta, f.Yq.p, f.cfQ.p, f.Xm.p);
   DDX_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_m2p<P>, dim3(tb, nchunk), dim3(kFT), 0, s, n, nchunk, kFmmRho, h->sph.p,
-                     f.Xm.p, f.part.p);
+                     f.perm.p, (float)std::log(kFmmFieldEps), f.Xm.p, f.part.p);
   DDX_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_near_field, dim3(n), dim3(kFT), 0, s, n, h->ngrid, nchunk, h->sph.p,
                      h->cavidx.p, h->ccav.p, zeta, f.toff.p, f.tidx.p, f.part.p, ez);
