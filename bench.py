@@ -415,7 +415,7 @@ def batch_bench(args):
         _upload(h, tb)
         for _ in range(2):
             h.run()
-            train_loss.append(h.guess_train(steps=8, lr=2e-3))
+            train_loss.append(h.guess_train(steps=16, lr=2e-3))
     _upload(h, decks)
     h.set_precision(1)
     mixed, es, fx = _timed_leg(h, args, iters_fn, 1)
@@ -460,7 +460,7 @@ def main():
     ap.add_argument("--workload", default="c4")
     ap.add_argument("--train", type=int, default=200,
                     help="c5: frames whose solutions train the guess network")
-    ap.add_argument("--train-batches", type=int, default=6,
+    ap.add_argument("--train-batches", type=int, default=24,
                     help="c4: batches of other molecules that train the guess network")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
                     help="precision of the timed solves; an fp64 run also times the mixed "
