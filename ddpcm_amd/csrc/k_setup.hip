@@ -216,72 +216,129 @@ __global__ __launch_bounds__(kSetupThreads) void k_ccav(const double4 *__restric
   ccav[3 * (size_t)ic + 2] = si.z + si.w * g.z;
 }
 
-// One wave per neighbour pair q = (container c = prow[q], owner o = nl[q]):
-// the number of points of o inside c.
-__global__ __launch_bounds__(kSetupThreads) void k_pair_count(
-    int nnl, int ngrid, const int *__restrict__ nl, const int *__restrict__ rev,
-    const unsigned long long *__restrict__ tmask, int *__restrict__ paircnt) {
-  int q = blockIdx.x * (kSetupThreads / kWave) + threadIdx.x / kWave;
-  if (q >= nnl) return;
-  int o = nl[q], kr = rev[q];
-  if (kr < 0) {   // cannot happen: the overlap test is symmetric
-    if (lane_id() == 0) paircnt[q] = 0;
-    return;
-  }
-  int cnt = 0;
-  for (int n = lane_id(); n < ngrid; n += kWave)
-    cnt += (int)((tmask[(size_t)o * ngrid + n] >> kr) & 1ull);
-  cnt = wave_sum_int(cnt);
-  if (lane_id() == 0) paircnt[q] = cnt;
+// OR of a 64-bit value over the wave (every lane receives it)
+__device__ __forceinline__ unsigned long long wave_or64(unsigned long long v) {
+  unsigned lo = (unsigned)v, hi = (unsigned)(v >> 32);
+  lo |= partner_u32<0>(lo); hi |= partner_u32<0>(hi);
+  lo |= partner_u32<1>(lo); hi |= partner_u32<1>(hi);
+  lo |= partner_u32<2>(lo); hi |= partner_u32<2>(hi);
+  lo |= partner_u32<3>(lo); hi |= partner_u32<3>(hi);
+  lo |= partner_u32<4>(lo); hi |= partner_u32<4>(hi);
+  lo |= partner_u32<5>(lo); hi |= partner_u32<5>(hi);
+  return ((unsigned long long)hi << 32) | lo;
 }
 
-__global__ __launch_bounds__(kSetupThreads) void k_pair_fill(
-    int nnl, int ngrid, double eta, const double4 *__restrict__ sph,
-    const double4 *__restrict__ grid, const int *__restrict__ nl,
-    const int *__restrict__ prow, const int *__restrict__ rev,
-    const unsigned long long *__restrict__ tmask, const double *__restrict__ fi,
-    const int *__restrict__ pairoff, const int *__restrict__ ptoff,
-    double4 *__restrict__ tgeo, double2 *__restrict__ ttw, int *__restrict__ tp,
-    int *__restrict__ pttri, int *__restrict__ tslot) {
-  int q = blockIdx.x * (kSetupThreads / kWave) + threadIdx.x / kWave;
-  if (q >= nnl) return;
-  int c = prow[q], o = nl[q], kr = rev[q];
-  if (kr < 0) return;
-  double4 sc = sph[c], so = sph[o];
-  int run = pairoff[q];
-  unsigned long long below = (1ull << kr) - 1ull;
+// The pair (container nl[inl[o] + kr], owner o) of owner o's neighbour kr,
+// as an index of the container-major pair list: the owner's rank in the
+// container's list is rev of the mirrored pair.
+__device__ __forceinline__ int pair_of(const int *__restrict__ inl, const int *__restrict__ nl,
+                                       const int *__restrict__ rev, int o, int kr) {
+  const int q = inl[o] + kr;
+  return inl[nl[q]] + rev[q];
+}
+
+// One wave per owner sphere o: the number of its points inside each of its
+// neighbours (lane kr counts neighbour kr), in one pass over its masks.
+__global__ __launch_bounds__(kSetupThreads) void k_pair_count(
+    int n, int ngrid, const int *__restrict__ inl, const int *__restrict__ nl,
+    const int *__restrict__ rev, const unsigned long long *__restrict__ tmask,
+    int *__restrict__ paircnt) {
+  const int o = blockIdx.x * (kSetupThreads / kWave) + threadIdx.x / kWave;
+  if (o >= n) return;
+  const int lane = lane_id();
+  const int deg = inl[o + 1] - inl[o];
+  int cnt = 0;
   for (int n0 = 0; n0 < ngrid; n0 += kWave) {
-    int n = n0 + lane_id();
-    size_t p = (size_t)o * ngrid + n;
-    unsigned long long m = (n < ngrid) ? tmask[p] : 0ull;
-    bool hit = (m >> kr) & 1ull;
-    unsigned long long b = __ballot(hit);
-    if (hit) {
-      int idx = run + ballot_prefix(b);
-      double4 g = grid[n];
-      double v[3];
-      point_vec(so, g, sc, v);
-      double vv = norm3(v);
-      double t = vv / sc.w;
-      double s0 = v[0] / vv, s1 = v[1] / vv, s2 = v[2] / vv;
-      double cthe = s2;
-      double sthe = sqrt(1.0 - cthe * cthe);
-      double cphi = 1.0, sphi = 0.0;
-      if (sthe != 0.0) {
-        cphi = s0 / sthe;
-        sphi = s1 / sthe;
-      }
-      double chi = fsw(t, eta);
-      double f = fi[p];
-      double w = (f > 1.0) ? chi / f : chi;
-      tgeo[idx] = make_double4(cthe, sthe, cphi, sphi);
-      ttw[idx] = make_double2(t, w);
-      tp[idx] = (int)p;
-      const int slot = ptoff[p] + __popcll(m & below);
-      pttri[slot] = idx;
-      tslot[idx] = slot;
+    const int pn = n0 + lane;
+    const unsigned long long m = pn < ngrid ? tmask[(size_t)o * ngrid + pn] : 0ull;
+    unsigned long long u = wave_or64(m);
+    while (u) {
+      const int kr = __ffsll((long long)u) - 1;
+      u &= u - 1;
+      const int c = __popcll(__ballot((m >> kr) & 1ull));
+      if (lane == kr) cnt += c;
     }
-    run += __popcll(b);
+  }
+  if (lane < deg) paircnt[pair_of(inl, nl, rev, o, lane)] = cnt;
+}
+
+// One wave per owner sphere o, in one pass over its masks: for each point n
+// and each neighbour c containing it, the triple's place in c's run of
+// owner o (points ascending), its point index (tp) and its slot in the
+// point's list (pttri, tslot).  Lane kr keeps the next place of neighbour
+// kr's run.
+__global__ __launch_bounds__(kSetupThreads) void k_pair_fill(
+    int n, int ngrid, const int *__restrict__ inl, const int *__restrict__ nl,
+    const int *__restrict__ rev, const unsigned long long *__restrict__ tmask,
+    const int *__restrict__ pairoff, const int *__restrict__ ptoff, int *__restrict__ tp,
+    int *__restrict__ pttri, int *__restrict__ tslot) {
+  const int o = blockIdx.x * (kSetupThreads / kWave) + threadIdx.x / kWave;
+  if (o >= n) return;
+  const int lane = lane_id();
+  const int deg = inl[o + 1] - inl[o];
+  int run = 0;
+  if (lane < deg) run = pairoff[pair_of(inl, nl, rev, o, lane)];
+  for (int n0 = 0; n0 < ngrid; n0 += kWave) {
+    const int pn = n0 + lane;
+    const size_t p = (size_t)o * ngrid + pn;
+    const unsigned long long m = pn < ngrid ? tmask[p] : 0ull;
+    unsigned long long u = wave_or64(m);
+    while (u) {
+      const int kr = __ffsll((long long)u) - 1;
+      u &= u - 1;
+      const bool hit = (m >> kr) & 1ull;
+      const unsigned long long b = __ballot(hit);
+      const int base = __shfl(run, kr, kWave);
+      if (hit) {
+        const int idx = base + ballot_prefix(b);
+        const int slot = ptoff[p] + __popcll(m & ((1ull << kr) - 1ull));
+        tp[idx] = (int)p;
+        pttri[slot] = idx;
+        tslot[idx] = slot;
+      }
+      if (lane == kr) run += __popcll(b);
+    }
+  }
+}
+
+// The record of triple (owner point p = o ngrid + n, container c), as calcv
+// and adjrhs evaluate it (ddcosmo.f90:1707-1727, 1325-1345; the angles as
+// ylmbas takes them, ddcosmo.f90:845-857, with cos phi = 1 on the axis).
+__device__ __forceinline__ void triple_record(const double4 &so, const double4 &g,
+                                              const double4 &sc, double f, double eta,
+                                              double4 &geo, double2 &tw) {
+  double v[3];
+  point_vec(so, g, sc, v);
+  const double vv = norm3(v);
+  const double t = vv / sc.w;
+  const double s0 = v[0] / vv, s1 = v[1] / vv, s2 = v[2] / vv;
+  const double cthe = s2;
+  const double sthe = sqrt(1.0 - cthe * cthe);
+  double cphi = 1.0, sphi = 0.0;
+  if (sthe != 0.0) {
+    cphi = s0 / sthe;
+    sphi = s1 / sthe;
+  }
+  const double chi = fsw(t, eta);
+  const double w = (f > 1.0) ? chi / f : chi;
+  geo = make_double4(cthe, sthe, cphi, sphi);
+  tw = make_double2(t, w);
+}
+
+// One wave per container c: the records of its triples, all lanes busy.
+__global__ __launch_bounds__(kSetupThreads) void k_triple_geo(
+    int n, int ngrid, double eta, const double4 *__restrict__ sph,
+    const double4 *__restrict__ grid, const int *__restrict__ inl,
+    const int *__restrict__ pairoff, const int *__restrict__ tp, const double *__restrict__ fi,
+    double4 *__restrict__ tgeo, double2 *__restrict__ ttw) {
+  const int c = blockIdx.x * (kSetupThreads / kWave) + threadIdx.x / kWave;
+  if (c >= n) return;
+  const double4 sc = sph[c];
+  const int end = pairoff[inl[c + 1]];
+  for (int k = pairoff[inl[c]] + lane_id(); k < end; k += kWave) {
+    const unsigned p = (unsigned)tp[k];
+    const unsigned o = p / (unsigned)ngrid, pn = p - o * (unsigned)ngrid;
+    triple_record(sph[o], grid[pn], sc, fi[p], eta, tgeo[k], ttw[k]);
   }
 }
 
@@ -406,9 +463,9 @@ int run_setup(ddx_handle *h) {
 
   // triples per neighbour pair, grouped by container
   if (nnl > 0) {
-    int wpb = kSetupThreads / kWave;
-    hipLaunchKernelGGL(k_pair_count, dim3((nnl + wpb - 1) / wpb), dim3(kSetupThreads), 0, s,
-                       nnl, ng, h->nl.p, h->rev.p, h->tmask.p, h->paircnt.p);
+    const int wpb = kSetupThreads / kWave;
+    hipLaunchKernelGGL(k_pair_count, dim3((n + wpb - 1) / wpb), dim3(kSetupThreads), 0, s, n, ng,
+                       h->inl.p, h->nl.p, h->rev.p, h->tmask.p, h->paircnt.p);
     DDX_HIP(hipGetLastError());
     DDX_TRY(scan_exclusive(h->paircnt.p, h->pairoff.p, nnl, h->scan_tmp, s));
   } else {
@@ -456,11 +513,15 @@ int run_setup(ddx_handle *h) {
   DDX_TRY(h->pttri.reserve(ntri + 1));
   DDX_TRY(h->tslot.reserve(ntri + 1));
   if (nnl > 0) {
-    int wpb = kSetupThreads / kWave;
-    hipLaunchKernelGGL(k_pair_fill, dim3((nnl + wpb - 1) / wpb), dim3(kSetupThreads), 0, s,
-                       nnl, ng, h->eta, h->sph.p, h->grid.p, h->nl.p, h->prow.p, h->rev.p,
-                       h->tmask.p, h->fi.p, h->pairoff.p, h->ptoff.p, h->tgeo.p, h->ttw.p,
-                       h->tp.p, h->pttri.p, h->tslot.p);
+    const int wpb = kSetupThreads / kWave;
+    const dim3 grid_o((n + wpb - 1) / wpb);
+    hipLaunchKernelGGL(k_pair_fill, grid_o, dim3(kSetupThreads), 0, s, n, ng, h->inl.p, h->nl.p,
+                       h->rev.p, h->tmask.p, h->pairoff.p, h->ptoff.p, h->tp.p, h->pttri.p,
+                       h->tslot.p);
+    DDX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_triple_geo, grid_o, dim3(kSetupThreads), 0, s, n, ng, h->eta, h->sph.p,
+                       h->grid.p, h->inl.p, h->pairoff.p, h->tp.p, h->fi.p, h->tgeo.p,
+                       h->ttw.p);
     DDX_HIP(hipGetLastError());
   }
   DDX_TRY(h->ccav.reserve(3 * (size_t)ncav + 3));
