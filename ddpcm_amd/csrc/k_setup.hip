@@ -153,18 +153,18 @@ __global__ void k_nbr_rev(int nnl, const int *__restrict__ inl, const int *__res
   rev[q] = r;
 }
 
-// One thread per point (n, i): fi, ui, zi, the t < 1 mask, the triple count
-// and the cavity flag.  blockIdx.y is the sphere.
+// One thread per point p = i ngrid + n, over all points of all spheres (a
+// sphere's 302..5810 points would leave most of a last workgroup idle): fi,
+// ui, zi, the t < 1 mask, the triple count and the cavity flag.
 __global__ __launch_bounds__(kSetupThreads) void k_points(
     const double4 *__restrict__ sph, const double4 *__restrict__ grid, int ngrid,
     const int *__restrict__ inl, const int *__restrict__ nl, double eta, int igrad,
     double *__restrict__ fi, double *__restrict__ ui, double *__restrict__ zi,
     unsigned long long *__restrict__ tmask, int *__restrict__ ptcnt,
-    int *__restrict__ cavflag) {
-  int i = blockIdx.y;
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= ngrid) return;
-  size_t p = (size_t)i * ngrid + n;
+    int *__restrict__ cavflag, size_t np) {
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const int i = (int)(p / (size_t)ngrid), n = (int)(p - (size_t)i * ngrid);
   double4 si = sph[i], g = grid[n];
   double f = 0.0, z0 = 0.0, z1 = 0.0, z2 = 0.0;
   unsigned long long mask = 0;
@@ -203,12 +203,11 @@ __global__ __launch_bounds__(kSetupThreads) void k_ccav(const double4 *__restric
                                                         const double4 *__restrict__ grid,
                                                         int ngrid, const int *__restrict__ cavflag,
                                                         const int *__restrict__ cavidx,
-                                                        double *__restrict__ ccav) {
-  int i = blockIdx.y;
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= ngrid) return;
-  size_t p = (size_t)i * ngrid + n;
+                                                        double *__restrict__ ccav, size_t np) {
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
   if (!cavflag[p]) return;
+  const int i = (int)(p / (size_t)ngrid), n = (int)(p - (size_t)i * ngrid);
   int ic = cavidx[p];
   double4 si = sph[i], g = grid[n];
   ccav[3 * (size_t)ic + 0] = si.x + si.w * g.x;
@@ -325,12 +324,14 @@ __device__ __forceinline__ void triple_record(const double4 &so, const double4 &
   tw = make_double2(t, w);
 }
 
-// One wave per container c: the records of its triples, all lanes busy.
+// One wave per container c: the records of its triples, all lanes busy, and
+// with g32 / w32 their float copies for the FP32 sweeps.
 __global__ __launch_bounds__(kSetupThreads) void k_triple_geo(
     int n, int ngrid, double eta, const double4 *__restrict__ sph,
     const double4 *__restrict__ grid, const int *__restrict__ inl,
     const int *__restrict__ pairoff, const int *__restrict__ tp, const double *__restrict__ fi,
-    double4 *__restrict__ tgeo, double2 *__restrict__ ttw) {
+    double4 *__restrict__ tgeo, double2 *__restrict__ ttw, float4 *__restrict__ g32,
+    float2 *__restrict__ w32) {
   const int c = blockIdx.x * (kSetupThreads / kWave) + threadIdx.x / kWave;
   if (c >= n) return;
   const double4 sc = sph[c];
@@ -338,7 +339,15 @@ __global__ __launch_bounds__(kSetupThreads) void k_triple_geo(
   for (int k = pairoff[inl[c]] + lane_id(); k < end; k += kWave) {
     const unsigned p = (unsigned)tp[k];
     const unsigned o = p / (unsigned)ngrid, pn = p - o * (unsigned)ngrid;
-    triple_record(sph[o], grid[pn], sc, fi[p], eta, tgeo[k], ttw[k]);
+    double4 a;
+    double2 b;
+    triple_record(sph[o], grid[pn], sc, fi[p], eta, a, b);
+    tgeo[k] = a;
+    ttw[k] = b;
+    if (g32) {
+      g32[k] = make_float4((float)a.x, (float)a.y, (float)a.z, (float)a.w);
+      w32[k] = make_float2((float)b.x, (float)b.y);
+    }
   }
 }
 
@@ -450,10 +459,10 @@ int run_setup(ddx_handle *h) {
   DDX_TRY(h->ptoff.reserve(np + 1));
   DDX_TRY(h->cavflag.reserve(np + 1));
   DDX_TRY(h->cavidx.reserve(np + 1));
-  dim3 pg((ng + kSetupThreads - 1) / kSetupThreads, n);
+  const dim3 pg((unsigned)((np + kSetupThreads - 1) / kSetupThreads));
   hipLaunchKernelGGL(k_points, pg, dim3(kSetupThreads), 0, s, h->sph.p, h->grid.p, ng,
                      h->inl.p, h->nl.p, h->eta, h->igrad, h->fi.p, h->ui.p,
-                     h->igrad ? h->zi.p : nullptr, h->tmask.p, h->ptcnt.p, h->cavflag.p);
+                     h->igrad ? h->zi.p : nullptr, h->tmask.p, h->ptcnt.p, h->cavflag.p, np);
   DDX_HIP(hipGetLastError());
   DDX_TRY(scan_exclusive(h->ptcnt.p, h->ptoff.p, np, h->scan_tmp, s));
   DDX_TRY(scan_exclusive(h->cavflag.p, h->cavidx.p, np, h->scan_tmp, s));
@@ -519,14 +528,22 @@ int run_setup(ddx_handle *h) {
                        h->rev.p, h->tmask.p, h->pairoff.p, h->ptoff.p, h->tp.p, h->pttri.p,
                        h->tslot.p);
     DDX_HIP(hipGetLastError());
+    // the float copies come with the records when the handle runs the mixed
+    // precision, else on the first FP32 sweep
+    const bool f32 = h->prec == DDX_PREC_MIXED;
+    if (f32) {
+      DDX_TRY(h->tgeo32.reserve(ntri + 1));
+      DDX_TRY(h->ttw32.reserve(ntri + 1));
+    }
     hipLaunchKernelGGL(k_triple_geo, grid_o, dim3(kSetupThreads), 0, s, n, ng, h->eta, h->sph.p,
                        h->grid.p, h->inl.p, h->pairoff.p, h->tp.p, h->fi.p, h->tgeo.p,
-                       h->ttw.p);
+                       h->ttw.p, f32 ? h->tgeo32.p : nullptr, f32 ? h->ttw32.p : nullptr);
     DDX_HIP(hipGetLastError());
+    h->have_geo32 = f32;
   }
   DDX_TRY(h->ccav.reserve(3 * (size_t)ncav + 3));
   hipLaunchKernelGGL(k_ccav, pg, dim3(kSetupThreads), 0, s, h->sph.p, h->grid.p, ng,
-                     h->cavflag.p, h->cavidx.p, h->ccav.p);
+                     h->cavflag.p, h->cavidx.p, h->ccav.p, np);
   DDX_HIP(hipGetLastError());
   // instance offsets of the cavity, the solver's block maps (once per
   // handle: they depend on the sphere counts only), Coulomb tiles
