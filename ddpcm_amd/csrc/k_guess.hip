@@ -13,6 +13,8 @@
 //
 // Layouts: activations row-major (sphere, feature), weights row-major
 // (in, out); widths padded to multiples of 16 with zeros.
+#include <cstdint>
+
 #include "ddx_device.hpp"
 #include "ddx_kernels.hpp"
 
@@ -75,6 +77,63 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K,
     if (r < M && j < N) {
       float *o = c + (long)r * ldc + j;
       *o = (accumulate && gridDim.y == 1) ? *o + acc[e] : acc[e];
+    }
+  }
+}
+
+// The same product where A's rows are contiguous (ak = 1) and N <= 16 NT:
+// one wave per 16 rows of C computes all NT column tiles, so that each row
+// of A is read once (8-byte loads of four BF16 values) instead of once per
+// column tile.  Every tile's chain of MFMAs is k_gemm_bf16's, so C is
+// bitwise the same.
+template <int NT>
+__global__ __launch_bounds__(256) void k_gemm_bf16_rows(int M, int N, int K,
+                                                        const unsigned short *__restrict__ a,
+                                                        long ai,
+                                                        const unsigned short *__restrict__ b,
+                                                        long bk, long bj, float *__restrict__ c,
+                                                        long ldc, int accumulate) {
+  const int lane = lane_id();
+  const long ti = (long)blockIdx.x * 4 + threadIdx.x / kWave;
+  if (ti * 16 >= M) return;
+  const int i = (int)ti * 16 + (lane & 15), col = lane & 15, kq = (lane >> 4) * 4;
+  const unsigned short *ar = a + (long)min(i, M - 1) * ai;
+  f32x4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    const int k = k0 + kq;
+    bf16x4 av;
+    if (i < M && k + 3 < K) {
+      const uint2 w = *reinterpret_cast<const uint2 *>(ar + k);
+      av[0] = (short)(w.x & 0xffffu);
+      av[1] = (short)(w.x >> 16);
+      av[2] = (short)(w.y & 0xffffu);
+      av[3] = (short)(w.y >> 16);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) av[e] = (i < M && k + e < K) ? (short)ar[k + e] : (short)0;
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int j = 16 * t + col;
+      bf16x4 bv;
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        bv[e] = (j < N && k + e < K) ? (short)b[(long)(k + e) * bk + (long)j * bj] : (short)0;
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int j = 16 * t + col;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int r = (int)ti * 16 + kq + e;
+      if (r < M && j < N) {
+        float *o = c + (long)r * ldc + j;
+        *o = accumulate ? *o + acc[t][e] : acc[t][e];
+      }
     }
   }
 }
@@ -331,6 +390,24 @@ int gemm_bf16(hipStream_t s, int M, int N, int K, const unsigned short *a, long 
     nsplit = (int)std::min<long>(nsplit, (long)(ws_floats / ((size_t)M * ldc)));
     if (nsplit > 1) kchunk = ((K + nsplit - 1) / nsplit + 15) / 16 * 16;
     nsplit = std::max(1, (K + kchunk - 1) / kchunk);
+  }
+  // contiguous rows of A (the activations): one wave per 16 rows, all columns
+  const int ntn = (N + 15) / 16;
+  if (nsplit == 1 && ak == 1 && ntn <= 8 && (ai % 4) == 0 &&
+      (reinterpret_cast<uintptr_t>(a) % 8) == 0) {
+    const dim3 g((unsigned)(((M + 15) / 16 + 3) / 4));
+    const int acc = accumulate ? 1 : 0;
+    switch (ntn) {
+#define DDX_GR(NT)                                                                          \
+  case NT:                                                                                 \
+    hipLaunchKernelGGL(k_gemm_bf16_rows<NT>, g, dim3(256), 0, s, M, N, K, a, ai, b, bk, bj, c, \
+                       ldc, acc);                                                          \
+    break;
+      DDX_GR(1) DDX_GR(2) DDX_GR(3) DDX_GR(4) DDX_GR(5) DDX_GR(6) DDX_GR(7) DDX_GR(8)
+#undef DDX_GR
+    }
+    DDX_HIP(hipGetLastError());
+    return DDX_OK;
   }
   hipLaunchKernelGGL(k_gemm_bf16, dim3((unsigned)((tiles + 3) / 4), nsplit), dim3(256), 0, s, M,
                      N, K, a, ai, ak, b, bk, bj, nsplit > 1 ? ws : c, ldc, accumulate ? 1 : 0,
