@@ -32,8 +32,8 @@ __host__ __device__ constexpr int lmi(int l, int m) { return l * l + l + m; }
 // the poles (sin th = 0) e_phi = (0, 1, 0) and the e_phi weight of m is
 // -dP_lm int(cos th) (ddcosmo.f90:970-971).  The Legendre functions run
 // l-outer, three rows in registers, with polleg's recurrence.
-template <int LMAX>
-__global__ __launch_bounds__(kWave) void k_force_triple(
+template <int LMAX, int BT>
+__global__ __launch_bounds__(BT) void k_force_triple(
     const int *__restrict__ inl, const int *__restrict__ pairoff,
     const double4 *__restrict__ tgeo, const double2 *__restrict__ ttw,
     const double *__restrict__ sigma, const double *__restrict__ cf,
@@ -41,10 +41,10 @@ __global__ __launch_bounds__(kWave) void k_force_triple(
   constexpr int NY = (LMAX + 1) * (LMAX + 1);
   __shared__ double xs[NY];
   const int c = blockIdx.x;
-  for (int k = threadIdx.x; k < NY; k += kWave) xs[k] = cf[k] * sigma[(size_t)c * NY + k];
+  for (int k = threadIdx.x; k < NY; k += BT) xs[k] = cf[k] * sigma[(size_t)c * NY + k];
   __syncthreads();
   const int beg = pairoff[inl[c]], end = pairoff[inl[c + 1]];
-  for (int k = beg + threadIdx.x; k < end; k += kWave) {
+  for (int k = beg + threadIdx.x; k < end; k += BT) {
     const double4 g = tgeo[k];
     const double t = ttw[k].x;
     const double cthe = g.x, sthe = g.y, cphi = g.z, sphi = g.w;
@@ -116,9 +116,19 @@ __global__ __launch_bounds__(kWave) void k_force_triple(
   }
 }
 
-// fdoka and the own-point term of fdoga for sphere o = blockIdx.x, one wave
-// over its points.
-__global__ __launch_bounds__(kWave) void k_force_owner(
+// Per-sphere reduction of the force kernels: one wave (BT = 64, large
+// systems) or four (BT = 256, few spheres, where one wave per sphere would
+// leave most SIMDs idle).
+template <int BT>
+__device__ __forceinline__ double force_sum(double v, double *lds) {
+  if constexpr (BT == kWave) return wave_sum(v);
+  else return block_sum_256(v, lds);
+}
+
+// fdoka and the own-point term of fdoga for sphere o = blockIdx.x, BT
+// threads over its points.
+template <int BT>
+__global__ __launch_bounds__(BT) void k_force_owner(
     int ngrid, double eta, const int *__restrict__ inl, const int *__restrict__ nl,
     const double4 *__restrict__ sph, const double4 *__restrict__ grid,
     const int *__restrict__ ptoff, const int *__restrict__ pttri,
@@ -127,11 +137,12 @@ __global__ __launch_bounds__(kWave) void k_force_owner(
     const double *__restrict__ ui, const double *__restrict__ zi,
     const double *__restrict__ xi, const double *__restrict__ phi,
     const int *__restrict__ cavidx, double *__restrict__ fo) {
+  __shared__ double lds[4];
   const int o = blockIdx.x;
   const double4 so = sph[o];
   const double tlow = 1.0 - eta;
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-  for (int n = threadIdx.x; n < ngrid; n += kWave) {
+  for (int n = threadIdx.x; n < ngrid; n += BT) {
     const size_t p = (size_t)o * ngrid + n;
     const double4 g = grid[n];
     const double fip = fi[p], xip = xi[p];
@@ -177,9 +188,9 @@ __global__ __launch_bounds__(kWave) void k_force_owner(
       f2 -= g.w * (ph * xip * z2);
     }
   }
-  f0 = wave_sum(f0);
-  f1 = wave_sum(f1);
-  f2 = wave_sum(f2);
+  f0 = force_sum<BT>(f0, lds);
+  f1 = force_sum<BT>(f1, lds);
+  f2 = force_sum<BT>(f2, lds);
   if (threadIdx.x == 0) {
     fo[3 * (size_t)o] = f0;
     fo[3 * (size_t)o + 1] = f1;
@@ -187,9 +198,10 @@ __global__ __launch_bounds__(kWave) void k_force_owner(
   }
 }
 
-// fdokb and the neighbour term of fdoga for container c = blockIdx.x, one
-// wave over its triples.
-__global__ __launch_bounds__(kWave) void k_force_container(
+// fdokb and the neighbour term of fdoga for container c = blockIdx.x, BT
+// threads over its triples.
+template <int BT>
+__global__ __launch_bounds__(BT) void k_force_container(
     int ngrid, double eta, const int *__restrict__ inl, const int *__restrict__ pairoff,
     const double4 *__restrict__ sph, const double4 *__restrict__ grid,
     const int *__restrict__ ptoff, const int *__restrict__ pttri,
@@ -198,12 +210,13 @@ __global__ __launch_bounds__(kWave) void k_force_container(
     const double *__restrict__ ui, const double *__restrict__ xi,
     const double *__restrict__ phi, const int *__restrict__ cavidx,
     double *__restrict__ fc) {
+  __shared__ double lds[4];
   const int c = blockIdx.x;
   const int beg = pairoff[inl[c]], end = pairoff[inl[c + 1]];
   const double4 sc = sph[c];
   const double tlow = 1.0 - eta;
   double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-  for (int k = beg + threadIdx.x; k < end; k += kWave) {
+  for (int k = beg + threadIdx.x; k < end; k += BT) {
     const int p = tp[k];
     const int o = p / ngrid, n = p - o * ngrid;
     const double4 g = grid[n];
@@ -251,9 +264,9 @@ __global__ __launch_bounds__(kWave) void k_force_container(
       f2 += g.w * (fa * s2);
     }
   }
-  f0 = wave_sum(f0);
-  f1 = wave_sum(f1);
-  f2 = wave_sum(f2);
+  f0 = force_sum<BT>(f0, lds);
+  f1 = force_sum<BT>(f1, lds);
+  f2 = force_sum<BT>(f2, lds);
   if (threadIdx.x == 0) {
     fc[3 * (size_t)c] = f0;
     fc[3 * (size_t)c + 1] = f1;
@@ -315,11 +328,21 @@ __global__ void k_sph3(int n, const double4 *__restrict__ sph, double *__restric
   c3[3 * i + 2] = s.z;
 }
 
+// waves per sphere of the force kernels: one where the spheres alone fill
+// the device several times over, else four
+int force_waves(const ddx_handle *h) { return h->n >= 16384 ? 1 : 4; }
+
 template <int LMAX>
 int launch_triple(ddx_handle *h, const double *sigma, double4 *fb) {
   TimedLaunch tl(h, DDX_K_FORCES, h->stream);
-  hipExtLaunchKernelGGL(k_force_triple<LMAX>, dim3(h->n), dim3(kWave), 0, h->stream, tl.b, tl.e,
-                        0, h->inl.p, h->pairoff.p, h->tgeo.p, h->ttw.p, sigma, h->cf.p, fb);
+  if (force_waves(h) == 1)
+    hipExtLaunchKernelGGL((k_force_triple<LMAX, kWave>), dim3(h->n), dim3(kWave), 0, h->stream,
+                          tl.b, tl.e, 0, h->inl.p, h->pairoff.p, h->tgeo.p, h->ttw.p, sigma,
+                          h->cf.p, fb);
+  else
+    hipExtLaunchKernelGGL((k_force_triple<LMAX, 256>), dim3(h->n), dim3(256), 0, h->stream, tl.b,
+                          tl.e, 0, h->inl.p, h->pairoff.p, h->tgeo.p, h->ttw.p, sigma, h->cf.p,
+                          fb);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
@@ -367,13 +390,16 @@ int forces_dd(ddx_handle *h, const double *d_phi, const double *d_sigma, const d
   double *fo = h->fpart.p, *fc = h->fpart.p + 3 * (size_t)n;
   if (n > 0) {
     DDX_TRY(force_triple(h, d_sigma, fb));
-    hipLaunchKernelGGL(k_force_owner, dim3(n), dim3(kWave), 0, s, h->ngrid, h->eta, h->inl.p,
-                       h->nl.p, h->sph.p, h->grid.p, h->ptoff.p, h->pttri.p, h->tmask.p,
-                       h->ttw.p, fb, h->fi.p, h->ui.p, h->zi.p, h->xi.p, d_phi, h->cavidx.p, fo);
+    const bool one = force_waves(h) == 1;
+    hipLaunchKernelGGL(one ? k_force_owner<kWave> : k_force_owner<256>, dim3(n),
+                       dim3(one ? kWave : 256), 0, s, h->ngrid, h->eta, h->inl.p, h->nl.p,
+                       h->sph.p, h->grid.p, h->ptoff.p, h->pttri.p, h->tmask.p, h->ttw.p, fb,
+                       h->fi.p, h->ui.p, h->zi.p, h->xi.p, d_phi, h->cavidx.p, fo);
     DDX_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_force_container, dim3(n), dim3(kWave), 0, s, h->ngrid, h->eta,
-                       h->inl.p, h->pairoff.p, h->sph.p, h->grid.p, h->ptoff.p, h->pttri.p,
-                       h->ttw.p, h->tp.p, fb, h->fi.p, h->ui.p, h->xi.p, d_phi, h->cavidx.p, fc);
+    hipLaunchKernelGGL(one ? k_force_container<kWave> : k_force_container<256>, dim3(n),
+                       dim3(one ? kWave : 256), 0, s, h->ngrid, h->eta, h->inl.p, h->pairoff.p,
+                       h->sph.p, h->grid.p, h->ptoff.p, h->pttri.p, h->ttw.p, h->tp.p, fb,
+                       h->fi.p, h->ui.p, h->xi.p, d_phi, h->cavidx.p, fc);
     DDX_HIP(hipGetLastError());
     double fep = 0.5 * (h->eps - 1.0) / h->eps;
     hipLaunchKernelGGL(k_force_combine, dim3((3 * n + 255) / 256), dim3(256), 0, s, n, fep, fo,
