@@ -49,7 +49,7 @@ struct SysArgs {
   const double *yp;          //   y(lm, c) = sum of yp(lm, s), s in segoff[c] ..
   const int *segoff;
   double *x, *xs;            // xs = cf x, kept for the L X sweep (null for L*)
-  double *xd, *ed, *xnew;    // DIIS history and the Jacobi step
+  double *xd, *ed, *xnew;    // DIIS history (ed: float in mixed precision) and the Jacobi step
   double *part, *part_e;     // Gram-row (26 rows of nbj) and increment-norm partial sums
   int *newslot;              // per instance
   DiisState *st;             // per instance, then the summary of a batch
@@ -64,6 +64,22 @@ struct Systems {
   SysArgs s[2];
 };
 
+//
+// a history value as double, converted where it is used: the empty asm keeps
+// the compiler from moving the conversion (and the wait for its load) into
+// the branch that loaded it, ahead of the element's other loads
+__device__ __forceinline__ double widen_late(double v) { return v; }
+__device__ __forceinline__ double widen_late(float v) {
+  asm volatile("" : "+v"(v));
+  return (double)v;
+}
+
+// E is the storage type of the residual history: double, or float in mixed
+// precision, where the history only steers the extrapolation coefficients.
+// Each residual is rounded to E once and that rounded value enters every dot
+// product it is part of (its self dot included), so that B stays the Gram
+// matrix of the stored vectors.
+template <typename E>
 __global__ __launch_bounds__(kJThreads) void k_jacobi(size_t nn, int nylm, int diis_max,
                                                       const int4 *__restrict__ jblk, int nbj,
                                                       const double *__restrict__ facl,
@@ -79,7 +95,7 @@ __global__ __launch_bounds__(kJThreads) void k_jacobi(size_t nn, int nylm, int d
   const int *__restrict__ segoff = q0.segoff;
   const double *__restrict__ x = q0.x;
   double *__restrict__ xd = q0.xd;
-  double *__restrict__ ed = q0.ed;
+  E *__restrict__ ed = reinterpret_cast<E *>(q0.ed);
   double *__restrict__ xnew = q0.xnew;
   double *__restrict__ part = q0.part;
   __shared__ double wpart[kJThreads / kWave][kPartStride];
@@ -87,11 +103,20 @@ __global__ __launch_bounds__(kJThreads) void k_jacobi(size_t nn, int nylm, int d
   const int off = (diis_max > 0 && nmat >= diis_max) ? 10 : 0;
   const int ndot = diis_max > 0 ? nmat - off - 1 : 0;
   double *xo = diis_max > 0 ? xd + (size_t)st->slot[nmat - 1] * nn : xnew;
-  double *eo = diis_max > 0 ? ed + (size_t)st->slot[nmat - 1] * nn : nullptr;
-  const double *eh[kDiisMax];
+  E *eo = diis_max > 0 ? ed + (size_t)st->slot[nmat - 1] * nn : nullptr;
+  const E *eh[kDiisMax];
 #pragma unroll
   for (int q = 0; q < kDiisMax; ++q)
     eh[q] = (q < ndot) ? ed + (size_t)st->slot[off + q] * nn : nullptr;
+  if constexpr (sizeof(E) == 4) {
+    // FP32 history: slots past ndot read the first slot's element again (a
+    // cache hit, its value unused), so that every load is unconditional and
+    // none waits at a branch for the one before it
+    const E *fill = ndot > 0 ? eh[0] : (eo ? eo : reinterpret_cast<const E *>(x));
+#pragma unroll
+    for (int q = 0; q < kDiisMax; ++q)
+      if (q >= ndot) eh[q] = fill;
+  }
   double acc[kDiisMax];
 #pragma unroll
   for (int q = 0; q < kDiisMax; ++q) acc[q] = 0.0;
@@ -101,10 +126,16 @@ __global__ __launch_bounds__(kJThreads) void k_jacobi(size_t nn, int nylm, int d
     const unsigned c = (unsigned)k / (unsigned)nylm;
     const unsigned lm = (unsigned)k - c * (unsigned)nylm;
     // the history residuals are read first, with the element's other
-    // operands, so that all of them are in flight together
-    double ev[kDiisMax];
+    // operands, so that all of them are in flight together (kept as E until
+    // the dot products, so that no conversion waits for its load early)
+    E ev[kDiisMax];
 #pragma unroll
-    for (int q = 0; q < kDiisMax; ++q) ev[q] = (q < ndot) ? eh[q][k] : 0.0;
+    for (int q = 0; q < kDiisMax; ++q) {
+      if constexpr (sizeof(E) == 4)
+        ev[q] = eh[q][k];
+      else
+        ev[q] = (q < ndot) ? eh[q][k] : E(0);
+    }
     double yk;
     if (yp) {
       yk = 0.0;
@@ -118,13 +149,14 @@ __global__ __launch_bounds__(kJThreads) void k_jacobi(size_t nn, int nylm, int d
       q0.yacc[k] = yk;
     }
     double xn = facl[lm] * (rhs[k] - yk);
-    double e = xn - x[k];
+    const E er = (E)(xn - x[k]);
+    const double e = (double)er;
     xo[k] = xn;
     if (eo) {
-      eo[k] = e;
+      eo[k] = er;
 #pragma unroll
       for (int q = 0; q < kDiisMax; ++q)
-        if (q < ndot) acc[q] += ev[q] * e;
+        if (q < ndot) acc[q] += widen_late(ev[q]) * e;
       self += e * e;
     }
   }
@@ -177,7 +209,7 @@ template <int S>
 __device__ __forceinline__ void pivot_stage(double &big, int &best) {
   const double pb = partner<S>(big);
   const int pi = partner<S>(best);
-  const bool t = (pb > big) || (pb == big && pi < best);
+  const bool t = (pb > big) | ((pb == big) & (pi < best));
   big = t ? pb : big;
   best = t ? pi : best;
 }
@@ -193,6 +225,44 @@ __device__ __forceinline__ void instance_done(DiisState *sum, int it, int ok) {
   if (atomicSub(&sum->remaining, 1) == 1) sum->done = 1;
 }
 
+// lanes per summed value in the Gram-row reduction of k_diis_small
+constexpr int kTPQ = 8;
+
+// partials r, r + kTPQ, ... of one value, added in that order; K loads per
+// lane are in flight together (indices past nb re-read the last partial and
+// are not added, so that no load is conditional)
+template <int K>
+__device__ __forceinline__ double sum_partials(const double *__restrict__ src, int nb, int r) {
+  double s = 0.0;
+  for (int b0 = r; b0 < nb; b0 += kTPQ * K) {
+    double v[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) v[i] = src[min(b0 + kTPQ * i, nb - 1)];
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+      if (b0 + kTPQ * i < nb) s += v[i];
+  }
+  return s;
+}
+
+// Gauss-Jordan of k_diis_small: columns per lane (8 groups of lanes, n1 <= 25)
+constexpr int kGJCols = 4;
+
+// the columns of wave W's two groups, W + 8 j (lanes 0..31) and W + 4 + 8 j
+// (lanes 32..63), of row lane & 31; lane L < kB holds row L of B in a
+// (static register indices; the upper half takes its row from lane L - 32)
+template <int W>
+__device__ __forceinline__ void group_columns(const double (&a)[kB], double (&g)[kGJCols]) {
+#pragma unroll
+  for (int j = 0; j < kGJCols; ++j) {
+    constexpr int lo0 = W, hi0 = W + 4;
+    const int lo = lo0 + 8 * j, hi = hi0 + 8 * j;
+    const double vl = (lo < kB) ? a[lo < kB ? lo : 0] : 0.0;
+    const double vh = partner<0>((hi < kB) ? a[hi < kB ? hi : 0] : 0.0);
+    g[j] = (lane_id() < 32) ? vl : vh;
+  }
+}
+
 __global__ __launch_bounds__(kSmallThreads) void k_diis_small(
     int diis_max, int nbj, const int *__restrict__ jboff, const int *__restrict__ eboff,
     const int *__restrict__ soff, int ninst, int maxit, Systems sys) {
@@ -200,81 +270,79 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(
   const SysArgs &q0 = sys.s[blockIdx.y];
   const int inst = blockIdx.x;
   DiisState *__restrict__ st = q0.st + inst;
-  if (st->done) return;
+  const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+  // every state word and map entry of the iteration, loaded together
+  const int done = st->done;
   const double tol = st->tol;
-  DiisState *sum = ninst > 1 ? q0.st + ninst : nullptr;
-  // this instance's workgroup partials
+  int nmat = st->nmat;
+  const int it0 = st->it;
+  const bool pending = st->pending_norm != 0;
   const int jb = jboff[inst], eb = eboff[inst];
   const int nblk = jboff[inst + 1] - jb, nblk_e = eboff[inst + 1] - eb;
   const int nsph = soff[inst + 1] - soff[inst];
+  if (done) return;
+  DiisState *sum = ninst > 1 ? q0.st + ninst : nullptr;
   const double *__restrict__ part = q0.part + jb;
   const double *__restrict__ part_e = q0.part_e + eb;
-  int *__restrict__ newslot_out = q0.newslot + inst;
+  // lane r: row r of B and logical slot r, in flight with the partial sums
+  double a[kB];
+#pragma unroll
+  for (int c = 0; c < kB; ++c) a[c] = (lane < kB) ? st->b[lane * kB + c] : 0.0;
+  int myslot = (lane < diis_max) ? st->slot[lane] : 0;
+  const int lastslot = st->slot[nmat - 1];
   __shared__ double red[kPartStride + 1];
-  __shared__ int stop;
-  const int tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
-  int nmat = st->nmat;
   const int off = (nmat >= diis_max) ? 10 : 0;
   const int ndot = nmat - off - 1;
-  const bool pending = st->pending_norm != 0;
   // Gram-row dot products and the increment norm of the previous iteration
   // (value 26): eight consecutive lanes share a value, each sums every eighth
   // workgroup partial, so that all loads are in flight together, and the
   // eight partial sums meet in three lane-pair stages within the group
   {
-    constexpr int kTPQ = 8;
     const int q = tid / kTPQ, r = tid % kTPQ;
-    double sum = 0.0;
+    double s = 0.0;
     if (q <= kPartStride) {
       const bool need = (q < ndot || q == kDiisMax) || (q == kPartStride && pending);
       if (need) {
         const double *src = (q == kPartStride) ? part_e : part + (size_t)q * nbj;
         const int nb = (q == kPartStride) ? nblk_e : nblk;
-#pragma unroll 8
-        for (int b = r; b < nb; b += kTPQ) sum += src[b];
+        if (max(nblk, nblk_e) <= kTPQ * 8)
+          s = sum_partials<8>(src, nb, r);
+        else
+          s = sum_partials<32>(src, nb, r);
       }
     }
-    sum += partner<3>(sum);
-    sum += partner<4>(sum);
-    sum += partner<5>(sum);
-    if (r == 0 && q <= kPartStride) red[q] = sum;
+    s += partner<3>(s);
+    s += partner<4>(s);
+    s += partner<5>(s);
+    if (r == 0 && q <= kPartStride) red[q] = s;
   }
   __syncthreads();
-  if (tid == 0) {
-    stop = 0;
-    if (pending) {
-      // rms of the per-sphere norms and the convergence test of the previous
-      // iteration (rmsvec, jacobi_diis.f90:405-438; jacobi_diis.f90:153-156)
-      const double norm = sqrt(red[kPartStride] / (double)nsph);
-      const int it = st->it;
-      if (it <= kMaxIter) st->hist[it] = norm;
-      st->it = it + 1;
+  if (pending) {
+    // rms of the per-sphere norms and the convergence test of the previous
+    // iteration (rmsvec, jacobi_diis.f90:405-438; jacobi_diis.f90:153-156),
+    // evaluated alike by every thread
+    const double norm = sqrt(red[kPartStride] / (double)nsph);
+    const bool stop = norm < tol || it0 + 1 >= maxit;
+    if (tid == 0) {
+      if (it0 <= kMaxIter) st->hist[it0] = norm;
+      st->it = it0 + 1;
       st->norm = norm;
       st->ok = norm < tol;
       st->pending_norm = 0;
-      if (norm < tol || it + 1 >= maxit) {
+      if (stop) {
         st->done = 1;
-        stop = 1;
-        instance_done(sum, it + 1, norm < tol);
+        instance_done(sum, it0 + 1, norm < tol);
       }
     }
+    if (stop) return;
   }
-  __syncthreads();
-  if (stop) return;
-  __shared__ double bsh[kB][kB + 1];   // B after this iteration's update
-  __shared__ int nmat_sh;
-  if (wave == 0) {
-  double a[kB];
-#pragma unroll
-  for (int c = 0; c < kB; ++c) a[c] = (lane < kB) ? st->b[lane * kB + c] : 0.0;
-  int myslot = (lane < diis_max) ? st->slot[lane] : 0;
+  // B update, by every wave alike (wave 0 persists it)
+  {
   // lane i + 1 keeps dot i; every lane keeps the self dot
   const int qi = (lane >= 1 && lane <= kDiisMax) ? lane - 1 : 0;
   double mydot = red[qi];
   const double self = red[kDiisMax];
-  if (lane == 0) st->pending_norm = 1;   // k_extrap leaves this iteration's norm
   if (!(lane >= 1 && lane - 1 < ndot)) mydot = 0.0;
-  if (lane == 0) *newslot_out = st->slot[nmat - 1];
   if (nmat >= diis_max) {
     const int m = nmat - 10;
     // rows 1..m-1 <- rows 11..m+9, then columns likewise
@@ -297,9 +365,10 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(
     if (lane == 0) { a[0] = 0.0; a[1] = 1.0; }
     if (lane == 1) { a[0] = 1.0; a[1] = self; }
   } else {
-    double rowv[kB];
+    double rowv[kB];   // rowv[c] = dot c-1, from the block's LDS copy
+    rowv[0] = 0.0;
 #pragma unroll
-    for (int c = 0; c < kB; ++c) rowv[c] = lane_value(mydot, c);   // dot c-1
+    for (int c = 1; c < kB; ++c) rowv[c] = red[c - 1];
 #pragma unroll
     for (int c = 0; c < kB; ++c) {
       if (c == nmat) {
@@ -315,127 +384,164 @@ __global__ __launch_bounds__(kSmallThreads) void k_diis_small(
         if (c < nmat) a[c] = rowv[c];
     }
   }
-  // persist B and the slot map
-  if (lane < kB) {
+  }
+  if (wave == 0) {
+    // persist B and the slot map; k_extrap leaves this iteration's norm
+    if (lane < kB) {
 #pragma unroll
-    for (int c = 0; c < kB; ++c) {
-      st->b[lane * kB + c] = a[c];
-      bsh[lane][c] = a[c];
+      for (int c = 0; c < kB; ++c) st->b[lane * kB + c] = a[c];
+    }
+    if (lane < diis_max) st->slot[lane] = myslot;
+    if (lane == 0) {
+      st->pending_norm = 1;
+      q0.newslot[inst] = lastslot;
     }
   }
-  if (lane < diis_max) st->slot[lane] = myslot;
-  if (lane == 0) nmat_sh = nmat;
-  }   // wave 0
-  __syncthreads();
-  nmat = nmat_sh;
 
-  // Gauss-Jordan with full pivoting on B, n1 = nmat + 1, by all four waves:
-  // lane r holds row r (its current row index is `pos`: the reference's row
-  // exchange becomes an exchange of two pos values) and wave w the columns
-  // w, w + 4, ...; the right-hand side e_1 travels with wave 0.  Columns
-  // already pivoted hold the in-place inverse, which neither the later
-  // pivots nor the solution read, so only live columns are updated; the
-  // arithmetic on every element is the reference's.
-  // per step and wave: its best candidate and the whole column it lies in,
-  // double-buffered so that one barrier per step suffices
+  // Gauss-Jordan with full pivoting on B, n1 = nmat + 1, by all four waves
+  // and both halves of each: lane L works on row r = L & 31 (its current row
+  // index is `pos`: the reference's row exchange becomes an exchange of two
+  // pos values) and on the columns G, G + 8, G + 16, G + 24 of its group
+  // G = wave + 4 (L >> 5); the right-hand side e_1 travels with wave 0.
+  // Columns already pivoted hold the in-place inverse, which neither the
+  // later pivots nor the solution read, so what they receive does not
+  // matter; the arithmetic on every live element is the reference's.  A
+  // candidate is coded row * 32 + column (n1 <= 25), which orders candidates
+  // row-major as the reference's nested loops do; the pivot is the largest
+  // value (an exact max), then the smallest code among its ties.
+  // Per step and wave: its candidate, the lane holding that row, the signed
+  // pivot value and the whole column it lies in, double-buffered so that
+  // one barrier per step suffices.
   constexpr int kNW = kSmallThreads / kWave;
-  __shared__ double cand_v[2][kNW];
-  __shared__ int cand_k[2][kNW];
-  __shared__ double ccol[2][kNW][kWave];
-  constexpr int kCW = (kB + 3) / 4;    // columns per wave
+  constexpr int kNone = 0x7fffffff;
+  struct alignas(16) Cand {
+    double v, d;   // |pivot| (-1: none) and the signed pivot
+    int k, l;      // candidate code and the lane holding its row
+  };
+  __shared__ Cand cand[2][kNW];
+  __shared__ double ccol[2][kNW][32];
   const int n1 = nmat + 1;
-  double g[kCW];
-#pragma unroll
-  for (int j = 0; j < kCW; ++j) {
-    const int c = wave + 4 * j;
-    g[j] = (lane < n1 && c < n1) ? bsh[lane][c] : 0.0;
+  const int r = lane & 31, half = lane >> 5;
+  double g[kGJCols];
+  switch (__builtin_amdgcn_readfirstlane(wave)) {
+    case 0: group_columns<0>(a, g); break;
+    case 1: group_columns<1>(a, g); break;
+    case 2: group_columns<2>(a, g); break;
+    default: group_columns<3>(a, g); break;
   }
+  const int grp = wave + 4 * half;
+#pragma unroll
+  for (int j = 0; j < kGJCols; ++j)
+    if (!(r < n1 && grp + 8 * j < n1)) g[j] = 0.0;
   double bv = (lane == 0) ? 1.0 : 0.0;
-  int pos = lane;
+  int pos = r;
   unsigned int live = (n1 >= 32) ? 0xffffffffu : ((1u << n1) - 1u);   // columns not pivoted
   bool ok = true;
   for (int it = 0; it < n1; ++it) {
     const int buf = it & 1;
     // pivot: largest |a(j, k)| > 0 over rows and columns not pivoted, first
     // in row-major order of the current row indices on ties
-    const bool rowlive = lane < n1 && ((live >> pos) & 1u);
+    const bool rowlive = r < n1 && ((live >> pos) & 1u);
     double big = -1.0;
-    int best = 0x7fffffff;
+    int best = kNone;
 #pragma unroll
-    for (int j = 0; j < kCW; ++j) {
-      const int c = wave + 4 * j;
+    for (int j = 0; j < kGJCols; ++j) {
+      const int c = grp + 8 * j;
       const double v = (rowlive && c < n1 && ((live >> c) & 1u)) ? fabs(g[j]) : -1.0;
       const bool t = v > big;   // columns ascend: the first one wins ties
       big = t ? v : big;
-      best = t ? pos * n1 + c : best;
+      best = t ? pos * 32 + c : best;
     }
-    if (!(big > 0.0)) best = 0x7fffffff;
-    // rows live in lanes 0..31: five lane-pair stages
-    pivot_stage<1>(big, best);
-    pivot_stage<2>(big, best);
-    pivot_stage<3>(big, best);
-    pivot_stage<4>(big, best);
-    pivot_stage<5>(big, best);
-    best = __builtin_amdgcn_readfirstlane(best);
+    if (!(big > 0.0)) best = kNone;
     {
-      // the column of this wave's candidate: the multipliers a(j, icol) if
-      // it wins
-      const int jj = (best == 0x7fffffff) ? 0 : ((best % n1) >> 2);
+      double m = big;
+      m = fmax(m, partner<0>(m));
+      m = fmax(m, partner<1>(m));
+      m = fmax(m, partner<2>(m));
+      m = fmax(m, partner<3>(m));
+      m = fmax(m, partner<4>(m));
+      m = fmax(m, partner<5>(m));
+      int key = (big == m) ? best : kNone;
+      key = min(key, partner<0>(key));
+      key = min(key, partner<1>(key));
+      key = min(key, partner<2>(key));
+      key = min(key, partner<3>(key));
+      key = min(key, partner<4>(key));
+      key = min(key, partner<5>(key));
+      big = m;
+      best = __builtin_amdgcn_readfirstlane(key);
+    }
+    {
+      // the column of this wave's candidate (the multipliers a(j, icol) if
+      // it wins), held by the half of its group, and, from the lane holding
+      // its row there, the signed pivot
+      const int cc = best & 31;
+      const int jj = (best == kNone) ? 0 : (cc >> 3);
+      const int hc = (best == kNone) ? 0 : ((cc >> 2) & 1);
       double v = 0.0;
 #pragma unroll
-      for (int j = 0; j < kCW; ++j) v = (j == jj) ? g[j] : v;
-      ccol[buf][wave][lane] = v;
-    }
-    if (lane == 0) {
-      cand_v[buf][wave] = big;
-      cand_k[buf][wave] = best;
+      for (int j = 0; j < kGJCols; ++j) v = (j == jj) ? g[j] : v;
+      if (half == hc) ccol[buf][wave][r] = v;
+      int lr = __ffsll((unsigned long long)__ballot(lane < n1 && pos == (best >> 5))) - 1;
+      lr = max(lr, 0);
+      const double dv = lane_value(v, lr + 32 * hc);
+      if (lane == 0) cand[buf][wave] = Cand{big, dv, best, lr};
     }
     __syncthreads();
-    big = cand_v[buf][0];
-    best = cand_k[buf][0];
+    Cand cw[kNW];
+#pragma unroll
+    for (int w = 0; w < kNW; ++w) cw[w] = cand[buf][w];
+    double m = cw[0].v;
+#pragma unroll
+    for (int w = 1; w < kNW; ++w) m = fmax(m, cw[w].v);
+    best = kNone;
     int win = 0;
 #pragma unroll
-    for (int w = 1; w < kNW; ++w) {
-      const double ov = cand_v[buf][w];
-      const int ok2 = cand_k[buf][w];
-      const bool t = (ov > big) || (ov == big && ok2 < best);
-      big = t ? ov : big;
-      best = t ? ok2 : best;
+    for (int w = 0; w < kNW; ++w) {
+      const bool t = (cw[w].v == m) & (cw[w].k < best);
+      best = t ? cw[w].k : best;
       win = t ? w : win;
     }
     best = __builtin_amdgcn_readfirstlane(best);
     win = __builtin_amdgcn_readfirstlane(win);
-    if (best == 0x7fffffff) { ok = false; break; }
-    const int irow = best / n1, icol = best % n1;
-    if (!((live >> icol) & 1u)) { ok = false; break; }
-    // the lane holding row irow becomes pivot row icol
-    const int lr = __ffsll((unsigned long long)__ballot(lane < n1 && pos == irow)) - 1;
-    if (lr < 0) { ok = false; break; }
-    // each row's multiplier a(j, icol), read before the pivot row is scaled
-    const double dum = ccol[buf][win][lane];
-    const double d = ccol[buf][win][lr];
-    if (d == 0.0) { ok = false; break; }
+    if (best == kNone) { ok = false; break; }
+    double d = cw[0].d;
+    int lr = cw[0].l;
+#pragma unroll
+    for (int w = 1; w < kNW; ++w) {
+      d = (win == w) ? cw[w].d : d;
+      lr = (win == w) ? cw[w].l : lr;
+    }
+    lr = __builtin_amdgcn_readfirstlane(lr);
+    const int irow = best >> 5, icol = best & 31;
+    if (!((live >> icol) & 1u) || d == 0.0) { ok = false; break; }
+    // each row's multiplier a(j, icol) and the pivot row's values in this
+    // lane's columns, read before the pivot row is scaled
+    const double dum = ccol[buf][win][r];
+    const int src = lr + 32 * half;
+    double pr[kGJCols];
+#pragma unroll
+    for (int j = 0; j < kGJCols; ++j) pr[j] = __shfl(g[j], src, kWave);
     const double pinv = 1.0 / d;
+    // the lane holding row irow becomes pivot row icol
     if (irow != icol) {
       int np = pos;
-      if (lane < n1 && pos == icol) np = irow;
-      if (lane == lr) np = icol;
+      if (r < n1 && pos == icol) np = irow;
+      if (r == lr) np = icol;
       pos = np;
     }
     live &= ~(1u << icol);
     // scale the pivot row (a(icol, :) * pivinv) and eliminate column icol
-    // from the other rows
+    // from the other rows, in every column of the lane (no branch)
 #pragma unroll
-    for (int j = 0; j < kCW; ++j) {
-      const int c = wave + 4 * j;
-      if (c < kB && ((live >> c) & 1u)) {
-        const double pc = lane_value(g[j], lr) * pinv;
-        g[j] = (lane == lr) ? pc : g[j] - pc * dum;
-      }
+    for (int j = 0; j < kGJCols; ++j) {
+      const double pc = pr[j] * pinv;
+      const double el = g[j] - pc * dum;
+      g[j] = (r == lr) ? pc : el;
     }
     if (wave == 0) {
       const double bp = lane_value(bv, lr) * pinv;
-      bv = (lane == lr) ? bp : bv - bp * dum;
+      bv = (r == lr) ? bp : bv - bp * dum;
     }
   }
   if (wave != 0) return;
@@ -909,8 +1015,13 @@ int enqueue_rounds(ddx_handle *h, Lane *const *lanes, int count, int nr) {
     DDX_TRY(rc);
     {
       TimedLaunch tl(h, DDX_K_JACOBI, s);
-      hipExtLaunchKernelGGL(k_jacobi, dim3(nbj, count), dim3(kJThreads), 0, s, tl.b, tl.e, 0, nn,
-                            h->nylm, diis_max, h->jblk.p, nbj, h->facl.p, sys);
+      // mixed precision keeps the residual history in FP32
+      if (l0.incr)
+        hipExtLaunchKernelGGL(k_jacobi<float>, dim3(nbj, count), dim3(kJThreads), 0, s, tl.b, tl.e,
+                              0, nn, h->nylm, diis_max, h->jblk.p, nbj, h->facl.p, sys);
+      else
+        hipExtLaunchKernelGGL(k_jacobi<double>, dim3(nbj, count), dim3(kJThreads), 0, s, tl.b,
+                              tl.e, 0, nn, h->nylm, diis_max, h->jblk.p, nbj, h->facl.p, sys);
     }
     DDX_HIP(hipGetLastError());
     if (diis_max > 0) {
@@ -1119,12 +1230,14 @@ int solve_run(ddx_handle *h, Lane *const *lanes, int count) {
 // kRefresh-th round apply the FP64 operator to the iterate; the rounds in
 // between apply the FP32 sweeps and products to the last increment
 // x_new - x and add the result to the FP64 image.  The Jacobi update, the
-// residual rhs - L x, the DIIS Gram row and bordered solve, the norms and
+// residual rhs - L x, the iterate history and extrapolation, the norms and
 // the convergence test are the FP64 path's (jacobi_diis.f90:102-183): the
 // FP32 rounding enters only through increments, which shrink with the
 // iteration, so the error carried in the image stays near 1e-7 of the
 // increments since the last refresh and the iteration counts stay those of
-// FP64 to within one or two.
+// FP64 to within one or two.  The residual history, which only sets the
+// extrapolation coefficients (its Gram matrix, accumulated in FP64, and the
+// bordered solve), is stored in FP32: half the bytes of the Gram-row pass.
 int solve_run_mixed(ddx_handle *h, Lane *const *lanes, int count) {
   DDX_TRY(make_tables32(h));
   const size_t nn = (size_t)h->nylm * h->n;
