@@ -36,9 +36,15 @@ SOURCES = ["ddx_api.hip", "k_scan.hip", "k_setup.hip", "k_gemm.hip", "k_sweep.hi
            "tables.cpp"]
 
 
+# per-source additions: the matrix products keep MFMA accumulators in VGPRs
+# (the AGPR form copies every loop-carried accumulator in and out per
+# iteration and doubles the register count)
+SOURCE_FLAGS = {"k_gemm.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC] + HIP_FLAGS + SOURCE_FLAGS.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj]
     if src.endswith(".cpp"):
         cmd = [HIPCC, "-O3", "-fPIC", "-std=c++17", "-x", "c++", "-I" + os.path.join(ROOT, "include"),
                "-c", os.path.join(CSRC, src), "-o", obj]

@@ -365,7 +365,8 @@ int create(ddx_handle **out, int n, int count, const int *nsph, const double *x,
       (rc = upload(h->Yw, yw.data(), yw.size(), s)) ||
       (rc = upload(h->facl, h->h_facl.data(), nylm, s)) ||
       (rc = upload(h->facs, h->h_facs.data(), nylm, s)) ||
-      (rc = upload(h->cf, cf.data(), cf.size(), s)) || (rc = sync(h)))
+      (rc = upload(h->cf, cf.data(), cf.size(), s)) || (rc = make_ptables(h)) ||
+      (rc = sync(h)))
     return fail(rc);
   if (count > 1) {
     std::vector<int> inst(n);

@@ -203,6 +203,13 @@ struct ddx_handle {
   ddx::DevArray<double> Y, Yw;    // basis (nylm, ngrid) and basis * w
   ddx::DevArray<float> Y32, Yw32; // their float copies (mixed precision)
   bool have_tables32 = false;
+  // zero-padded layouts for the products' coalesced operand loads: Yw with
+  // lm fastest in columns of ldm = 16 * row tiles over ldnp points (whole
+  // groups of 32), and Y lm-major, nylm4 rows (whole k-steps of 4) of ldnt
+  // points (whole tiles of 16); double and float
+  ddx::DevArray<double> YwP, YT;
+  ddx::DevArray<float> YwP32, YT32;
+  int ldm = 0, ldnp = 0, ldnt = 0, nylm4 = 0;
   ddx::DevArray<double> facl, cf; // (2l+1)/4pi and facs * 4pi/(2l+1)
   ddx::DevArray<double> facs;
 

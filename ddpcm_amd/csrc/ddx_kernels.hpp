@@ -31,6 +31,8 @@ int expand32(ddx_handle *h, hipStream_t s, const double *S, float *xi, bool wsca
              const DiisState *gate);
 // float copies of the harmonic tables (made once per handle)
 int make_tables32(ddx_handle *h);
+// the zero-padded layouts of Y and Yw (handle set-up, after Y and Yw)
+int make_ptables(ddx_handle *h);
 
 // k_sweep.hip — off-diagonal operators (matvec.f90:94-235), on the scratch
 // and stream of a lane.  L X reads X' = cf X from ln.xs: apply_lx makes it

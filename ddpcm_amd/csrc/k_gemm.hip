@@ -181,17 +181,121 @@ __global__ __launch_bounds__(256) void k_project32(int nylm, int ngrid, int nsph
   }
 }
 
+// The projection of large systems, both precisions: one workgroup per 64
+// spheres, wave w owning spheres 16 w .. 16 w + 15 over all points and all
+// NT row tiles.  Points go in groups of 32 (8 k-steps): the group's P tile
+// (64 spheres x 32 points) is staged in LDS, each quarter-wave loading one
+// sphere's 32 points as one contiguous run, and read back as the MFMA's B
+// operand; A comes from YwP (lm fastest, zero-padded), 16 consecutive lm per
+// quarter-wave.  The next group's tile is loaded while this one is used.
+// In FP64 a wave keeps one accumulator per class of k-steps modulo 4 and
+// adds them in class order at the end: the order of k_project's four waves,
+// so that both kernels give the same bits.
+template <typename T, int NT>
+__global__ __launch_bounds__(256) void k_project_s(int nylm, int ngrid, int ldm, int nsph,
+                                                   double alpha, const T *__restrict__ YwP,
+                                                   const T *__restrict__ P,
+                                                   double *__restrict__ y,
+                                                   const DiisState *__restrict__ gate) {
+  typedef T vt __attribute__((ext_vector_type(4)));
+  typedef T v2t __attribute__((ext_vector_type(2)));
+  constexpr int R = sizeof(T) == 8 ? 4 : 1;
+  constexpr int kLd = 65;   // LDS row: 64 spheres and one pad
+  if (gate && gate->done) return;
+  __shared__ T sP[2][32 * kLd];
+  const int lane = lane_id(), wave = threadIdx.x / kWave;
+  const int i = lane & 15, kq = lane >> 4;
+  const int s0 = blockIdx.x * 64;
+  const int ngrp = (ngrid + 31) / 32;
+  // staging: quarter-wave qd (0..15) loads spheres qd, qd + 16, qd + 32,
+  // qd + 48; lane l of the quarter points 2 l and 2 l + 1 of the group
+  const int qd = threadIdx.x >> 4, l = threadIdx.x & 15;
+  const T *Pq[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) Pq[r] = P + (size_t)min(s0 + qd + 16 * r, nsph - 1) * ngrid;
+  v2t pr[4];
+  auto load_tile = [&](int g) {
+    const int n = min(32 * g + 2 * l, ngrid - 2);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) pr[r] = *reinterpret_cast<const v2t *>(Pq[r] + n);
+  };
+  auto store_tile = [&](T *dst) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      dst[(2 * l) * kLd + qd + 16 * r] = pr[r].x;
+      dst[(2 * l + 1) * kLd + qd + 16 * r] = pr[r].y;
+    }
+  };
+  vt acc[R][NT];
+#pragma unroll
+  for (int c = 0; c < R; ++c)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[c][t] = vt{T(0), T(0), T(0), T(0)};
+  load_tile(0);
+  store_tile(sP[0]);
+  __syncthreads();
+  for (int g = 0; g < ngrp; ++g) {
+    const int cur = g & 1;
+    // the next group's tile (the last group loads its own again: the loop
+    // body stays one block, and that copy goes to the buffer no one reads)
+    load_tile(min(g + 1, ngrp - 1));
+    // A of the group's k-steps, KH at a time: YwP(16 t + i, 32 g + 4 k + kq)
+    constexpr int KH = sizeof(T) == 8 ? 4 : 8;
+    const T *Ag = YwP + i + (size_t)ldm * (32 * g + kq);
+    const T *B = sP[cur] + kq * kLd + 16 * wave + i;
+#pragma unroll
+    for (int k0 = 0; k0 < 8; k0 += KH) {
+      T a[KH][NT];
+#pragma unroll
+      for (int k = 0; k < KH; ++k)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) a[k][t] = Ag[16 * t + (size_t)ldm * 4 * (k0 + k)];
+#pragma unroll
+      for (int k = 0; k < KH; ++k) {
+        const T b = B[4 * (k0 + k) * kLd];
+        const int c = (k0 + k) % R;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          if constexpr (sizeof(T) == 8)
+            acc[c][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[k][t], b, acc[c][t], 0, 0, 0);
+          else
+            acc[c][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[k][t], b, acc[c][t], 0, 0, 0);
+        }
+      }
+    }
+    store_tile(sP[cur ^ 1]);
+    __syncthreads();
+  }
+  const int sph = s0 + 16 * wave + i;
+  if (sph >= nsph) return;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    vt sum = acc[0][t];
+#pragma unroll
+    for (int c = 1; c < R; ++c) sum = sum + acc[c][t];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      // D(i, j) in lane j + 16 (i % 4), element i / 4 (f64), or lane
+      // j + 16 (i / 4), element i % 4 (f32)
+      const int m = 16 * t + (sizeof(T) == 8 ? kq + 4 * q : 4 * kq + q);
+      if (m < nylm) y[m + (size_t)nylm * sph] = alpha * (double)sum[q];
+    }
+  }
+}
+
 // The expansion, both precisions: xi(n, i) = [w_n] sum_lm Y(lm, n) S(lm, i)
 // with T = double (v_mfma_f64_16x16x4f64) or float (v_mfma_f32_16x16x4f32,
 // S rounded as it is staged).  One workgroup per 16 J spheres: their rows
 // of S, contiguous in memory, are staged in LDS once; the four waves take
 // the point tiles of 16 (w, w + 4, ...), each an MFMA chain over lm with A
-// = the Y rows of the tile's points (a table that stays in the caches) and
-// B from LDS.  Each finished tile goes back through LDS so that 16 lanes
-// store one sphere's 16 points as one contiguous run.
+// = the Y rows of the tile's points from YT (lm-major, zero-padded: each
+// quarter-wave reads 16 consecutive points, and the A operands of 8
+// k-steps are loaded before their MFMAs) and B from LDS.  Each finished
+// tile goes back through LDS so that 16 lanes store one sphere's 16 points
+// as one contiguous run.
 template <typename T, int J>
-__global__ __launch_bounds__(256) void k_expand_t(int nylm, int ngrid, int nsph,
-                                                  const T *__restrict__ Y,
+__global__ __launch_bounds__(256) void k_expand_t(int nylm, int ngrid, int ldnt, int nsph,
+                                                  const T *__restrict__ YT,
                                                   const double *__restrict__ S,
                                                   const double4 *__restrict__ grid, int wscale,
                                                   T *__restrict__ xi,
@@ -216,16 +320,26 @@ __global__ __launch_bounds__(256) void k_expand_t(int nylm, int ngrid, int nsph,
     vt acc[J];
 #pragma unroll
     for (int j = 0; j < J; ++j) acc[j] = vt{T(0), T(0), T(0), T(0)};
-    for (int k0 = 0; k0 < nylm; k0 += 4) {
-      const int k = k0 + kq;
-      const T a = (na < ngrid && k < nylm) ? Y[k + (size_t)nylm * na] : T(0);
+    // YT(k, na): zero past nylm rows (nylm4 of them) and ngrid points
+    const T *Ak = YT + (size_t)kq * ldnt + na;
+    const int nk = (nylm + 3) / 4;
+    for (int c0 = 0; c0 < nk; c0 += 8) {
+      T a[8];
 #pragma unroll
-      for (int j = 0; j < J; ++j) {
-        const T b = k < nylm ? sS[(16 * j + row) * nylm + k] : T(0);
-        if constexpr (sizeof(T) == 8)
-          acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
-        else
-          acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[j], 0, 0, 0);
+      for (int u = 0; u < 8; ++u) a[u] = Ak[(size_t)4 * ldnt * min(c0 + u, nk - 1)];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (c0 + u < nk) {
+          const int k = 4 * (c0 + u) + kq;
+#pragma unroll
+          for (int j = 0; j < J; ++j) {
+            const T b = k < nylm ? sS[(16 * j + row) * nylm + k] : T(0);
+            if constexpr (sizeof(T) == 8)
+              acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b, acc[j], 0, 0, 0);
+            else
+              acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b, acc[j], 0, 0, 0);
+          }
+        }
       }
     }
     // D(i = point, j = sphere) sits in lane j + 16 (i % 4), element i / 4
@@ -271,12 +385,51 @@ inline int expand_split(int nx, int ngrid) {
   return std::max(1, std::min(want, (ntile + 3) / 4));
 }
 
+// YwP(m, n) = Yw(m, n) in columns of ldm (lm fastest) and YT(k, n) = Y(k, n)
+// in rows of ldnt (lm-major), zero in the padding; and their float copies
+__global__ void k_pad_tables(int nylm, int ngrid, int ldm, int ldnp, int nylm4, int ldnt,
+                             const double *__restrict__ Y, const double *__restrict__ Yw,
+                             double *__restrict__ YwP, double *__restrict__ YT,
+                             float *__restrict__ YwP32, float *__restrict__ YT32) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n1 = (size_t)ldm * ldnp, n2 = (size_t)nylm4 * ldnt;
+  if (k < n1) {
+    const int m = (int)(k % ldm), n = (int)(k / ldm);
+    const double v = (m < nylm && n < ngrid) ? Yw[m + (size_t)nylm * n] : 0.0;
+    YwP[k] = v;
+    YwP32[k] = (float)v;
+  } else if (k < n1 + n2) {
+    const size_t q = k - n1;
+    const int m = (int)(q / ldnt), n = (int)(q % ldnt);
+    const double v = (m < nylm && n < ngrid) ? Y[m + (size_t)nylm * n] : 0.0;
+    YT[q] = v;
+    YT32[q] = (float)v;
+  }
+}
+
 __global__ void k_to_f32(size_t n, const double *__restrict__ a, float *__restrict__ b) {
   size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) b[k] = (float)a[k];
 }
 
 }  // namespace
+
+int make_ptables(ddx_handle *h) {
+  h->ldm = (h->nylm + 15) / 16 * 16;
+  h->ldnp = (h->ngrid + 31) / 32 * 32;
+  h->nylm4 = (h->nylm + 3) / 4 * 4;
+  h->ldnt = (h->ngrid + 15) / 16 * 16;
+  const size_t n1 = (size_t)h->ldm * h->ldnp, n2 = (size_t)h->nylm4 * h->ldnt;
+  DDX_TRY(h->YwP.reserve(n1));
+  DDX_TRY(h->YwP32.reserve(n1));
+  DDX_TRY(h->YT.reserve(n2));
+  DDX_TRY(h->YT32.reserve(n2));
+  hipLaunchKernelGGL(k_pad_tables, dim3((n1 + n2 + 255) / 256), dim3(256), 0, h->stream, h->nylm,
+                     h->ngrid, h->ldm, h->ldnp, h->nylm4, h->ldnt, h->Y.p, h->Yw.p, h->YwP.p,
+                     h->YT.p, h->YwP32.p, h->YT32.p);
+  DDX_HIP(hipGetLastError());
+  return DDX_OK;
+}
 
 int make_tables32(ddx_handle *h) {
   if (h->have_tables32) return DDX_OK;
@@ -292,10 +445,43 @@ int make_tables32(ddx_handle *h) {
   return DDX_OK;
 }
 
+template <typename T, int NT>
+void launch_ps(ddx_handle *h, hipStream_t s, double alpha, const T *YwP, const T *P, double *y,
+               const DiisState *gate, hipEvent_t b, hipEvent_t e) {
+  hipExtLaunchKernelGGL((k_project_s<T, NT>), dim3((h->n + 63) / 64), dim3(256), 0, s, b, e, 0,
+                        h->nylm, h->ngrid, h->ldm, h->n, alpha, YwP, P, y, gate);
+}
+
+// the staged projection: systems of at least 1024 workgroups of 64 spheres,
+// one point range, an even point count, up to NTMAX row tiles
+template <typename T, int NTMAX>
+bool launch_project_s(ddx_handle *h, hipStream_t s, double alpha, const T *YwP, const T *P,
+                      double *y, const DiisState *gate, int ysplit, hipEvent_t b, hipEvent_t e) {
+  const int ntile = (h->nylm + 15) / 16;
+  if (ysplit != 1 || ntile > NTMAX || h->ngrid % 2 != 0 || h->ngrid < 2 ||
+      (h->n + 63) / 64 < 1024)
+    return false;
+  switch (ntile) {
+    case 1: launch_ps<T, 1>(h, s, alpha, YwP, P, y, gate, b, e); break;
+    case 2: launch_ps<T, 2>(h, s, alpha, YwP, P, y, gate, b, e); break;
+    case 3: launch_ps<T, 3>(h, s, alpha, YwP, P, y, gate, b, e); break;
+    case 4: launch_ps<T, 4>(h, s, alpha, YwP, P, y, gate, b, e); break;
+    case 5: if constexpr (NTMAX >= 5) launch_ps<T, 5>(h, s, alpha, YwP, P, y, gate, b, e); break;
+    case 6: if constexpr (NTMAX >= 6) launch_ps<T, 6>(h, s, alpha, YwP, P, y, gate, b, e); break;
+    case 7: if constexpr (NTMAX >= 7) launch_ps<T, 7>(h, s, alpha, YwP, P, y, gate, b, e); break;
+    default: if constexpr (NTMAX >= 8) launch_ps<T, 8>(h, s, alpha, YwP, P, y, gate, b, e); break;
+  }
+  return true;
+}
+
 int project32(ddx_handle *h, hipStream_t s, double alpha, const float *P, double *y,
               const DiisState *gate, int ysplit) {
   if (h->n <= 0) return DDX_OK;
   TimedLaunch tl(h, DDX_K_PROJECT, s);
+  if (launch_project_s<float, 8>(h, s, alpha, h->YwP32.p, P, y, gate, ysplit, tl.b, tl.e)) {
+    DDX_HIP(hipGetLastError());
+    return DDX_OK;
+  }
   hipExtLaunchKernelGGL(k_project32, dim3((h->n + 15) / 16, ysplit), dim3(256), 0, s, tl.b, tl.e,
                         0, h->nylm, h->ngrid, h->n, alpha, h->Yw32.p, P, y, gate);
   DDX_HIP(hipGetLastError());
@@ -311,8 +497,8 @@ int expand32(ddx_handle *h, hipStream_t s, const double *S, float *xi, bool wsca
   const int nx = (h->n + 16 * J - 1) / (16 * J);
   hipExtLaunchKernelGGL(J == 2 ? k_expand_t<float, 2> : k_expand_t<float, 1>,
                         dim3(nx, expand_split(nx, h->ngrid)), dim3(256), lds, s, tl.b, tl.e, 0,
-                        h->nylm, h->ngrid, h->n, h->Y32.p, S, h->grid.p, wscale ? 1 : 0, xi,
-                        gate);
+                        h->nylm, h->ngrid, h->ldnt, h->n, h->YT32.p, S, h->grid.p, wscale ? 1 : 0,
+                        xi, gate);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
@@ -321,6 +507,10 @@ int project(ddx_handle *h, hipStream_t s, double alpha, const double *P, double 
             const DiisState *gate, int ysplit) {
   if (h->n <= 0) return DDX_OK;
   TimedLaunch tl(h, DDX_K_PROJECT, s);
+  if (launch_project_s<double, 4>(h, s, alpha, h->YwP.p, P, y, gate, ysplit, tl.b, tl.e)) {
+    DDX_HIP(hipGetLastError());
+    return DDX_OK;
+  }
   hipExtLaunchKernelGGL(k_project, dim3((h->n + 15) / 16, ysplit), dim3(256), 0, s, tl.b, tl.e, 0,
                         h->nylm, h->ngrid, h->n, alpha, h->Yw.p, P, y, gate);
   DDX_HIP(hipGetLastError());
@@ -336,8 +526,8 @@ int expand(ddx_handle *h, hipStream_t s, const double *S, double *xi, bool wscal
   const int nx = (h->n + 16 * J - 1) / (16 * J);
   hipExtLaunchKernelGGL(J == 2 ? k_expand_t<double, 2> : k_expand_t<double, 1>,
                         dim3(nx, expand_split(nx, h->ngrid)), dim3(256), lds, s, tl.b, tl.e, 0,
-                        h->nylm, h->ngrid, h->n, h->Y.p, S, h->grid.p, wscale ? 1 : 0, xi,
-                        gate);
+                        h->nylm, h->ngrid, h->ldnt, h->n, h->YT.p, S, h->grid.p, wscale ? 1 : 0,
+                        xi, gate);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }

@@ -57,6 +57,25 @@ def test_fp32_operators(name):
     assert relmax(h.lx(v), g["probe_lx"]) < 1e-12
 
 
+def test_fp32_operators_large_shape():
+    """The FP32 products of large systems (the per-wave projection from
+    65 536 spheres, the paired-operand expansion from 32 768): L X and L* S
+    of the 512-molecule batch (113 185 spheres) against the FP64 operators,
+    to the same 2e-6 as the reference's probes above."""
+    from ddpcm_amd import inputs
+    from ddpcm_amd.ddx import DDCosmoBatch
+    b = DDCosmoBatch.from_inputs(inputs.batch_inputs())
+    assert b.nsph >= 65536
+    rng = np.random.default_rng(7)
+    v = rng.standard_normal((b.nsph, b.nylm))
+    lx64, lsx64 = b.lx(v), b.lstarx(v)
+    b.set_precision(MIXED)
+    lx, lsx = b.lx(v), b.lstarx(v)
+    print("large shape: L X %.2e, L* S %.2e" % (relmax(lx, lx64), relmax(lsx, lsx64)))
+    assert relmax(lx, lx64) < 2e-6 and relmax(lsx, lsx64) < 2e-6
+    assert relmax(lx, lx64) > 1e-12 and relmax(lsx, lsx64) > 1e-12
+
+
 _EXACT = {}
 
 
