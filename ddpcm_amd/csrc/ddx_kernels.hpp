@@ -29,6 +29,11 @@ int project32(ddx_handle *h, hipStream_t s, double alpha, const float *P, double
               const DiisState *gate, int ysplit);
 int expand32(ddx_handle *h, hipStream_t s, const double *S, float *xi, bool wscale,
              const DiisState *gate);
+// the FP32 projection of L X summing the sweep's point terms rw itself
+// (ptoff ranges, the gather's order) where the system is large enough for
+// it; *done says whether it ran (else gather P and project32)
+int project32_gathered(ddx_handle *h, hipStream_t s, double alpha, const float *rw, double *y,
+                       const DiisState *gate, int ysplit, bool *done);
 // float copies of the harmonic tables (made once per handle)
 int make_tables32(ddx_handle *h);
 // the zero-padded layouts of Y and Yw (handle set-up, after Y and Yw)

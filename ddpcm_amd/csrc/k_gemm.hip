@@ -181,6 +181,35 @@ __global__ __launch_bounds__(256) void k_project32(int nylm, int ngrid, int nsph
   }
 }
 
+// The A tile of a point group for the two projections below: columns
+// 32 g .. 32 g + 31 of YwP (ldm = 16 NT lm each), one contiguous block, staged
+// in LDS with 16-byte loads into rows of ldm + 16 (the pad puts the two
+// half-waves' MFMA operand reads on distinct banks).  Loads past the block
+// are clamped to its last vector, which is then written twice alike.
+template <typename T, int NT>
+struct ATile {
+  typedef T v16 __attribute__((ext_vector_type(16 / sizeof(T))));
+  static constexpr int kLdm = 16 * NT, kRow = kLdm + 16, kV = 16 / sizeof(T);
+  static constexpr int kCount = 32 * kLdm / kV;          // vectors per tile
+  static constexpr int kPer = (kCount + 255) / 256;      // per thread
+  v16 r[kPer];
+  __device__ __forceinline__ void load(const T *__restrict__ YwP, int g) {
+    const T *src = YwP + (size_t)kLdm * 32 * g;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int f = min((int)threadIdx.x + 256 * j, kCount - 1) * kV;
+      r[j] = *reinterpret_cast<const v16 *>(src + f);
+    }
+  }
+  __device__ __forceinline__ void store(T *dst) const {
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int f = min((int)threadIdx.x + 256 * j, kCount - 1) * kV;
+      *reinterpret_cast<v16 *>(dst + (f / kLdm) * kRow + f % kLdm) = r[j];
+    }
+  }
+};
+
 // The projection of large systems, both precisions: one workgroup per 64
 // spheres, wave w owning spheres 16 w .. 16 w + 15 over all points and all
 // NT row tiles.  Points go in groups of 32 (8 k-steps): the group's P tile
@@ -203,6 +232,9 @@ __global__ __launch_bounds__(256) void k_project_s(int nylm, int ngrid, int ldm,
   constexpr int kLd = 65;   // LDS row: 64 spheres and one pad
   if (gate && gate->done) return;
   __shared__ T sP[2][32 * kLd];
+  typedef ATile<T, NT> AT;
+  __shared__ T sA[2][32 * AT::kRow];
+  AT at;
   const int lane = lane_id(), wave = threadIdx.x / kWave;
   const int i = lane & 15, kq = lane >> 4;
   const int s0 = blockIdx.x * 64;
@@ -232,16 +264,19 @@ __global__ __launch_bounds__(256) void k_project_s(int nylm, int ngrid, int ldm,
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[c][t] = vt{T(0), T(0), T(0), T(0)};
   load_tile(0);
+  at.load(YwP, 0);
   store_tile(sP[0]);
+  at.store(sA[0]);
   __syncthreads();
   for (int g = 0; g < ngrp; ++g) {
     const int cur = g & 1;
-    // the next group's tile (the last group loads its own again: the loop
-    // body stays one block, and that copy goes to the buffer no one reads)
+    // the next group's tiles (the last group loads its own again: the loop
+    // body stays one block, and that copy goes to the buffers no one reads)
     load_tile(min(g + 1, ngrp - 1));
+    at.load(YwP, min(g + 1, ngrp - 1));
     // A of the group's k-steps, KH at a time: YwP(16 t + i, 32 g + 4 k + kq)
     constexpr int KH = sizeof(T) == 8 ? 4 : 8;
-    const T *Ag = YwP + i + (size_t)ldm * (32 * g + kq);
+    const T *Ag = sA[cur] + i + AT::kRow * kq;
     const T *B = sP[cur] + kq * kLd + 16 * wave + i;
 #pragma unroll
     for (int k0 = 0; k0 < 8; k0 += KH) {
@@ -249,7 +284,7 @@ __global__ __launch_bounds__(256) void k_project_s(int nylm, int ngrid, int ldm,
 #pragma unroll
       for (int k = 0; k < KH; ++k)
 #pragma unroll
-        for (int t = 0; t < NT; ++t) a[k][t] = Ag[16 * t + (size_t)ldm * 4 * (k0 + k)];
+        for (int t = 0; t < NT; ++t) a[k][t] = Ag[16 * t + AT::kRow * 4 * (k0 + k)];
 #pragma unroll
       for (int k = 0; k < KH; ++k) {
         const T b = B[4 * (k0 + k) * kLd];
@@ -264,6 +299,7 @@ __global__ __launch_bounds__(256) void k_project_s(int nylm, int ngrid, int ldm,
       }
     }
     store_tile(sP[cur ^ 1]);
+    at.store(sA[cur ^ 1]);
     __syncthreads();
   }
   const int sph = s0 + 16 * wave + i;
@@ -281,6 +317,135 @@ __global__ __launch_bounds__(256) void k_project_s(int nylm, int ngrid, int ldm,
       if (m < nylm) y[m + (size_t)nylm * sph] = alpha * (double)sum[q];
     }
   }
+}
+
+// The FP32 projection of large systems fused with the point gather of L X:
+// the P tile of a group (64 spheres x 32 points) is formed in LDS from the
+// sweep's terms, P(n, o) = sum of rw[ptoff[p] .. ptoff[p + 1]) in order
+// (k_gather_points), so that P never goes through memory.  Thread t sums
+// point t % 32 of the spheres t / 32 + 8 r (r = 0..7); the first kTerms
+// terms of each point are loaded unconditionally (indices clamped, values
+// masked), further ones in a loop.  The term ranges of the group after next
+// and the terms of the next group are loaded while this group's MFMAs run.
+constexpr int kTerms = 4;
+template <int NT>
+__global__ __launch_bounds__(256) void k_project_g(int nylm, int ngrid, int ldm, int nsph,
+                                                   int ntri, double alpha,
+                                                   const float *__restrict__ YwP,
+                                                   const int *__restrict__ ptoff,
+                                                   const float *__restrict__ rw,
+                                                   double *__restrict__ y,
+                                                   const DiisState *__restrict__ gate) {
+  constexpr int kLd = 65;
+  if (gate && gate->done) return;
+  __shared__ float sP[2][32 * kLd];
+  typedef ATile<float, NT> AT;
+  __shared__ float sA[2][32 * AT::kRow];
+  AT at;
+  const int lane = lane_id(), wave = threadIdx.x / kWave;
+  const int i = lane & 15, kq = lane >> 4;
+  const int s0 = blockIdx.x * 64;
+  const int ngrp = (ngrid + 31) / 32;
+  const int pl = threadIdx.x & 31, q8 = threadIdx.x >> 5;
+  const int last = max(ntri - 1, 0);
+  size_t prow[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) prow[r] = (size_t)min(s0 + q8 + 8 * r, nsph - 1) * ngrid;
+  int b[8], e[8];
+  float tv[8][kTerms];
+  auto load_ranges = [&](int g) {
+    const int n = min(32 * g + pl, ngrid - 1);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      b[r] = ptoff[prow[r] + n];
+      e[r] = ptoff[prow[r] + n + 1];
+    }
+  };
+  auto load_terms = [&]() {
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+      for (int u = 0; u < kTerms; ++u) tv[r][u] = rw[min(b[r] + u, last)];
+  };
+  auto store_tile = [&](int g, float *dst) {
+    const bool okn = 32 * g + pl < ngrid;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      float sum = 0.f;
+#pragma unroll
+      for (int u = 0; u < kTerms; ++u)
+        if (b[r] + u < e[r]) sum += tv[r][u];
+      for (int k = b[r] + kTerms; k < e[r]; ++k) sum += rw[k];
+      dst[pl * kLd + q8 + 8 * r] = okn ? sum : 0.f;
+    }
+  };
+  v4f acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+  load_ranges(0);
+  load_terms();
+  at.load(YwP, 0);
+  store_tile(0, sP[0]);
+  at.store(sA[0]);
+  load_ranges(min(1, ngrp - 1));
+  __syncthreads();
+  for (int g = 0; g < ngrp; ++g) {
+    const int cur = g & 1;
+    // terms of group g + 1 (ranges loaded in the previous step), then the
+    // ranges of group g + 2 once the terms' indices are taken
+    load_terms();
+    int bn[8], en[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      bn[r] = b[r];
+      en[r] = e[r];
+    }
+    load_ranges(min(g + 2, ngrp - 1));
+    at.load(YwP, min(g + 1, ngrp - 1));
+    const float *Ag = sA[cur] + i + AT::kRow * kq;
+    const float *B = sP[cur] + kq * kLd + 16 * wave + i;
+    float a[8][NT];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) a[k][t] = Ag[16 * t + AT::kRow * 4 * k];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float bb = B[4 * k * kLd];
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[k][t], bb, acc[t], 0, 0, 0);
+    }
+    // the next group's tile from its terms (the last step forms its own
+    // again, into the buffer no one reads)
+    {
+      int bs[8], es[8];
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        bs[r] = b[r];
+        es[r] = e[r];
+        b[r] = bn[r];
+        e[r] = en[r];
+      }
+      store_tile(min(g + 1, ngrp - 1), sP[cur ^ 1]);
+      at.store(sA[cur ^ 1]);
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        b[r] = bs[r];
+        e[r] = es[r];
+      }
+    }
+    __syncthreads();
+  }
+  const int sph = s0 + 16 * wave + i;
+  if (sph >= nsph) return;
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int m = 16 * t + 4 * kq + q;
+      if (m < nylm) y[m + (size_t)nylm * sph] = alpha * (double)acc[t][q];
+    }
 }
 
 // The expansion, both precisions: xi(n, i) = [w_n] sum_lm Y(lm, n) S(lm, i)
@@ -472,6 +637,35 @@ bool launch_project_s(ddx_handle *h, hipStream_t s, double alpha, const T *YwP, 
     default: if constexpr (NTMAX >= 8) launch_ps<T, 8>(h, s, alpha, YwP, P, y, gate, b, e); break;
   }
   return true;
+}
+
+template <int NT>
+void launch_pg(ddx_handle *h, hipStream_t s, double alpha, const float *rw, double *y,
+               const DiisState *gate, hipEvent_t b, hipEvent_t e) {
+  hipExtLaunchKernelGGL(k_project_g<NT>, dim3((h->n + 63) / 64), dim3(256), 0, s, b, e, 0,
+                        h->nylm, h->ngrid, h->ldm, h->n, h->ntri, alpha, h->YwP32.p, h->ptoff.p,
+                        rw, y, gate);
+}
+
+int project32_gathered(ddx_handle *h, hipStream_t s, double alpha, const float *rw, double *y,
+                       const DiisState *gate, int ysplit, bool *done) {
+  *done = false;
+  const int ntile = (h->nylm + 15) / 16;
+  if (h->n <= 0 || ysplit != 1 || ntile > 8 || (h->n + 63) / 64 < 1024) return DDX_OK;
+  TimedLaunch tl(h, DDX_K_PROJECT, s);
+  switch (ntile) {
+    case 1: launch_pg<1>(h, s, alpha, rw, y, gate, tl.b, tl.e); break;
+    case 2: launch_pg<2>(h, s, alpha, rw, y, gate, tl.b, tl.e); break;
+    case 3: launch_pg<3>(h, s, alpha, rw, y, gate, tl.b, tl.e); break;
+    case 4: launch_pg<4>(h, s, alpha, rw, y, gate, tl.b, tl.e); break;
+    case 5: launch_pg<5>(h, s, alpha, rw, y, gate, tl.b, tl.e); break;
+    case 6: launch_pg<6>(h, s, alpha, rw, y, gate, tl.b, tl.e); break;
+    case 7: launch_pg<7>(h, s, alpha, rw, y, gate, tl.b, tl.e); break;
+    default: launch_pg<8>(h, s, alpha, rw, y, gate, tl.b, tl.e); break;
+  }
+  DDX_HIP(hipGetLastError());
+  *done = true;
+  return DDX_OK;
 }
 
 int project32(ddx_handle *h, hipStream_t s, double alpha, const float *P, double *y,

@@ -684,6 +684,11 @@ int finish_lx(ddx_handle *h, Lane &ln, double *y, const DiisState *gate, int ysp
   hipStream_t s = ln.stream;
   R *rw = reinterpret_cast<R *>(ln.rw.p);
   R *P = reinterpret_cast<R *>(ln.P.p);
+  if constexpr (sizeof(R) == 4) {
+    bool done = false;
+    DDX_TRY(project32_gathered(h, s, -1.0, rw, y, gate, ysplit, &done));
+    if (done) return DDX_OK;
+  }
   hipLaunchKernelGGL(k_gather_points<R>, dim3((h->np + 255) / 256), dim3(256), 0, s, h->np,
                      h->ptoff.p, rw, P, gate);
   DDX_HIP(hipGetLastError());
