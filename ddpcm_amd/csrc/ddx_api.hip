@@ -211,9 +211,23 @@ int pipeline(ddx_handle *h) {
     return noconv(true);
   }
   if (h->igrad) {
-    DDX_TRY(forces_dd(h, h->phi.p, h->sigma.p, h->s.p, h->fx.p));
-    DDX_TRY(make_zeta(h, h->s.p, h->zeta.p, true));
-    DDX_TRY(field_forces(h, h->charge.p, h->zeta.p, h->fx.p, true));
+    // xi = basis^T S serves both branches that follow: zeta and its field
+    // at the nuclei (compute-bound all-pairs sums) on the second stream,
+    // beside the force kernels (memory-bound) on the first; joined before
+    // the contractions add into fx
+    DDX_TRY(h->xi.reserve(h->np));
+    DDX_TRY(expand(h, s, h->s.p, h->xi.p, false, nullptr));
+    DDX_HIP(hipEventRecord(h->fev[0], s));
+    DDX_HIP(hipStreamWaitEvent(h->stream2, h->fev[0], 0));
+    h->stream = h->stream2;
+    int rc = make_zeta(h, h->s.p, h->zeta.p, true);
+    if (rc == DDX_OK) rc = field_forces_fields(h, h->charge.p, h->zeta.p, true);
+    h->stream = s;
+    DDX_TRY(rc);
+    DDX_HIP(hipEventRecord(h->fev[1], h->stream2));
+    DDX_TRY(forces_dd(h, h->phi.p, h->sigma.p, h->s.p, h->fx.p, true));
+    DDX_HIP(hipStreamWaitEvent(s, h->fev[1], 0));
+    DDX_TRY(field_forces_contract(h, h->charge.p, h->zeta.p, h->fx.p));
     DDX_HIP(hipEventRecord(h->ev[5], s));
   }
   DDX_TRY(sync(h));
@@ -300,10 +314,15 @@ int create(ddx_handle **out, int n, int count, const int *nsph, const double *x,
     ddx_free(h);
     return code;
   };
-  if (hipStreamCreate(&h->stream) != hipSuccess) {
+  if (hipStreamCreate(&h->stream) != hipSuccess || hipStreamCreate(&h->stream2) != hipSuccess) {
     set_error("hipStreamCreate failed");
     return fail(DDX_ERR_HIP);
   }
+  for (auto &e : h->fev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      set_error("hipEventCreate failed");
+      return fail(DDX_ERR_HIP);
+    }
   for (Lane &ln : h->lane) ln.stream = h->stream;
   {
     int dev = 0;
@@ -433,9 +452,12 @@ int ddx_free(ddx_handle *h) {
   }
   for (auto &e : h->ev)
     if (e) (void)hipEventDestroy(e);
-  hipStream_t s = h->stream;
+  for (auto &e : h->fev)
+    if (e) (void)hipEventDestroy(e);
+  hipStream_t s = h->stream, s2 = h->stream2;
   delete h;   // device arrays release themselves
   if (s) (void)hipStreamDestroy(s);
+  if (s2) (void)hipStreamDestroy(s2);
   return DDX_OK;
 }
 

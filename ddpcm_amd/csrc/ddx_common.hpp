@@ -197,6 +197,11 @@ struct ddx_handle {
 
   hipStream_t stream = nullptr;
   hipEvent_t ev[10] = {};
+  // a second stream (default priority) for work that overlaps the first:
+  // the field of zeta at the nuclei beside the force kernels; fork / join
+  // events without timing
+  hipStream_t stream2 = nullptr;
+  hipEvent_t fev[2] = {};
 
   // tables
   ddx::DevArray<double4> grid;    // (x, y, z, w) per Lebedev point
@@ -284,6 +289,7 @@ struct ddx_handle {
 
   // problem data / results
   ddx::DevArray<double> phi, psi, sigma, s, xi, fx, zeta, ef, energy, fpart;
+  ddx::DevArray<double> fdd;        // owner and container sums of forces_dd
   ddx::DevArray<int> scan_tmp, misc;
 
   // initial guess of the solves (DDX_GUESS_*) and the learned guess's

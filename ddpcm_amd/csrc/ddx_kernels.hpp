@@ -121,14 +121,20 @@ int fmm_potential(ddx_handle *h, const double *q, double *phi, double *ef);
 int fmm_zeta_field(ddx_handle *h, const double *zeta, double *ez);
 
 // k_forces.hip — forces_dd, ddmkzeta and the field contractions
+// (xi_ready: h->xi already holds basis^T s)
 int forces_dd(ddx_handle *h, const double *d_phi, const double *d_sigma,
-              const double *d_s, double *d_fx);
+              const double *d_s, double *d_fx, bool xi_ready = false);
 // (xi_ready: h->xi already holds basis^T s, as forces_dd leaves it)
 int make_zeta(ddx_handle *h, const double *d_s, double *d_zeta, bool xi_ready = false);
 // solute_field: h->ef already holds the solute field at the cavity points
 // (from coulomb_potential)
 int field_forces(ddx_handle *h, const double *d_charge, const double *d_zeta,
                  double *d_fx, bool solute_field);
+// its two parts: the fields (on h->stream), then fx -= the contractions
+int field_forces_fields(ddx_handle *h, const double *d_charge, const double *d_zeta,
+                        bool solute_field);
+int field_forces_contract(ddx_handle *h, const double *d_charge, const double *d_zeta,
+                          double *d_fx);
 // size of h->ef that field_forces uses
 size_t field_buffer_size(const ddx_handle *h);
 

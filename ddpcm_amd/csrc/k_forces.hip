@@ -375,7 +375,7 @@ int force_triple(ddx_handle *h, const double *sigma, double4 *fb) {
 // forces_dd: xi = basis^T s, per-triple beta/alp, owner and container passes,
 // scale by 1/2 (eps-1)/eps (forces_dd.f90:160-161).
 int forces_dd(ddx_handle *h, const double *d_phi, const double *d_sigma, const double *d_s,
-              double *d_fx) {
+              double *d_fx, bool xi_ready) {
   if (!h->igrad) {
     set_error("forces need igrad = 1 at ddx_init (zi is built only then)");
     return DDX_ERR_STATE;
@@ -383,11 +383,11 @@ int forces_dd(ddx_handle *h, const double *d_phi, const double *d_sigma, const d
   hipStream_t s = h->stream;
   int n = h->n;
   DDX_TRY(h->xi.reserve(h->np));
-  DDX_TRY(expand(h, h->stream, d_s, h->xi.p, false, nullptr));
+  if (!xi_ready) DDX_TRY(expand(h, h->stream, d_s, h->xi.p, false, nullptr));
   DDX_TRY(h->r.reserve(4 * (size_t)h->ntri + 4));
   double4 *fb = reinterpret_cast<double4 *>(h->r.p);
-  DDX_TRY(h->fpart.reserve(6 * (size_t)n + 6));
-  double *fo = h->fpart.p, *fc = h->fpart.p + 3 * (size_t)n;
+  DDX_TRY(h->fdd.reserve(6 * (size_t)n + 6));
+  double *fo = h->fdd.p, *fc = h->fdd.p + 3 * (size_t)n;
   if (n > 0) {
     DDX_TRY(force_triple(h, d_sigma, fb));
     const bool one = force_waves(h) == 1;
@@ -425,8 +425,8 @@ size_t field_buffer_size(const ddx_handle *h) {
   return 3 * ((size_t)h->ncav + h->n) + 6 + 3 * (size_t)h->n + (size_t)h->ncav + 1;
 }
 
-int field_forces(ddx_handle *h, const double *d_charge, const double *d_zeta, double *d_fx,
-                 bool solute_field) {
+int field_forces_fields(ddx_handle *h, const double *d_charge, const double *d_zeta,
+                        bool solute_field) {
   hipStream_t s = h->stream;
   int n = h->n;
   size_t nef = 3 * ((size_t)h->ncav + n) + 6;
@@ -451,10 +451,23 @@ int field_forces(ddx_handle *h, const double *d_charge, const double *d_zeta, do
   else
     DDX_TRY(coulomb_field(h, h->ftile.p, h->nftile, h->fgrp.p, h->nfgrp, d_zeta, h->ccav.p, c3,
                           ez));
-  hipLaunchKernelGGL(k_field_contract, dim3(n), dim3(256), 0, s, h->ngrid, h->cavidx.p, d_zeta, ef,
-                     d_charge, ez, d_fx);
+  return DDX_OK;
+}
+
+int field_forces_contract(ddx_handle *h, const double *d_charge, const double *d_zeta,
+                          double *d_fx) {
+  const int n = h->n;
+  double *ef = h->ef.p, *ez = h->ef.p + 3 * (size_t)h->ncav + 3;
+  hipLaunchKernelGGL(k_field_contract, dim3(n), dim3(256), 0, h->stream, h->ngrid, h->cavidx.p,
+                     d_zeta, ef, d_charge, ez, d_fx);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
+}
+
+int field_forces(ddx_handle *h, const double *d_charge, const double *d_zeta, double *d_fx,
+                 bool solute_field) {
+  DDX_TRY(field_forces_fields(h, d_charge, d_zeta, solute_field));
+  return field_forces_contract(h, d_charge, d_zeta, d_fx);
 }
 
 }  // namespace ddx
