@@ -425,8 +425,9 @@ def batch_bench(args):
                           "one batched handle" % (h.count, int(np.diff(h.sph_off).min()),
                                                   int(np.diff(h.sph_off).max()), h.nsph,
                                                   d.lmax, h.ngrid, d.eps, d.eta, d.iconv),
-              "precision": "mixed (FP32 sweeps and harmonic products on the increments, "
-                           "FP64 residual, Jacobi/DIIS and reductions)",
+              "precision": "mixed (FP32 sweeps and harmonic products on the increments "
+                           "and FP32 DIIS residual history; FP64 residual, Jacobi update, "
+                           "iterate history, extrapolation and reductions)",
               "guess": "learned (trained on %d batches of other molecules)" % args.train_batches,
               "nsph": h.nsph, "ncav": h.ncav, "ntriples": h.stats().ntriples,
               "molecules": h.count}
@@ -434,7 +435,8 @@ def batch_bench(args):
         "metric": METRIC, "value": mixed["value"], "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": mixed["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 sweeps and products, f64 residual and DIIS", "data": "synthetic",
+        "dtype": "f32 sweeps, products and DIIS residuals; f64 residual, iterates",
+        "data": "synthetic",
         "config": config,
         "speedup_vs_fp64": mixed["value"] / fp64["value"],
         "stages_ms": mixed["stages_ms"], "iterations": mixed["iterations"],
@@ -596,7 +598,8 @@ def main():
         "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32 sweeps, f64 residual and DIIS",
+        "dtype": ("f64" if args.precision == "fp64"
+                  else "f32 sweeps, products and DIIS residuals; f64 residual, iterates"),
         "data": "synthetic",
         "config": config,
         "stages_ms": {"setup": st.ms_setup, "rhs": st.ms_rhs,

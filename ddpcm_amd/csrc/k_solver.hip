@@ -907,7 +907,9 @@ SysArgs sys_args(const ddx_handle *h, Lane &ln) {
   a.st = ln.diis.p;
   a.yacc = ln.incr ? ln.yacc.p : nullptr;
   a.acc = 0;
-  a.inc = ln.incr ? ln.inc.p : nullptr;
+  // the increment is read by the FP32 L* operator only: the L X sweep of
+  // an increment round reads xs = cf inc, which k_extrap writes as well
+  a.inc = (ln.incr && ln.star) ? ln.inc.p : nullptr;
   a.xs_inc = 0;
   return a;
 }
