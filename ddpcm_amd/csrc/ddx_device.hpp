@@ -62,6 +62,22 @@ __device__ __forceinline__ bool spheres_overlap(const double4 &a, const double4 
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
+// Whether spheres s0 .. s1 - 1 (s1 > s0) still have work: gate is the
+// solve's summary state (null: always); in a batch (inst: the instance of
+// each sphere) the ninst instance states precede it, and the range is done
+// when every instance it touches is.  Instances own contiguous sphere
+// ranges, so a range touches inst[s0] .. inst[s1 - 1].
+__device__ __forceinline__ bool spheres_active(const DiisState *gate, const int *inst, int ninst,
+                                               int s0, int s1) {
+  if (!gate) return true;
+  if (gate->done) return false;
+  if (!inst) return true;
+  const int q1 = inst[s1 - 1];
+  for (int q = inst[s0]; q <= q1; ++q)
+    if (!gate[q - ninst].done) return true;
+  return false;
+}
+
 // Value of this lane's partner at stage S of a six-stage cross-lane
 // combine, without LDS: the lane 32 apart and the lane 16 apart (gfx950
 // row swaps), then the mirror lane within 16 and within 8 and the quad

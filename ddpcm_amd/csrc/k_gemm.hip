@@ -225,12 +225,15 @@ __global__ __launch_bounds__(256) void k_project_s(int nylm, int ngrid, int ldm,
                                                    double alpha, const T *__restrict__ YwP,
                                                    const T *__restrict__ P,
                                                    double *__restrict__ y,
-                                                   const DiisState *__restrict__ gate) {
+                                                   const DiisState *__restrict__ gate,
+                                                   const int *__restrict__ inst, int ninst) {
   typedef T vt __attribute__((ext_vector_type(4)));
   typedef T v2t __attribute__((ext_vector_type(2)));
   constexpr int R = sizeof(T) == 8 ? 4 : 1;
   constexpr int kLd = 65;   // LDS row: 64 spheres and one pad
-  if (gate && gate->done) return;
+  // spheres whose molecules have all converged are not read again
+  if (!spheres_active(gate, inst, ninst, blockIdx.x * 64, min(blockIdx.x * 64 + 64, nsph)))
+    return;
   __shared__ T sP[2][32 * kLd];
   typedef ATile<T, NT> AT;
   __shared__ T sA[2][32 * AT::kRow];
@@ -335,9 +338,11 @@ __global__ __launch_bounds__(256) void k_project_g(int nylm, int ngrid, int ldm,
                                                    const int *__restrict__ ptoff,
                                                    const float *__restrict__ rw,
                                                    double *__restrict__ y,
-                                                   const DiisState *__restrict__ gate) {
+                                                   const DiisState *__restrict__ gate,
+                                                   const int *__restrict__ inst, int ninst) {
   constexpr int kLd = 65;
-  if (gate && gate->done) return;
+  if (!spheres_active(gate, inst, ninst, blockIdx.x * 64, min(blockIdx.x * 64 + 64, nsph)))
+    return;
   __shared__ float sP[2][32 * kLd];
   typedef ATile<float, NT> AT;
   __shared__ float sA[2][32 * AT::kRow];
@@ -464,14 +469,16 @@ __global__ __launch_bounds__(256) void k_expand_t(int nylm, int ngrid, int ldnt,
                                                   const double *__restrict__ S,
                                                   const double4 *__restrict__ grid, int wscale,
                                                   T *__restrict__ xi,
-                                                  const DiisState *__restrict__ gate) {
+                                                  const DiisState *__restrict__ gate,
+                                                  const int *__restrict__ inst, int ninst) {
   typedef T vt __attribute__((ext_vector_type(4)));
-  if (gate && gate->done) return;
   extern __shared__ double smem_d[];
   T *sS = reinterpret_cast<T *>(smem_d);                 // 16 J spheres x nylm
   T *tr = sS + ((16 * J * nylm + 1) & ~1);               // 4 waves x J x 16 x 17
   const int i0 = blockIdx.x * 16 * J;
   const int nsp = min(16 * J, nsph - i0);
+  // spheres whose molecules have all converged are not read again
+  if (!spheres_active(gate, inst, ninst, i0, i0 + nsp)) return;
   for (int k = threadIdx.x; k < 16 * J * nylm; k += 256)
     sS[k] = k < nsp * nylm ? (T)S[(size_t)nylm * i0 + k] : T(0);
   __syncthreads();
@@ -614,7 +621,8 @@ template <typename T, int NT>
 void launch_ps(ddx_handle *h, hipStream_t s, double alpha, const T *YwP, const T *P, double *y,
                const DiisState *gate, hipEvent_t b, hipEvent_t e) {
   hipExtLaunchKernelGGL((k_project_s<T, NT>), dim3((h->n + 63) / 64), dim3(256), 0, s, b, e, 0,
-                        h->nylm, h->ngrid, h->ldm, h->n, alpha, YwP, P, y, gate);
+                        h->nylm, h->ngrid, h->ldm, h->n, alpha, YwP, P, y, gate,
+                        h->ninst > 1 ? h->inst.p : nullptr, h->ninst);
 }
 
 // the staged projection: systems of at least 1024 workgroups of 64 spheres,
@@ -644,7 +652,7 @@ void launch_pg(ddx_handle *h, hipStream_t s, double alpha, const float *rw, doub
                const DiisState *gate, hipEvent_t b, hipEvent_t e) {
   hipExtLaunchKernelGGL(k_project_g<NT>, dim3((h->n + 63) / 64), dim3(256), 0, s, b, e, 0,
                         h->nylm, h->ngrid, h->ldm, h->n, h->ntri, alpha, h->YwP32.p, h->ptoff.p,
-                        rw, y, gate);
+                        rw, y, gate, h->ninst > 1 ? h->inst.p : nullptr, h->ninst);
 }
 
 int project32_gathered(ddx_handle *h, hipStream_t s, double alpha, const float *rw, double *y,
@@ -692,7 +700,7 @@ int expand32(ddx_handle *h, hipStream_t s, const double *S, float *xi, bool wsca
   hipExtLaunchKernelGGL(J == 2 ? k_expand_t<float, 2> : k_expand_t<float, 1>,
                         dim3(nx, expand_split(nx, h->ngrid)), dim3(256), lds, s, tl.b, tl.e, 0,
                         h->nylm, h->ngrid, h->ldnt, h->n, h->YT32.p, S, h->grid.p, wscale ? 1 : 0,
-                        xi, gate);
+                        xi, gate, h->ninst > 1 ? h->inst.p : nullptr, h->ninst);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }
@@ -721,7 +729,7 @@ int expand(ddx_handle *h, hipStream_t s, const double *S, double *xi, bool wscal
   hipExtLaunchKernelGGL(J == 2 ? k_expand_t<double, 2> : k_expand_t<double, 1>,
                         dim3(nx, expand_split(nx, h->ngrid)), dim3(256), lds, s, tl.b, tl.e, 0,
                         h->nylm, h->ngrid, h->ldnt, h->n, h->YT.p, S, h->grid.p, wscale ? 1 : 0,
-                        xi, gate);
+                        xi, gate, h->ninst > 1 ? h->inst.p : nullptr, h->ninst);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
 }

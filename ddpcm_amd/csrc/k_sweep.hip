@@ -450,11 +450,19 @@ __global__ __launch_bounds__(kWave, 2) void k_sweep(
 // with ui = 1, which calcv skips (ddcosmo.f90:1698), are exactly the points
 // without triples (ui < 1 iff some t < 1), so their P is zero here as well.
 template <typename R>
-__global__ __launch_bounds__(256) void k_gather_points(size_t np, const int *__restrict__ ptoff,
+__global__ __launch_bounds__(256) void k_gather_points(size_t np, int ngrid,
+                                                       const int *__restrict__ ptoff,
                                                        const R *__restrict__ rw,
                                                        R *__restrict__ P,
-                                                       const DiisState *__restrict__ gate) {
-  if (gate && gate->done) return;
+                                                       const DiisState *__restrict__ gate,
+                                                       const int *__restrict__ inst, int ninst) {
+  // points of spheres whose molecules have all converged are not read again
+  {
+    const size_t p0 = (size_t)blockIdx.x * blockDim.x;
+    const size_t p1 = min(np, p0 + blockDim.x);
+    if (!spheres_active(gate, inst, ninst, (int)(p0 / ngrid), (int)((p1 - 1) / ngrid) + 1))
+      return;
+  }
   size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const int b = ptoff[p], e = ptoff[p + 1];
@@ -690,7 +698,8 @@ int finish_lx(ddx_handle *h, Lane &ln, double *y, const DiisState *gate, int ysp
     if (done) return DDX_OK;
   }
   hipLaunchKernelGGL(k_gather_points<R>, dim3((h->np + 255) / 256), dim3(256), 0, s, h->np,
-                     h->ptoff.p, rw, P, gate);
+                     h->ngrid, h->ptoff.p, rw, P, gate, h->ninst > 1 ? h->inst.p : nullptr,
+                     h->ninst);
   DDX_HIP(hipGetLastError());
   // y(lm, o) = - sum_n Yw(lm, n) P(n, o)
   if constexpr (sizeof(R) == 4) return project32(h, s, -1.0, P, y, gate, ysplit);
