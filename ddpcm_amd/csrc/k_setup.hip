@@ -153,50 +153,77 @@ __global__ void k_nbr_rev(int nnl, const int *__restrict__ inl, const int *__res
   rev[q] = r;
 }
 
-// One thread per point p = i ngrid + n, over all points of all spheres (a
-// sphere's 302..5810 points would leave most of a last workgroup idle): fi,
-// ui, zi, the t < 1 mask, the triple count and the cavity flag.
+// fi, ui, zi of every point (ddcosmo.f90:441-498), the mask of the
+// neighbours that contain it and the cavity flags.  One wave per sphere i:
+// its neighbours' centres and radii are staged in LDS (lane k loads
+// neighbour k; the degree is at most 64), then the lanes take the sphere's
+// points n = lane, lane + 64, ..., each adding its terms over the
+// neighbours in list order.  The exact terms (square root, divisions) are
+// evaluated only for the neighbours that may contain the point, |v|^2 at
+// most r^2 (1 + 1e-12): every t < 1 passes that test by far more than any
+// rounding.
 __global__ __launch_bounds__(kSetupThreads) void k_points(
     const double4 *__restrict__ sph, const double4 *__restrict__ grid, int ngrid,
     const int *__restrict__ inl, const int *__restrict__ nl, double eta, int igrad,
     double *__restrict__ fi, double *__restrict__ ui, double *__restrict__ zi,
     unsigned long long *__restrict__ tmask, int *__restrict__ ptcnt,
-    int *__restrict__ cavflag, size_t np) {
-  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np) return;
-  const int i = (int)(p / (size_t)ngrid), n = (int)(p - (size_t)i * ngrid);
-  double4 si = sph[i], g = grid[n];
-  double f = 0.0, z0 = 0.0, z1 = 0.0, z2 = 0.0;
-  unsigned long long mask = 0;
-  int beg = inl[i], end = inl[i + 1];
-  for (int k = beg; k < end; ++k) {
-    double4 sj = sph[nl[k]];
-    double v[3];
-    point_vec(si, g, sj, v);
-    double vv = norm3(v);
-    double t = vv / sj.w;
-    if (t < 1.0) {
-      f += fsw(t, eta);
-      mask |= 1ull << (k - beg);
-      if (igrad && t > 1.0 - eta) {
-        double fac = dfsw(t, eta) / sj.w;
-        z0 += fac * v[0] / vv;
-        z1 += fac * v[1] / vv;
-        z2 += fac * v[2] / vv;
+    int *__restrict__ cavflag, int n) {
+  constexpr int kW = kSetupThreads / kWave;
+  __shared__ double4 nb[kW][kWave];
+  const int wave = threadIdx.x / kWave, lane = lane_id();
+  const int i = blockIdx.x * kW + wave;
+  if (i >= n) return;
+  const int beg = inl[i], deg = inl[i + 1] - beg;
+  if (lane < deg) nb[wave][lane] = sph[nl[beg + lane]];
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  const double4 si = sph[i];
+  for (int pn = lane; pn < ngrid; pn += kWave) {
+    const size_t p = (size_t)i * ngrid + pn;
+    const double4 g = grid[pn];
+    double f = 0.0, z0 = 0.0, z1 = 0.0, z2 = 0.0;
+    unsigned long long mask = 0, cand = 0;
+    // the neighbours that may contain the point (the cheap test), then the
+    // exact terms of those only, in neighbour order: the lanes of a wave
+    // would otherwise all wait through the square root and divisions of
+    // every neighbour that contains any one of them
+    for (int k = 0; k < deg; ++k) {
+      const double4 sj = nb[wave][k];
+      double v[3];
+      point_vec(si, g, sj, v);
+      const double d2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+      if (d2 <= sj.w * sj.w * (1.0 + 1e-12)) cand |= 1ull << k;
+    }
+    for (; cand; cand &= cand - 1) {
+      const int k = __ffsll((long long)cand) - 1;
+      const double4 sj = nb[wave][k];
+      double v[3];
+      point_vec(si, g, sj, v);
+      double vv = norm3(v);
+      double t = vv / sj.w;
+      if (t < 1.0) {
+        f += fsw(t, eta);
+        mask |= 1ull << k;
+        if (igrad && t > 1.0 - eta) {
+          double fac = dfsw(t, eta) / sj.w;
+          z0 += fac * v[0] / vv;
+          z1 += fac * v[1] / vv;
+          z2 += fac * v[2] / vv;
+        }
       }
     }
+    double u = (f <= 1.0) ? 1.0 - f : 0.0;
+    fi[p] = f;
+    ui[p] = u;
+    if (igrad) {
+      zi[3 * p + 0] = z0;
+      zi[3 * p + 1] = z1;
+      zi[3 * p + 2] = z2;
+    }
+    tmask[p] = mask;
+    ptcnt[p] = __popcll(mask);
+    cavflag[p] = u > 0.0 ? 1 : 0;
   }
-  double u = (f <= 1.0) ? 1.0 - f : 0.0;
-  fi[p] = f;
-  ui[p] = u;
-  if (igrad) {
-    zi[3 * p + 0] = z0;
-    zi[3 * p + 1] = z1;
-    zi[3 * p + 2] = z2;
-  }
-  tmask[p] = mask;
-  ptcnt[p] = __popcll(mask);
-  cavflag[p] = u > 0.0 ? 1 : 0;
 }
 
 __global__ __launch_bounds__(kSetupThreads) void k_ccav(const double4 *__restrict__ sph,
@@ -460,9 +487,10 @@ int run_setup(ddx_handle *h) {
   DDX_TRY(h->cavflag.reserve(np + 1));
   DDX_TRY(h->cavidx.reserve(np + 1));
   const dim3 pg((unsigned)((np + kSetupThreads - 1) / kSetupThreads));
-  hipLaunchKernelGGL(k_points, pg, dim3(kSetupThreads), 0, s, h->sph.p, h->grid.p, ng,
-                     h->inl.p, h->nl.p, h->eta, h->igrad, h->fi.p, h->ui.p,
-                     h->igrad ? h->zi.p : nullptr, h->tmask.p, h->ptcnt.p, h->cavflag.p, np);
+  hipLaunchKernelGGL(k_points, dim3((n + kSetupThreads / kWave - 1) / (kSetupThreads / kWave)),
+                     dim3(kSetupThreads), 0, s, h->sph.p, h->grid.p, ng, h->inl.p, h->nl.p, h->eta,
+                     h->igrad, h->fi.p, h->ui.p, h->igrad ? h->zi.p : nullptr, h->tmask.p,
+                     h->ptcnt.p, h->cavflag.p, n);
   DDX_HIP(hipGetLastError());
   DDX_TRY(scan_exclusive(h->ptcnt.p, h->ptoff.p, np, h->scan_tmp, s));
   DDX_TRY(scan_exclusive(h->cavflag.p, h->cavidx.p, np, h->scan_tmp, s));
