@@ -287,7 +287,8 @@ __global__ void k_zeta(size_t np, int ngrid, double scale, const double4 *__rest
   size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   double u = ui[p];
-  if (u > 0.0) zeta[cavidx[p]] = scale * (grid[p % ngrid].w * u * xi[p]);
+  // 32-bit point indices (run_setup checks the count)
+  if (u > 0.0) zeta[cavidx[p]] = scale * (grid[(unsigned)p % (unsigned)ngrid].w * u * xi[p]);
 }
 
 // fx_i -= sum over the cavity points of sphere i of zeta E  (main.f90:245-253)

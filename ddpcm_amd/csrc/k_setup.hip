@@ -234,7 +234,8 @@ __global__ __launch_bounds__(kSetupThreads) void k_ccav(const double4 *__restric
   const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   if (!cavflag[p]) return;
-  const int i = (int)(p / (size_t)ngrid), n = (int)(p - (size_t)i * ngrid);
+  // 32-bit point indices (run_setup checks the count)
+  const unsigned i = (unsigned)p / (unsigned)ngrid, n = (unsigned)p - i * (unsigned)ngrid;
   int ic = cavidx[p];
   double4 si = sph[i], g = grid[n];
   ccav[3 * (size_t)ic + 0] = si.x + si.w * g.x;
@@ -421,6 +422,11 @@ int run_setup(ddx_handle *h) {
   size_t np = (size_t)n * ng;
   h->np = np;
   drop_round_graphs(h);   // captured solver rounds hold the old buffers
+  if (np > 0x7fffffffull) {
+    // point indices are 32-bit in the per-point kernels
+    set_error("nsph * ngrid = %zu points exceed the 32-bit point index", np);
+    return DDX_ERR_CAPACITY;
+  }
 
   // neighbour list on the spatial hash
   unsigned T = 64;
