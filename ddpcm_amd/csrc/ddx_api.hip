@@ -369,14 +369,17 @@ int create(ddx_handle **out, int n, int count, const int *nsph, const double *x,
   std::vector<double4> g4(ng);
   for (int i = 0; i < ng; ++i)
     g4[i] = make_double4(h->h_grid[3 * i], h->h_grid[3 * i + 1], h->h_grid[3 * i + 2], h->h_w[i]);
-  // cf = facs / facl, then 1/(l+1) (hsnorm) and 1/facl = 4pi/(2l+1) (intmlp)
-  std::vector<double> cf(3 * (size_t)nylm);
+  // cf = facs / facl, then 1/(l+1) (hsnorm), 1/facl = 4pi/(2l+1) (intmlp),
+  // the scales K of the FP32 sweep's basis and cf K
+  std::vector<double> cf(5 * (size_t)nylm);
   for (int l = 0; l <= lmax; ++l)
     for (int m = -l; m <= l; ++m) {
       int k = l * l + l + m;
       cf[k] = h->h_facs[k] / h->h_facl[k];
       cf[nylm + k] = 1.0 / (1.0 + (double)l);
       cf[2 * nylm + k] = 1.0 / h->h_facl[k];
+      cf[3 * nylm + k] = ddx::sweep_basis_scale(l, m < 0 ? -m : m);
+      cf[4 * nylm + k] = cf[k] * cf[3 * nylm + k];
     }
   hipStream_t s = h->stream;
   if ((rc = upload(h->grid, g4.data(), ng, s)) ||

@@ -19,6 +19,10 @@ constexpr int kDiisMax = 25;       // ndiis of the reference (ddcosmo.f90:87)
 constexpr int kMaxIter = 200;      // n_iter of cosmo (cosmo.f90:81)
 constexpr int kSweepWorkersPerCU = 8;  // one-wave sweep workers per compute unit (at least)
 constexpr int kUnitsPerWorker = 12;    // units of 64 triples per sweep worker (default)
+// one-wave sweep workers per workgroup: one, since a workgroup keeps its
+// slot on the compute unit until its slowest wave ends and the workers'
+// unit ranges end at different times (four to a workgroup: 6% slower)
+constexpr int kSweepWaves = 1;
 
 void set_error(const char *fmt, ...);
 

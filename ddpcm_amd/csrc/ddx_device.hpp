@@ -62,6 +62,14 @@ __device__ __forceinline__ bool spheres_overlap(const double4 &a, const double4 
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
+// Orders the LDS accesses of one wave before and after this point: a wave's
+// LDS operations complete in issue order, so only the compiler must not move
+// them (for data one lane writes and others of the same wave read).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+}
+
 // Whether spheres s0 .. s1 - 1 (s1 > s0) still have work: gate is the
 // solve's summary state (null: always); in a batch (inst: the instance of
 // each sphere) the ninst instance states precede it, and the range is done

@@ -56,6 +56,8 @@ int apply_pair(ddx_handle *h, Lane &ld, double *yd, Lane &la, const double *xa, 
 int fmm_p2l(ddx_handle *h, int P, double rho, const double *q, const double *cfP, double *A);
 // xs = cf x
 int cf_scale(ddx_handle *h, hipStream_t s, const double *x, double *xs);
+// K(l, m), m >= 0: the scale of the FP32 sweep's basis entry (l, +-m)
+double sweep_basis_scale(int l, int m);
 // y = sum of the per-segment partial sums of L* S
 int seg_sum(ddx_handle *h, hipStream_t s, const double *yp, double *y, const DiisState *gate);
 

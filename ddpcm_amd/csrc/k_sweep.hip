@@ -75,12 +75,16 @@ __host__ __device__ constexpr int aidx(int lmax, int m0, int m1, int l, int m, b
   return mspan(lmax, m0, m) + (m == 0 ? l : 2 * (l - m) + (neg ? 1 : 0));
 }
 
+// X' of the container in the wave's LDS row
+template <typename R>
+using LdsPtr = __attribute__((address_space(3))) const R *;
+
 // One triple: returns sum_lm p~_lm trig_m Xc'_lm when DIR and adds
 // coef p~_lm trig_m to the accumulators of AM0 <= |m| < AM1 (trig_m =
 // cos(m phi) for m >= 0, sin(|m| phi) for m < 0).  Without DIR the
 // recurrences in m only step up to AM0 before the l recurrence starts.
 template <typename R, int LMAX, bool DIR, int AM0, int AM1>
-__device__ __forceinline__ R sweep_triple(R cthe, R sthe, R cphi, R sphi, R t, const R *Xc, R coef,
+__device__ __forceinline__ R sweep_triple(R cthe, R sthe, R cphi, R sphi, R t, LdsPtr<R> Xc, R coef,
                                           R *acc) {
   constexpr bool ADJ = AM0 < AM1;
   constexpr int MB = DIR ? 0 : AM0;          // first m with the l recurrence
@@ -101,12 +105,8 @@ __device__ __forceinline__ R sweep_triple(R cthe, R sthe, R cphi, R sphi, R t, c
       R a = 0.0, b = 0.0;
       // X' of this m is read from LDS only after the previous m's sum: the
       // compiler would otherwise keep all nylm values in registers
-      const R *Xm = Xc;
-      if (DIR) {
-        int off = 0;
-        asm volatile("" : "+v"(off) : "v"(s));
-        Xm = Xc + off;
-      }
+      LdsPtr<R> Xm = Xc;
+      if (DIR) asm volatile("" : "+v"(Xm) : "v"(s));
       auto visit = [&](int l, R p) {
         if (DIR) {
           a += p * Xm[lmi(l, m)];
@@ -140,6 +140,102 @@ __device__ __forceinline__ R sweep_triple(R cthe, R sthe, R cphi, R sphi, R t, c
       R cn, sn;
       if (m == 0) { cn = cphi; sn = sphi; }
       else { cn = R(2) * cphi * cm - cprev; sn = R(2) * cphi * sm - sprev; }
+      cprev = cm; sprev = sm; cm = cn; sm = sn;
+    }
+  }
+  return s;
+}
+
+// Ratio K(l, m) / K(l - 1, m) of the scales of the FP32 basis (l > m), and
+// the coefficient of t^2 q(l - 2, m) in its recurrence (l >= m + 2).
+__host__ __device__ constexpr double basis_ratio(int l, int m) {
+  return l == m + 1 ? 2.0 * m + 1.0 : (2.0 * l - 1.0) / (double)(l - m);
+}
+__host__ __device__ constexpr double basis_beta(int l, int m) {
+  return ((double)(l + m - 1) / (double)(l - m)) / (basis_ratio(l, m) * basis_ratio(l - 1, m));
+}
+
+// sweep_triple in FP32 on the scaled basis q_lm = p~_lm / K_lm, where
+//     K(m, m) = -(2m - 1) K(m - 1, m - 1),  K(m + 1, m) = (2m + 1) K(m, m),
+//     K(l, m) = (2l - 1) / (l - m) K(l - 1, m),
+// so that every recurrence step has one product less:
+//     q(m, m) = st q(m - 1, m - 1),  q(m + 1, m) = ct q(m, m),
+//     q(l, m) = ct q(l - 1, m) - beta(l, m) t^2 q(l - 2, m).
+// Xc holds K X' and the caller multiplies the adjoint sums by K.
+template <int LMAX, bool DIR, int AM0, int AM1>
+__device__ __forceinline__ float sweep_triple32(float cthe, float sthe, float cphi, float sphi,
+                                                float t, LdsPtr<float> Xc, float coef,
+                                                float *acc) {
+  constexpr bool ADJ = AM0 < AM1;
+  constexpr int MB = DIR ? 0 : AM0;          // first m with the l recurrence
+  constexpr int ME = DIR ? LMAX + 1 : AM1;   // one past the last m needed
+  const float ct = cthe * t, st = sthe * t, t2 = t * t, c2 = 2.0f * cphi;
+  float s = 0.0f;
+  float qmm = 1.0f;
+  float cm = 1.0f, sm = 0.0f, cprev = 0.0f, sprev = 0.0f;
+#pragma unroll
+  for (int m = 0; m < ME; ++m) {
+    if (m >= MB) {
+      const bool acc_m = ADJ && m >= AM0 && m < AM1;
+      float qc = 0.0f, qs = 0.0f;
+      if (acc_m) {
+        qc = (m == 0) ? coef : coef * cm;
+        qs = (m == 0) ? 0.0f : coef * sm;
+      }
+      float a = 0.0f, b = 0.0f;
+      LdsPtr<float> Xm = Xc;
+      if (DIR) asm volatile("" : "+v"(Xm) : "v"(s));
+      auto visit = [&](int l, float q) {
+        if (DIR) {
+          if (l == m) {
+            a = q * Xm[lmi(l, m)];
+            if (m > 0) b = q * Xm[lmi(l, -m)];
+          } else {
+            a = fmaf(q, Xm[lmi(l, m)], a);
+            if (m > 0) b = fmaf(q, Xm[lmi(l, -m)], b);
+          }
+        }
+        if (acc_m) {
+          float &ap = acc[aidx(LMAX, AM0, AM1, l, m, false)];
+          ap = fmaf(q, qc, ap);
+          if (m > 0) {
+            float &an = acc[aidx(LMAX, AM0, AM1, l, m, true)];
+            an = fmaf(q, qs, an);
+          }
+        }
+      };
+      float q0 = qmm;
+      visit(m, q0);
+      if (m < LMAX) {
+        float q1 = ct * q0;
+        visit(m + 1, q1);
+#pragma unroll
+        for (int l = m + 2; l <= LMAX; ++l) {
+          const float q2 = fmaf(-(float)basis_beta(l, m), t2 * q0, ct * q1);
+          visit(l, q2);
+          q0 = q1;
+          q1 = q2;
+        }
+      }
+      if (DIR) {
+        if (m == 0) {
+          s = a;
+        } else {
+          s = fmaf(cm, a, s);
+          s = fmaf(sm, b, s);
+        }
+      }
+    }
+    if (m + 1 < ME) {
+      qmm = st * qmm;
+      float cn, sn;
+      if (m == 0) {
+        cn = cphi;
+        sn = sphi;
+      } else {
+        cn = fmaf(c2, cm, -cprev);
+        sn = fmaf(c2, sm, -sprev);
+      }
       cprev = cm; sprev = sm; cm = cn; sm = sn;
     }
   }
@@ -180,7 +276,7 @@ __device__ __forceinline__ float dpp_partner(float v) {
 template <int S, int N, typename R>
 __device__ __forceinline__ void wave_reduce_split(R *v, int &base, int &len) {
   constexpr int H = (N + 1) / 2;
-  const bool side = (threadIdx.x >> (5 - S)) & 1;
+  const bool side = (lane_id() >> (5 - S)) & 1;
 #pragma unroll
   for (int k = 0; k < H; ++k) {
     const R x = v[k];
@@ -244,17 +340,20 @@ struct Geo<float> {
   typedef float2 G2;
 };
 
-// One pass over triples beg..end-1 of a container: rw[tslot] = W (R . X_c)
-// when DIR (X' in xsh), and when AM0 < AM1 the (l, +-m), AM0 <= m < AM1,
-// entries of the chunk's share of (L*S)_c, -cf * sum_t w_n xi(n, o) W p~
-// trig, into the segment row yp.
+// One pass over triples beg..end-1 (end > beg) of a container: rw[tslot] =
+// W (R . X_c) when DIR (X' in xsh), and when AM0 < AM1 the (l, +-m),
+// AM0 <= m < AM1, entries of the chunk's share of (L*S)_c, -cf * sum_t w_n
+// xi(n, o) W p~ trig, into the segment row yp.  The FP32 pass works on the
+// scaled basis of sweep_triple32: xsh holds K X' and the sums are scaled by
+// cf K (the table after cf, 1/(l+1), 1/facl and K in cf).
 template <typename R, int LMAX, bool DIR, int AM0, int AM1>
 __device__ __forceinline__ void sweep_pass(
     int beg, int end, const typename Geo<R>::G4 *__restrict__ tgeo,
     const typename Geo<R>::G2 *__restrict__ ttw,
-    const int *__restrict__ tslot, const int *__restrict__ tp, const R *xsh,
-    R *__restrict__ rw, const R *__restrict__ xiw, const double *__restrict__ cf,
+    const int *__restrict__ tslot, const int *__restrict__ tp, LdsPtr<R> xsh,
+    R *__restrict__ rw, int spare, const R *__restrict__ xiw, const double *__restrict__ cf,
     double *__restrict__ yp) {
+  constexpr bool F = sizeof(R) == 4;
   constexpr bool ADJ = AM0 < AM1;
   constexpr int NA = ADJ ? mspan(LMAX, AM0, AM1) : 1;
   R acc[NA];
@@ -264,54 +363,108 @@ __device__ __forceinline__ void sweep_pass(
   // triple k + 128 and xi of triple k + 64 load while triple k is evaluated
   typedef typename Geo<R>::G4 G4;
   typedef typename Geo<R>::G2 G2;
-  int k = beg + (int)threadIdx.x;
-  G4 z4;
-  z4.x = z4.y = z4.z = z4.w = R(0);
-  G2 z2;
-  z2.x = z2.y = R(0);
-  G4 g0 = z4, g1 = z4;
-  G2 w0 = z2, w1 = z2;
-  int s0 = 0, s1 = 0, p1 = 0;
-  R x0 = R(0);
-  if (k < end) {
-    g0 = tgeo[k];
-    w0 = ttw[k];
-    if (DIR) s0 = tslot[k];
-    if (ADJ) x0 = xiw[tp[k]];
-  }
-  if (k + kWave < end) {
-    g1 = tgeo[k + kWave];
-    w1 = ttw[k + kWave];
-    if (DIR) s1 = tslot[k + kWave];
-    if (ADJ) p1 = tp[k + kWave];
-  }
-  for (; k < end; k += kWave) {
-    const int k2 = k + 2 * kWave;
-    G4 g2 = z4;
-    G2 w2 = z2;
-    int s2 = 0, p2 = 0;
-    if (k2 < end) {
-      g2 = tgeo[k2];
-      w2 = ttw[k2];
-      if (DIR) s2 = tslot[k2];
-      if (ADJ) p2 = tp[k2];
+  int k = beg + lane_id();
+  if constexpr (F) {
+    // Indices past the chunk read its last triple instead, so that no load
+    // is conditional, and the loop is unrolled by three so that the three
+    // stages rotate by name (a register copy of a load in flight waits for
+    // it).  The rounds have no exit between their steps, which would let the
+    // compiler sink the loads to their uses: a lane past the chunk's end
+    // evaluates its clamped triple with weight 0 and stores into the spare
+    // slot rw[spare].
+    struct Stage {
+      G4 g;
+      G2 w;
+      int s, p;
+      R x;
+    };
+    const int last = end - 1;
+    auto fetch = [&](Stage &st, int j) {
+      j = min(j, last);
+      st.g = tgeo[j];
+      st.w = ttw[j];
+      if constexpr (DIR) st.s = tslot[j];
+      if constexpr (ADJ) st.p = tp[j];
+    };
+    Stage A, B, C;
+    fetch(A, k);
+    if constexpr (ADJ) A.x = xiw[A.p];
+    fetch(B, k + kWave);
+    // triple k from cur; triple k + 128 into fut, xi of triple k + 64 into nxt
+    auto step = [&](Stage &cur, Stage &nxt, Stage &fut) {
+      fetch(fut, k + 2 * kWave);
+      if constexpr (ADJ) nxt.x = xiw[nxt.p];
+      const bool live = k < end;
+      const R coef = ADJ ? (live ? cur.x * cur.w.y : R(0)) : R(0);
+      const R v = sweep_triple32<LMAX, DIR, AM0, AM1>(cur.g.x, cur.g.y, cur.g.z, cur.g.w,
+                                                      cur.w.x, xsh, coef, acc);
+      if constexpr (DIR) rw[live ? cur.s : spare] = cur.w.y * v;
+      k += kWave;
+    };
+    const int nit = (end - beg + kWave - 1) / kWave;   // the same in all lanes
+    int it = 0;
+    for (; it + 3 <= nit; it += 3) {
+      step(A, B, C);
+      step(B, C, A);
+      step(C, A, B);
     }
-    R x1 = R(0);
-    if (ADJ && k + kWave < end) x1 = xiw[p1];
-    const R coef = ADJ ? x0 * (R)w0.y : R(0);
-    const R v = sweep_triple<R, LMAX, DIR, AM0, AM1>((R)g0.x, (R)g0.y, (R)g0.z, (R)g0.w,
-                                                     (R)w0.x, xsh, coef, acc);
-    if (DIR) rw[s0] = (R)w0.y * v;
-    g0 = g1; g1 = g2;
-    w0 = w1; w1 = w2;
-    s0 = s1; s1 = s2;
-    x0 = x1; p1 = p2;
+    if (it < nit) {
+      step(A, B, C);
+      if (it + 1 < nit) step(B, C, A);
+    }
+  } else {
+    // FP64: loads under their bounds tests, stages copied down after each
+    // triple (the passes of lmax >= 8 need the registers a third stage takes)
+    G4 z4;
+    z4.x = z4.y = z4.z = z4.w = R(0);
+    G2 z2;
+    z2.x = z2.y = R(0);
+    G4 g0 = z4, g1 = z4;
+    G2 w0 = z2, w1 = z2;
+    int s0 = 0, s1 = 0, p1 = 0;
+    R x0 = R(0);
+    if (k < end) {
+      g0 = tgeo[k];
+      w0 = ttw[k];
+      if (DIR) s0 = tslot[k];
+      if (ADJ) x0 = xiw[tp[k]];
+    }
+    if (k + kWave < end) {
+      g1 = tgeo[k + kWave];
+      w1 = ttw[k + kWave];
+      if (DIR) s1 = tslot[k + kWave];
+      if (ADJ) p1 = tp[k + kWave];
+    }
+    for (; k < end; k += kWave) {
+      const int k2 = k + 2 * kWave;
+      G4 g2 = z4;
+      G2 w2 = z2;
+      int s2 = 0, p2 = 0;
+      if (k2 < end) {
+        g2 = tgeo[k2];
+        w2 = ttw[k2];
+        if (DIR) s2 = tslot[k2];
+        if (ADJ) p2 = tp[k2];
+      }
+      R x1 = R(0);
+      if (ADJ && k + kWave < end) x1 = xiw[p1];
+      const R coef = ADJ ? x0 * (R)w0.y : R(0);
+      const R v = sweep_triple<R, LMAX, DIR, AM0, AM1>((R)g0.x, (R)g0.y, (R)g0.z, (R)g0.w,
+                                                       (R)w0.x, xsh, coef, acc);
+      if (DIR) rw[s0] = (R)w0.y * v;
+      g0 = g1; g1 = g2;
+      w0 = w1; w1 = w2;
+      s0 = s1; s1 = s2;
+      x0 = x1; p1 = p2;
+    }
   }
   if constexpr (ADJ) {
     // sum of acc over the 64 lanes, entry by entry, spread over the lanes
     int base = 0, len = NA;
     wave_reduce_split<0, NA, R>(acc, base, len);
     constexpr int NF = reduced_len(NA, 6);
+    constexpr int NY = (LMAX + 1) * (LMAX + 1);
+    const double *cs = F ? cf + 4 * NY : cf;   // cf K or cf
 #pragma unroll
     for (int j = 0; j < NF; ++j) {
       if (j < len) {
@@ -327,7 +480,7 @@ __device__ __forceinline__ void sweep_pass(
             lm = lmi(l, (m > 0 && (r & 1)) ? -m : m);
           }
         }
-        yp[lm] = -(cf[lm] * (double)acc[j]);
+        yp[lm] = -(cs[lm] * (double)acc[j]);
       }
     }
   }
@@ -338,42 +491,50 @@ template <typename R, int LMAX, bool DIR, int B>
 __device__ __forceinline__ void sweep_blocks(
     int beg, int end, const typename Geo<R>::G4 *__restrict__ tgeo,
     const typename Geo<R>::G2 *__restrict__ ttw,
-    const int *__restrict__ tslot, const int *__restrict__ tp, const R *xsh,
-    R *__restrict__ rw, const R *__restrict__ xiw, const double *__restrict__ cf,
+    const int *__restrict__ tslot, const int *__restrict__ tp, LdsPtr<R> xsh,
+    R *__restrict__ rw, int spare, const R *__restrict__ xiw, const double *__restrict__ cf,
     double *__restrict__ yp) {
   constexpr bool F = sizeof(R) == 4;
   constexpr int M0 = mblock_begin(LMAX, DIR, B, F), M1 = mblock_begin(LMAX, DIR, B + 1, F);
-  sweep_pass<R, LMAX, DIR && B == 0, M0, M1>(beg, end, tgeo, ttw, tslot, tp, xsh, rw, xiw, cf,
-                                             yp);
+  sweep_pass<R, LMAX, DIR && B == 0, M0, M1>(beg, end, tgeo, ttw, tslot, tp, xsh, rw, spare, xiw,
+                                             cf, yp);
   if constexpr (M1 <= LMAX) sweep_blocks<R, LMAX, DIR, B + 1>(beg, end, tgeo, ttw, tslot, tp,
-                                                             xsh, rw, xiw, cf, yp);
+                                                             xsh, rw, spare, xiw, cf, yp);
 }
 
-// Triples beg..end-1 of container c: rw[tslot] = W (R . X_c) when DIR, and
-// when ADJ the chunk's share of (L*S)_c into the segment row yp.
+// Triples beg..end-1 of container c: rw[tslot] = W (R . X_c) when DIR
+// (rw[spare]: scratch), and when ADJ the chunk's share of (L*S)_c into the
+// segment row yp.
 template <typename R, int LMAX, bool DIR, bool ADJ>
 __device__ __forceinline__ void sweep_chunk(
     int c, int beg, int end, const typename Geo<R>::G4 *__restrict__ tgeo,
     const typename Geo<R>::G2 *__restrict__ ttw,
     const int *__restrict__ tslot, const int *__restrict__ tp, const double *__restrict__ Xs,
-    R *__restrict__ rw, const R *__restrict__ xiw, const double *__restrict__ cf,
+    R *__restrict__ rw, int spare, const R *__restrict__ xiw, const double *__restrict__ cf,
     double *__restrict__ yp) {
   constexpr int NY = (LMAX + 1) * (LMAX + 1);
-  // X'_c in LDS (rounded to R): every lane reads the same word (broadcast)
-  __shared__ R xsh[DIR ? NY : 1];
+  // X'_c in the wave's LDS row (rounded to R): every lane reads the same
+  // word (broadcast)
+  // (FP32: K X', the scaled basis of sweep_triple32)
+  __shared__ R xsh_all[kSweepWaves][DIR ? NY : 1];
+  R *xsh = xsh_all[__builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave))];
   if (DIR) {
-    __syncthreads();   // reads of the previous segment's X' are done
-    for (int k = threadIdx.x; k < NY; k += kWave) xsh[k] = (R)Xs[(size_t)c * NY + k];
-    __syncthreads();
+    wave_lds_sync();   // reads of the previous segment's X' are done
+    for (int k = lane_id(); k < NY; k += kWave) {
+      const double x = Xs[(size_t)c * NY + k];
+      xsh[k] = sizeof(R) == 4 ? (R)(x * cf[3 * NY + k]) : (R)x;
+    }
+    wave_lds_sync();
   }
+  const LdsPtr<R> xl = (LdsPtr<R>)xsh;
   if constexpr (ADJ)
-    sweep_blocks<R, LMAX, DIR, 0>(beg, end, tgeo, ttw, tslot, tp, xsh, rw, xiw, cf, yp);
+    sweep_blocks<R, LMAX, DIR, 0>(beg, end, tgeo, ttw, tslot, tp, xl, rw, spare, xiw, cf, yp);
   else
-    sweep_pass<R, LMAX, true, 0, 0>(beg, end, tgeo, ttw, tslot, tp, xsh, rw, xiw, cf, yp);
+    sweep_pass<R, LMAX, true, 0, 0>(beg, end, tgeo, ttw, tslot, tp, xl, rw, spare, xiw, cf, yp);
 }
 
-// One wave per worker (blockIdx.x of G): units floor(w U / G) ..
-// floor((w+1) U / G) - 1, container by container.  DIR: L X terms (rw,
+// One wave per worker w (kSweepWaves to a workgroup, w < G): units
+// floor(w U / G) .. floor((w+1) U / G) - 1, container by container.  DIR: L X terms (rw,
 // slot order); ADJ: one partial L* S per segment (yp).  A side whose solve
 // has ended (gate done) is skipped; in a batch (inst: the instance of each
 // sphere, gates: the summaries after ninst instance states) so is every
@@ -381,20 +542,25 @@ __device__ __forceinline__ void sweep_chunk(
 // (two waves per SIMD: the register budget that keeps a block of
 // accumulators and the two-deep load pipeline without spilling)
 template <typename R, int LMAX, bool DIR, bool ADJ>
-__global__ __launch_bounds__(kWave, 2) void k_sweep(
+__global__ __launch_bounds__(kWave * kSweepWaves, 2) void k_sweep(
     long long U, long long G, const int *__restrict__ uoff, const int *__restrict__ segoff,
     const int *__restrict__ inl, const int *__restrict__ pairoff, int n,
     const typename Geo<R>::G4 *__restrict__ tgeo,
     const typename Geo<R>::G2 *__restrict__ ttw,
     const int *__restrict__ tslot, const int *__restrict__ tp, const double *__restrict__ Xs,
-    R *__restrict__ rw, const R *__restrict__ xiw, const double *__restrict__ cf,
+    R *__restrict__ rw, int spare, const R *__restrict__ xiw, const double *__restrict__ cf,
     double *__restrict__ yp, const DiisState *__restrict__ gdir,
     const DiisState *__restrict__ gadj, const int *__restrict__ inst, int ninst) {
   const bool dir = DIR && !(gdir && gdir->done);
   const bool adj = ADJ && !(gadj && gadj->done);
   if (!dir && !adj) return;
   constexpr int NY = (LMAX + 1) * (LMAX + 1);
-  const long long w = blockIdx.x;
+  // (the wave index through readfirstlane: the compiler does not see that
+  // threadIdx.x / 64 is uniform across a wave, and every quantity below
+  // would otherwise live in vector registers and load per lane)
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+  const long long w = (long long)blockIdx.x * kSweepWaves + wave;
+  if (w >= G) return;
   const int u0 = (int)(w * U / G), u1 = (int)((w + 1) * U / G);
   // container of unit u0: the last c with uoff[c] <= u0
   int lo = 0, hi = n;   // uoff[lo] <= u0 < uoff[hi]
@@ -429,17 +595,17 @@ __global__ __launch_bounds__(kWave, 2) void k_sweep(
     }
     if constexpr (DIR && ADJ) {
       if (dc && ac)
-        sweep_chunk<R, LMAX, true, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
+        sweep_chunk<R, LMAX, true, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, spare, xiw, cf, ys);
       else if (dc)
-        sweep_chunk<R, LMAX, true, false>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
+        sweep_chunk<R, LMAX, true, false>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, spare, xiw, cf, ys);
       else if (ac)
-        sweep_chunk<R, LMAX, false, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
+        sweep_chunk<R, LMAX, false, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, spare, xiw, cf, ys);
     } else if constexpr (DIR) {
       if (dc)
-        sweep_chunk<R, LMAX, true, false>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
+        sweep_chunk<R, LMAX, true, false>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, spare, xiw, cf, ys);
     } else {
       if (ac)
-        sweep_chunk<R, LMAX, false, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, xiw, cf, ys);
+        sweep_chunk<R, LMAX, false, true>(c, beg, end, tgeo, ttw, tslot, tp, Xs, rw, spare, xiw, cf, ys);
     }
     u = ue;
     ++seg;
@@ -640,10 +806,12 @@ int launch_sweep(ddx_handle *h, hipStream_t s, const double *Xs, R *rw, const R 
     ttw = h->ttw.p;
   }
   TimedLaunch tl(h, (DIR && ADJ) ? DDX_K_SWEEP_PAIR : (DIR ? DDX_K_SWEEP_L : DDX_K_SWEEP_LSTAR), s);
-  hipExtLaunchKernelGGL((k_sweep<R, LMAX, DIR, ADJ>), dim3(h->nworker), dim3(kWave), 0, s, tl.b,
+  hipExtLaunchKernelGGL((k_sweep<R, LMAX, DIR, ADJ>),
+                        dim3((h->nworker + kSweepWaves - 1) / kSweepWaves),
+                        dim3(kWave * kSweepWaves), 0, s, tl.b,
                         tl.e, 0, (long long)h->nunit, (long long)h->nworker, h->uoff.p,
                         h->segoff.p, h->inl.p, h->pairoff.p, h->n, tgeo, ttw,
-                        h->tslot.p, h->tp.p, Xs, rw, xiw, h->cf.p, yp, gdir, gadj,
+                        h->tslot.p, h->tp.p, Xs, rw, (int)h->ntri, xiw, h->cf.p, yp, gdir, gadj,
                         h->ninst > 1 ? h->inst.p : nullptr, h->ninst);
   DDX_HIP(hipGetLastError());
   return DDX_OK;
@@ -754,6 +922,13 @@ int reserve_lstarx(ddx_handle *h, Lane &ln) {
 }
 
 }  // namespace
+
+double sweep_basis_scale(int l, int m) {
+  double K = 1.0;
+  for (int j = 1; j <= m; ++j) K *= -(2.0 * j - 1.0);
+  for (int j = m + 1; j <= l; ++j) K *= basis_ratio(j, m);
+  return K;
+}
 
 int cf_scale(ddx_handle *h, hipStream_t s, const double *x, double *xs) {
   size_t nn = (size_t)h->nylm * h->n;
