@@ -527,7 +527,8 @@ int run_setup(ddx_handle *h) {
   h->ncav = ncav;
   h->nunit = nunit;
   // workers: whole multiples of the R = 8 per CU one-wave workgroups that
-  // are resident at once (two waves per SIMD), as many as give about
+  // the FP64 sweeps keep resident (two waves per SIMD; the lmax 6 FP32 sweep
+  // holds four), as many as give about
   // kUnitsPerWorker units each (or as many per compute unit as ddx_autotune
   // chose).  Short ranges keep waves queued behind the resident ones, which
   // hides the memory latency of their triples; a partial last round of
