@@ -407,6 +407,21 @@ __global__ void k_seg_count(int n, const int *__restrict__ uoff, long long U, lo
   segcnt[c] = cnt;
 }
 
+// the largest neighbour count into *out (zeroed before): a block maximum,
+// then one atomic maximum per workgroup
+__global__ __launch_bounds__(256) void k_max_degree(int n, const int *__restrict__ deg,
+                                                    int *__restrict__ out) {
+  __shared__ int wm[256 / kWave];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  int m = wave_max_int(i < n ? deg[i] : 0);
+  if (lane_id() == 0) wm[threadIdx.x / kWave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 256 / kWave; ++w) m = max(m, wm[w]);
+    atomicMax(out, max(m, wm[0]));
+  }
+}
+
 // cavity offset of each instance: the cavity index of its first point
 __global__ void k_cav_offsets(int ninst, int ngrid, const int *__restrict__ soff,
                               const int *__restrict__ cavidx, int *__restrict__ out) {
@@ -455,13 +470,17 @@ int run_setup(ddx_handle *h) {
     DDX_HIP(hipGetLastError());
   }
   DDX_TRY(scan_exclusive(h->degcnt.p, h->inl.p, n, h->scan_tmp, s));
-  std::vector<int> deg(n);
-  int nnl = 0;
-  DDX_HIP(hipMemcpyAsync(deg.data(), h->degcnt.p, n * sizeof(int), hipMemcpyDeviceToHost, s));
+  // the largest degree into degcnt[n] (the scan reads the first n)
+  DDX_HIP(hipMemsetAsync(h->degcnt.p + n, 0, sizeof(int), s));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_max_degree, dim3((n + 255) / 256), dim3(256), 0, s, n, h->degcnt.p,
+                       h->degcnt.p + n);
+    DDX_HIP(hipGetLastError());
+  }
+  int nnl = 0, maxdeg = 0;
+  DDX_HIP(hipMemcpyAsync(&maxdeg, h->degcnt.p + n, sizeof(int), hipMemcpyDeviceToHost, s));
   DDX_HIP(hipMemcpyAsync(&nnl, h->inl.p + n, sizeof(int), hipMemcpyDeviceToHost, s));
   DDX_HIP(hipStreamSynchronize(s));
-  int maxdeg = 0;
-  for (int d : deg) maxdeg = d > maxdeg ? d : maxdeg;
   h->nnl = nnl;
   h->maxdeg = maxdeg;
   if (maxdeg > kMaxDegree) {
@@ -522,6 +541,21 @@ int run_setup(ddx_handle *h) {
   DDX_TRY(scan_exclusive(h->ucnt.p, h->uoff.p, n, h->scan_tmp, s));
   int nunit = 0;
   DDX_HIP(hipMemcpyAsync(&nunit, h->uoff.p + n, sizeof(int), hipMemcpyDeviceToHost, s));
+  // the cavity offset of each instance, read back with the counts
+  if (h->ninst > 1) {
+    if (h->maps_n != n) {
+      DDX_TRY(h->soff.reserve(h->ninst + 1));
+      DDX_HIP(hipMemcpyAsync(h->soff.p, h->h_soff.data(), (h->ninst + 1) * sizeof(int),
+                             hipMemcpyHostToDevice, s));
+    }
+    DDX_TRY(h->cavoff.reserve((size_t)h->ninst + 1));
+    hipLaunchKernelGGL(k_cav_offsets, dim3((h->ninst + 256) / 256), dim3(256), 0, s, h->ninst,
+                       ng, h->soff.p, h->cavidx.p, h->cavoff.p);
+    DDX_HIP(hipGetLastError());
+    h->h_cavoff.resize(h->ninst + 1);
+    DDX_HIP(hipMemcpyAsync(h->h_cavoff.data(), h->cavoff.p, (h->ninst + 1) * sizeof(int),
+                           hipMemcpyDeviceToHost, s));
+  }
   DDX_HIP(hipStreamSynchronize(s));
   h->ntri = ntri;
   h->ncav = ncav;
@@ -580,27 +614,19 @@ int run_setup(ddx_handle *h) {
   hipLaunchKernelGGL(k_ccav, pg, dim3(kSetupThreads), 0, s, h->sph.p, h->grid.p, ng,
                      h->cavflag.p, h->cavidx.p, h->ccav.p, np);
   DDX_HIP(hipGetLastError());
-  // instance offsets of the cavity, the solver's block maps (once per
-  // handle: they depend on the sphere counts only), Coulomb tiles
+  // the solver's block maps (once per handle: they depend on the sphere
+  // counts only; a batch's instance offsets went up with the counts above),
+  // Coulomb tiles
   if (h->maps_n != n) {
-    DDX_TRY(h->soff.reserve(h->ninst + 1));
-    DDX_HIP(hipMemcpyAsync(h->soff.p, h->h_soff.data(), (h->ninst + 1) * sizeof(int),
-                           hipMemcpyHostToDevice, s));
+    if (h->ninst == 1) {
+      DDX_TRY(h->soff.reserve(h->ninst + 1));
+      DDX_HIP(hipMemcpyAsync(h->soff.p, h->h_soff.data(), (h->ninst + 1) * sizeof(int),
+                             hipMemcpyHostToDevice, s));
+    }
     DDX_TRY(build_solver_maps(h));
     h->maps_n = n;
   }
-  if (h->ninst == 1) {
-    h->h_cavoff.assign({0, ncav});
-  } else {
-    DDX_TRY(h->cavoff.reserve((size_t)h->ninst + 1));
-    hipLaunchKernelGGL(k_cav_offsets, dim3((h->ninst + 256) / 256), dim3(256), 0, s, h->ninst,
-                       ng, h->soff.p, h->cavidx.p, h->cavoff.p);
-    DDX_HIP(hipGetLastError());
-    h->h_cavoff.resize(h->ninst + 1);
-    DDX_HIP(hipMemcpyAsync(h->h_cavoff.data(), h->cavoff.p, (h->ninst + 1) * sizeof(int),
-                           hipMemcpyDeviceToHost, s));
-    DDX_HIP(hipStreamSynchronize(s));
-  }
+  if (h->ninst == 1) h->h_cavoff.assign({0, ncav});
   DDX_TRY(build_coulomb_tiles(h));
   h->have_setup = true;
   h->stats.ntriples = ntri;
