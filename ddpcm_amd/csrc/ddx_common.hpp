@@ -246,8 +246,8 @@ struct ddx_handle {
   int ncu = 0;                      // compute units of the device
   ddx::DevArray<double4> tgeo;    // (cos theta, sin theta, cos phi, sin phi)
   ddx::DevArray<double2> ttw;     // (t, W)
-  ddx::DevArray<float4> tgeo32;   // float copies for the FP32 sweeps
-  ddx::DevArray<float2> ttw32;
+  ddx::DevArray<float4> tgeo32;   // the FP32 sweeps' records (geo32_of)
+  ddx::DevArray<float> ttw32;     //   and W
   bool have_geo32 = false;        // the copies match tgeo / ttw
   ddx::DevArray<int> tp;          // owner point index i*ngrid + n
   ddx::DevArray<int> pttri;       // per point: its triples in neighbour order

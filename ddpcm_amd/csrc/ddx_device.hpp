@@ -62,6 +62,13 @@ __device__ __forceinline__ bool spheres_overlap(const double4 &a, const double4 
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
+// The FP32 sweep's record of a triple from its FP64 one (cos theta, sin theta,
+// cos phi, sin phi) and (t, W): (t cos theta, t sin theta, cos phi, sin phi),
+// with W apart; t^2 is the sum of the first two squared.
+__device__ __forceinline__ float4 geo32_of(const double4 &g, const double2 &w) {
+  return make_float4((float)(g.x * w.x), (float)(g.y * w.x), (float)g.z, (float)g.w);
+}
+
 // Orders the LDS accesses of one wave before and after this point: a wave's
 // LDS operations complete in issue order, so only the compiler must not move
 // them (for data one lane writes and others of the same wave read).

@@ -353,13 +353,13 @@ __device__ __forceinline__ void triple_record(const double4 &so, const double4 &
 }
 
 // One wave per container c: the records of its triples, all lanes busy, and
-// with g32 / w32 their float copies for the FP32 sweeps.
+// with g32 / w32 the FP32 sweeps' records of them (geo32_of, and W).
 __global__ __launch_bounds__(kSetupThreads) void k_triple_geo(
     int n, int ngrid, double eta, const double4 *__restrict__ sph,
     const double4 *__restrict__ grid, const int *__restrict__ inl,
     const int *__restrict__ pairoff, const int *__restrict__ tp, const double *__restrict__ fi,
     double4 *__restrict__ tgeo, double2 *__restrict__ ttw, float4 *__restrict__ g32,
-    float2 *__restrict__ w32) {
+    float *__restrict__ w32) {
   const int c = blockIdx.x * (kSetupThreads / kWave) + threadIdx.x / kWave;
   if (c >= n) return;
   const double4 sc = sph[c];
@@ -373,8 +373,8 @@ __global__ __launch_bounds__(kSetupThreads) void k_triple_geo(
     tgeo[k] = a;
     ttw[k] = b;
     if (g32) {
-      g32[k] = make_float4((float)a.x, (float)a.y, (float)a.z, (float)a.w);
-      w32[k] = make_float2((float)b.x, (float)b.y);
+      g32[k] = geo32_of(a, b);
+      w32[k] = (float)b.y;
     }
   }
 }

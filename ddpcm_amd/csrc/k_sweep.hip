@@ -155,7 +155,8 @@ __host__ __device__ constexpr double basis_beta(int l, int m) {
   return ((double)(l + m - 1) / (double)(l - m)) / (basis_ratio(l, m) * basis_ratio(l - 1, m));
 }
 
-// sweep_triple in FP32 on the scaled basis q_lm = p~_lm / K_lm, where
+// sweep_triple in FP32 on the scaled basis q_lm = p~_lm / K_lm (ct = t cos
+// theta, st = t sin theta, t^2 = ct^2 + st^2), where
 //     K(m, m) = -(2m - 1) K(m - 1, m - 1),  K(m + 1, m) = (2m + 1) K(m, m),
 //     K(l, m) = (2l - 1) / (l - m) K(l - 1, m),
 // so that every recurrence step has one product less:
@@ -163,13 +164,12 @@ __host__ __device__ constexpr double basis_beta(int l, int m) {
 //     q(l, m) = ct q(l - 1, m) - beta(l, m) t^2 q(l - 2, m).
 // Xc holds K X' and the caller multiplies the adjoint sums by K.
 template <int LMAX, bool DIR, int AM0, int AM1>
-__device__ __forceinline__ float sweep_triple32(float cthe, float sthe, float cphi, float sphi,
-                                                float t, LdsPtr<float> Xc, float coef,
-                                                float *acc) {
+__device__ __forceinline__ float sweep_triple32(float ct, float st, float cphi, float sphi,
+                                                LdsPtr<float> Xc, float coef, float *acc) {
   constexpr bool ADJ = AM0 < AM1;
   constexpr int MB = DIR ? 0 : AM0;          // first m with the l recurrence
   constexpr int ME = DIR ? LMAX + 1 : AM1;   // one past the last m needed
-  const float ct = cthe * t, st = sthe * t, t2 = t * t, c2 = 2.0f * cphi;
+  const float t2 = fmaf(ct, ct, st * st), c2 = 2.0f * cphi;
   float s = 0.0f;
   float qmm = 1.0f;
   float cm = 1.0f, sm = 0.0f, cprev = 0.0f, sprev = 0.0f;
@@ -327,8 +327,9 @@ __device__ __forceinline__ void wave_reduce_split(R *v, int &base, int &len) {
 }
 
 // The per-triple records in the sweep's arithmetic: (cos theta, sin theta,
-// cos phi, sin phi) and (t, W) as double4 / double2 for the FP64 sweep, as
-// float4 / float2 copies (half the bytes) for the FP32 sweep.
+// cos phi, sin phi) and (t, W) as double4 / double2 for the FP64 sweep;
+// (t cos theta, t sin theta, cos phi, sin phi) as float4 and W as float for
+// the FP32 sweep (20 bytes: t enters only through those products and t^2).
 template <typename R>
 struct Geo {
   typedef double4 G4;
@@ -336,8 +337,8 @@ struct Geo {
 };
 template <>
 struct Geo<float> {
-  typedef float4 G4;
-  typedef float2 G2;
+  typedef float4 G4;   // (t cos theta, t sin theta, cos phi, sin phi)
+  typedef float G2;    // W
 };
 
 // One pass over triples beg..end-1 (end > beg) of a container: rw[tslot] =
@@ -374,7 +375,7 @@ __device__ __forceinline__ void sweep_pass(
     // slot rw[spare].
     struct Stage {
       G4 g;
-      G2 w;
+      G2 w;   // W
       int s, p;
       R x;
     };
@@ -395,10 +396,10 @@ __device__ __forceinline__ void sweep_pass(
       fetch(fut, k + 2 * kWave);
       if constexpr (ADJ) nxt.x = xiw[nxt.p];
       const bool live = k < end;
-      const R coef = ADJ ? (live ? cur.x * cur.w.y : R(0)) : R(0);
-      const R v = sweep_triple32<LMAX, DIR, AM0, AM1>(cur.g.x, cur.g.y, cur.g.z, cur.g.w,
-                                                      cur.w.x, xsh, coef, acc);
-      if constexpr (DIR) rw[live ? cur.s : spare] = cur.w.y * v;
+      const R coef = ADJ ? (live ? cur.x * cur.w : R(0)) : R(0);
+      const R v = sweep_triple32<LMAX, DIR, AM0, AM1>(cur.g.x, cur.g.y, cur.g.z, cur.g.w, xsh,
+                                                      coef, acc);
+      if constexpr (DIR) rw[live ? cur.s : spare] = cur.w * v;
       k += kWave;
     };
     const int nit = (end - beg + kWave - 1) / kWave;   // the same in all lanes
@@ -764,17 +765,17 @@ __global__ __launch_bounds__(kWave, 2) void k_p2l(int n, const double4 *__restri
   p2l_blocks<P, 0>(i, si, rc * rc, sb, se, sph, q, cf, A);
 }
 
-// float copies of the per-triple records for the FP32 sweeps, made once
-// per setup (on the first FP32 sweep after it)
+// the FP32 sweeps' per-triple records (geo32_of, and W), made once per
+// setup (on the first FP32 sweep after it)
 __global__ void k_geo_to_f32(long long n, const double4 *__restrict__ g,
                              const double2 *__restrict__ w, float4 *__restrict__ g32,
-                             float2 *__restrict__ w32) {
+                             float *__restrict__ w32) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const double4 a = g[k];
   const double2 b = w[k];
-  g32[k] = make_float4((float)a.x, (float)a.y, (float)a.z, (float)a.w);
-  w32[k] = make_float2((float)b.x, (float)b.y);
+  g32[k] = geo32_of(a, b);
+  w32[k] = (float)b.y;
 }
 
 int geometry32(ddx_handle *h, hipStream_t s) {
