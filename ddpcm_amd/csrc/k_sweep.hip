@@ -514,9 +514,8 @@ __device__ __forceinline__ void sweep_chunk(
     R *__restrict__ rw, int spare, const R *__restrict__ xiw, const double *__restrict__ cf,
     double *__restrict__ yp) {
   constexpr int NY = (LMAX + 1) * (LMAX + 1);
-  // X'_c in the wave's LDS row (rounded to R): every lane reads the same
-  // word (broadcast)
-  // (FP32: K X', the scaled basis of sweep_triple32)
+  // X'_c in the wave's LDS row (rounded to R; K X' in FP32, for the scaled
+  // basis of sweep_triple32): every lane reads the same word (broadcast)
   __shared__ R xsh_all[kSweepWaves][DIR ? NY : 1];
   R *xsh = xsh_all[__builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave))];
   if (DIR) {
@@ -535,8 +534,8 @@ __device__ __forceinline__ void sweep_chunk(
 }
 
 // One wave per worker w (kSweepWaves to a workgroup, w < G): units
-// floor(w U / G) .. floor((w+1) U / G) - 1, container by container.  DIR: L X terms (rw,
-// slot order); ADJ: one partial L* S per segment (yp).  A side whose solve
+// floor(w U / G) .. floor((w+1) U / G) - 1, container by container.  DIR:
+// L X terms (rw, slot order); ADJ: one partial L* S per segment (yp).  A side whose solve
 // has ended (gate done) is skipped; in a batch (inst: the instance of each
 // sphere, gates: the summaries after ninst instance states) so is every
 // container of an instance that has converged.
